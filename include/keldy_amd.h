@@ -1,0 +1,154 @@
+/*
+ * keldy_amd.h -- C-ABI of the MI355X Keldysh-sum integrand and QMC driver.
+ *
+ * Plain pointers and sizes only.  One kd_ctx owns one HIP device, one HIP
+ * stream and the device buffers of one integrator; it is not thread-safe
+ * (neither is keldy's integrator).  Every call returns KD_OK (0) or a
+ * negative KD_ERR_*; kd_last_error() gives the message.  Calls are
+ * synchronous at return unless stated otherwise.
+ *
+ * Reference interfaces replaced (macekmar/keldy, paths relative to its root):
+ *   kd_set_g0_table        g0_keldysh_contour_t (c++/keldy/impurity_oneband/model.hpp:238-268)
+ *                          + TRIQS gf<retime> evaluation of g0_lesser/g0_greater
+ *   kd_set_integrand       integrand_g_direct ctor (impurity_oneband/wick_direct.hpp:48-60)
+ *   kd_eval_batch          integrand_g_direct::operator() (impurity_oneband/wick_direct.cpp:43-158)
+ *   kd_set_warper_chain    warper_train_t (c++/keldy/warpers/warpers.hpp:41-137)
+ *   kd_eval_warped_batch   integrator::evaluate_warped_integrand (c++/keldy/integrator.hpp:68-79)
+ *   kd_set_sobol           integrator::reset_rng / keldy::sobol (integrator.hpp:97-112, qrng.hpp:49-114)
+ *   kd_run                 integrator::run (integrator.hpp:81-95)
+ *   kd_get_accum/kd_reset_accum  reduce_result / reduce_nr_points_* (integrator.hpp:123-131)
+ */
+#ifndef KELDY_AMD_H
+#define KELDY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KD_OK 0
+#define KD_ERR_ARG (-1)      /* invalid argument (maps to RuntimeError in the host API) */
+#define KD_ERR_HIP (-2)      /* HIP runtime failure */
+#define KD_ERR_STATE (-3)    /* required set-up call missing */
+#define KD_ERR_RANGE (-4)    /* Sobol generator exhausted (more than 2^31 points) */
+#define KD_ERR_NODEVICE (-5) /* no HIP device */
+
+#define KD_MAX_ORDER 12
+
+/* LU precision of the Keldysh sum; the sum itself and the QMC accumulation are
+ * always FP64. */
+#define KD_PREC_FP64 0
+#define KD_PREC_FP32_LU 1
+
+typedef struct kd_ctx kd_ctx;
+
+/* gf_index_t (model.hpp:61-76) */
+typedef struct {
+  double time;
+  int32_t k_idx;   /* 0 forward, 1 backward */
+  int32_t spin;    /* 0 up, 1 down */
+  int32_t orbital; /* 0 dot, 1 lead (needs a 2-orbital table) */
+  int32_t pad;
+} kd_gf_index;
+
+/* One-dimensional inverse-CDF maps (warpers/product_1d_simple.hpp, product_1d.hpp) */
+#define KD_MAP_UNIFORM 0     /* f = 1/u_max */
+#define KD_MAP_EXPONENTIAL 1 /* make_product_1d_simple_exponential (product_1d_simple.hpp:560-576) */
+#define KD_MAP_INVERSE 2     /* make_product_1d_simple_inverse (578-592) */
+#define KD_MAP_INVERSE_SQ 3  /* make_product_1d_simple_inverse_square (594-608) */
+#define KD_MAP_INVERSE_POW 4 /* make_product_1d_simple_inverse_power (610-630) */
+#define KD_MAP_NEAREST 5     /* warper_product_1d_simple_interp_nearest_t (174-379) */
+#define KD_MAP_HYBRID 6      /* warper_product_1d_simple_interp_hybrid_t (381-554), f = w^p/(w+u)^p */
+
+typedef struct {
+  int32_t kind;
+  int32_t power;
+  double u_max, w;
+  double norm;   /* analytic maps: the maker's normalisation; HYBRID: f1 normalisation */
+  int32_t n_pts; /* tabulated maps: knots of the forward spline */
+  int32_t n_inv; /* tabulated maps: knots of the inverse spline (cut_index + 1) */
+  int64_t t_off; /* offsets (in doubles) into the chain's table buffer */
+  int64_t f_off; /* NEAREST: normalised f1 at knots */
+  int64_t F_off; /* integrated, normalised, negated CDF knots in [-1, 0] */
+  int64_t c_off; /* HYBRID: Steffen coefficients of F^-1, (n_inv-1) x 4 */
+} kd_map1d;
+
+#define KD_WARP_IDENTITY 0 /* warper_identity_t (warpers/identity.hpp:30-49) */
+#define KD_WARP_PLASMA 1   /* warper_plasma_uv_t (warpers/plasma_uv.hpp:35-81) */
+#define KD_WARP_PRODUCT 2  /* product of 1D maps: n_maps == 1 (same map on every axis) or == order */
+#define KD_WARP_FLOW 3     /* learned affine-coupling flow (no reference counterpart) */
+
+typedef struct {
+  int32_t kind;
+  int32_t n_maps;
+  double t_max;
+  int64_t map_offset; /* PRODUCT: index of the first kd_map1d of this warper */
+} kd_warper_desc;
+
+typedef struct {
+  double re, im;         /* sum of warped integrand values */
+  uint64_t n_points;     /* points run */
+  uint64_t n_in_domain;  /* points inside the integration domain */
+  uint64_t n_fallback;   /* Keldysh configurations recomputed by the pivoted FP64 LU */
+} kd_accum;
+
+int kd_create(kd_ctx **out, int device);
+int kd_destroy(kd_ctx *ctx);
+const char *kd_last_error(const kd_ctx *ctx);
+int kd_device_count(int *count);
+
+/* g0 tables on the time mesh t_k = -time_max + k*dt, dt = 2 time_max/(n_pts-1).
+ * lesser/greater: [n_orb][n_orb][n_pts] complex as interleaved (re, im) doubles.
+ * alpha: the alpha shift of equal-time internal contractions (model.hpp:263-264). */
+int kd_set_g0_table(kd_ctx *ctx, const double *lesser, const double *greater, int32_t n_orb,
+                    int32_t n_pts, double time_max, double alpha);
+
+/* External points A, B and cutoff of integrand_g_direct; order = number of vertices. */
+int kd_set_integrand(kd_ctx *ctx, int32_t order, const kd_gf_index *A, const kd_gf_index *B,
+                     double cutoff);
+int kd_set_precision(kd_ctx *ctx, int32_t precision);
+
+/* Integrand at n_points vertex-time vectors u (row-major n_points x order, host memory).
+ * out: n_points x (re, im); in_domain: n_points (may be NULL). */
+int kd_eval_batch(kd_ctx *ctx, const double *u, int64_t n_points, int32_t keep_u_hypercube,
+                  double *out, int32_t *in_domain);
+/* Same with device pointers (inputs already in HBM); asynchronous on the ctx stream. */
+int kd_eval_batch_device(kd_ctx *ctx, const double *d_u, int64_t n_points, int32_t keep_u_hypercube,
+                         double *d_out, int32_t *d_in_domain);
+
+/* Warper chain, in the order warpers were added (applied last-to-first).
+ * maps: all kd_map1d of PRODUCT warpers; tables: the doubles they index. */
+int kd_set_warper_chain(kd_ctx *ctx, const kd_warper_desc *warpers, int32_t n_warpers,
+                        const kd_map1d *maps, int32_t n_maps, const double *tables,
+                        int64_t n_tables);
+/* Warped integrand (times Jacobian) at n_points hypercube points l (host memory). */
+int kd_eval_warped_batch(kd_ctx *ctx, const double *l, int64_t n_points, double *out,
+                         int32_t *in_domain, double *jacobian);
+
+/* Sobol generator: Csr = bit-reversed (scrambled) generating matrices [dim][31],
+ * shfr = bit-reversed digital shift [dim]; state k (next point is k+1). */
+int kd_set_sobol(kd_ctx *ctx, int32_t dim, const uint32_t *Csr, const uint32_t *shfr, uint32_t k);
+int kd_get_sobol_state(const kd_ctx *ctx, uint32_t *k);
+
+/* Run n_steps Sobol points through warper chain and integrand, accumulating on
+ * the device; returns the accumulator totals since the last kd_reset_accum. */
+int kd_run(kd_ctx *ctx, uint64_t n_steps, kd_accum *out);
+int kd_get_accum(kd_ctx *ctx, kd_accum *out);
+int kd_reset_accum(kd_ctx *ctx);
+
+/* Per-point samples of the warped integrand at the next n_steps Sobol points
+ * (advances the generator): l (n x order), value (n x 2). Device pointers. */
+int kd_sample_device(kd_ctx *ctx, uint64_t n_steps, double *d_l, double *d_value);
+
+/* Milliseconds of the last kd_run's integrand kernel(s), timed with HIP events
+ * on the ctx stream, and the number of kernel launches it made. */
+int kd_last_kernel_ms(const kd_ctx *ctx, double *ms, int32_t *launches);
+/* Executed FP operations per point of the last configured order/precision
+ * (the algorithm's own count, used for roofline figures). */
+int kd_flops_per_point(const kd_ctx *ctx, double *flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+"""ctypes binding of the C-ABI in include/keldy_amd.h (libkeldy_amd.so).
+
+This is the same boundary a ctypes/cgo/JNI caller of the reference path would
+bind (see the header for the reference interface each entry point replaces).
+The library is loaded from keldy_amd/lib/; importing this module on a machine
+without the built library raises at load time -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _build
+
+KD_OK = 0
+KD_ERR_ARG, KD_ERR_HIP, KD_ERR_STATE, KD_ERR_RANGE, KD_ERR_NODEVICE = -1, -2, -3, -4, -5
+KD_MAX_ORDER = 12
+KD_PREC_FP64, KD_PREC_FP32_LU = 0, 1
+
+KD_MAP_UNIFORM, KD_MAP_EXPONENTIAL, KD_MAP_INVERSE, KD_MAP_INVERSE_SQ = 0, 1, 2, 3
+KD_MAP_INVERSE_POW, KD_MAP_NEAREST, KD_MAP_HYBRID = 4, 5, 6
+KD_WARP_IDENTITY, KD_WARP_PLASMA, KD_WARP_PRODUCT, KD_WARP_FLOW = 0, 1, 2, 3
+
+
+class GfIndex(C.Structure):
+    _fields_ = [("time", C.c_double), ("k_idx", C.c_int32), ("spin", C.c_int32),
+                ("orbital", C.c_int32), ("pad", C.c_int32)]
+
+
+class Map1d(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("power", C.c_int32), ("u_max", C.c_double),
+                ("w", C.c_double), ("norm", C.c_double), ("n_pts", C.c_int32),
+                ("n_inv", C.c_int32), ("t_off", C.c_int64), ("f_off", C.c_int64),
+                ("F_off", C.c_int64), ("c_off", C.c_int64)]
+
+
+class WarperDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n_maps", C.c_int32), ("t_max", C.c_double),
+                ("map_offset", C.c_int64)]
+
+
+class Accum(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double), ("n_points", C.c_uint64),
+                ("n_in_domain", C.c_uint64), ("n_fallback", C.c_uint64)]
+
+
+# every symbol the header declares
+SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_set_g0_table",
+           "kd_set_integrand", "kd_set_precision", "kd_eval_batch", "kd_eval_batch_device",
+           "kd_set_warper_chain", "kd_eval_warped_batch", "kd_set_sobol", "kd_get_sobol_state",
+           "kd_run", "kd_get_accum", "kd_reset_accum", "kd_sample_device", "kd_last_kernel_ms",
+           "kd_flops_per_point"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = _build.lib_path()
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is not built: run __graft_entry__.build() (no CPU fallback)")
+        L = C.CDLL(path)
+        dp, ip, up = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_uint32)
+        vp = C.c_void_p
+        L.kd_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.kd_destroy.argtypes = [vp]
+        L.kd_last_error.argtypes = [vp]
+        L.kd_last_error.restype = C.c_char_p
+        L.kd_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.kd_set_g0_table.argtypes = [vp, dp, dp, C.c_int32, C.c_int32, C.c_double, C.c_double]
+        L.kd_set_integrand.argtypes = [vp, C.c_int32, C.POINTER(GfIndex), C.POINTER(GfIndex), C.c_double]
+        L.kd_set_precision.argtypes = [vp, C.c_int32]
+        L.kd_eval_batch.argtypes = [vp, dp, C.c_int64, C.c_int32, dp, ip]
+        L.kd_eval_batch_device.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
+        L.kd_set_warper_chain.argtypes = [vp, C.POINTER(WarperDesc), C.c_int32, C.POINTER(Map1d),
+                                          C.c_int32, dp, C.c_int64]
+        L.kd_eval_warped_batch.argtypes = [vp, dp, C.c_int64, dp, ip, dp]
+        L.kd_set_sobol.argtypes = [vp, C.c_int32, up, up, C.c_uint32]
+        L.kd_get_sobol_state.argtypes = [vp, up]
+        L.kd_run.argtypes = [vp, C.c_uint64, C.POINTER(Accum)]
+        L.kd_get_accum.argtypes = [vp, C.POINTER(Accum)]
+        L.kd_reset_accum.argtypes = [vp]
+        L.kd_sample_device.argtypes = [vp, C.c_uint64, vp, vp]
+        L.kd_last_kernel_ms.argtypes = [vp, dp, ip]
+        L.kd_flops_per_point.argtypes = [vp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class KdError(RuntimeError):
+    pass
+
+
+class Context:
+    """Owner of one kd_ctx (one device, one stream)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().kd_create(C.byref(self.h), device)
+        if rc != KD_OK:
+            raise KdError(f"kd_create failed ({rc}): no usable HIP device")
+
+    def _check(self, rc):
+        if rc != KD_OK:
+            msg = lib().kd_last_error(self.h).decode()
+            raise KdError(msg or f"keldy_amd error {rc}")
+
+    def close(self):
+        if self.h:
+            lib().kd_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up ----------------------------------------------------------
+    def set_g0_table(self, lesser, greater, time_max, alpha=0.0):
+        les = np.ascontiguousarray(lesser, dtype=np.complex128)
+        gre = np.ascontiguousarray(greater, dtype=np.complex128)
+        if les.ndim == 1:
+            les = les.reshape(1, 1, -1)
+            gre = gre.reshape(1, 1, -1)
+        self._g0 = (les, gre)
+        self._check(lib().kd_set_g0_table(self.h, _dp(les.view(np.float64)), _dp(gre.view(np.float64)),
+                                          les.shape[0], les.shape[-1], time_max, alpha))
+
+    def set_integrand(self, order, A, B, cutoff=0.0):
+        def gi(x):
+            return x if isinstance(x, GfIndex) else GfIndex(float(x[0]), int(x[2]), int(x[1]),
+                                                             int(x[3]) if len(x) > 3 else 0, 0)
+        self._check(lib().kd_set_integrand(self.h, order, C.byref(gi(A)), C.byref(gi(B)), cutoff))
+        self.order = order
+
+    def set_precision(self, prec):
+        self._check(lib().kd_set_precision(self.h, prec))
+
+    def set_warper_chain(self, descs, maps, tables):
+        nd, nm = len(descs), len(maps)
+        darr = (WarperDesc * max(nd, 1))(*descs)
+        marr = (Map1d * max(nm, 1))(*maps)
+        tab = np.ascontiguousarray(tables if len(tables) else np.zeros(1), dtype=np.float64)
+        self._tab = tab
+        self._check(lib().kd_set_warper_chain(self.h, darr, nd, marr, nm, _dp(tab), len(tables)))
+
+    def set_sobol(self, Csr, shfr, k=0):
+        Csr = np.ascontiguousarray(Csr, dtype=np.uint32)
+        shfr = np.ascontiguousarray(shfr, dtype=np.uint32)
+        self._check(lib().kd_set_sobol(self.h, shfr.shape[0], Csr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                       shfr.ctypes.data_as(C.POINTER(C.c_uint32)), k))
+
+    # -- evaluation ------------------------------------------------------
+    def eval_batch(self, u, keep_u_hypercube=True):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        n = u.shape[0]
+        out = np.empty((n, 2))
+        dom = np.empty(n, dtype=np.int32)
+        self._check(lib().kd_eval_batch(self.h, _dp(u), n, int(keep_u_hypercube), _dp(out),
+                                        dom.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out[:, 0] + 1j * out[:, 1], dom
+
+    def eval_warped_batch(self, l):
+        l = np.ascontiguousarray(l, dtype=np.float64)
+        n = l.shape[0]
+        out = np.empty((n, 2))
+        dom = np.empty(n, dtype=np.int32)
+        jac = np.empty(n)
+        self._check(lib().kd_eval_warped_batch(self.h, _dp(l), n, _dp(out),
+                                               dom.ctypes.data_as(C.POINTER(C.c_int32)), _dp(jac)))
+        return out[:, 0] + 1j * out[:, 1], dom, jac
+
+    def run(self, n_steps):
+        acc = Accum()
+        self._check(lib().kd_run(self.h, n_steps, C.byref(acc)))
+        return acc
+
+    def accum(self):
+        acc = Accum()
+        self._check(lib().kd_get_accum(self.h, C.byref(acc)))
+        return acc
+
+    def reset_accum(self):
+        self._check(lib().kd_reset_accum(self.h))
+
+    def sobol_state(self):
+        k = C.c_uint32()
+        self._check(lib().kd_get_sobol_state(self.h, C.byref(k)))
+        return k.value
+
+    def last_kernel_ms(self):
+        ms = C.c_double()
+        n = C.c_int32()
+        self._check(lib().kd_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def flops_per_point(self):
+        f = C.c_double()
+        self._check(lib().kd_flops_per_point(self.h, C.byref(f)))
+        return f.value
+
+
+def device_count():
+    n = C.c_int()
+    lib().kd_device_count(C.byref(n))
+    return n.value

@@ -1,0 +1,218 @@
+// kd_device.hpp -- device-side building blocks of the Keldysh-sum kernel (gfx950).
+//
+// Complex arithmetic on double2/float2, the g0 contour lookup, the Sobol point
+// generator and the warper chain.  Reference semantics cited per function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/keldy_amd.h"
+
+namespace kd {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+template <typename T> struct cvec;
+template <> struct cvec<double> { using type = double2; };
+template <> struct cvec<float> { using type = float2; };
+template <typename T> using cx = typename cvec<T>::type;
+
+template <typename T> __device__ __forceinline__ cx<T> cmk(T re, T im) { cx<T> z; z.x = re; z.y = im; return z; }
+template <typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return cmk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a - l*b
+template <typename T> __device__ __forceinline__ cx<T> cfms(cx<T> a, cx<T> l, cx<T> b) {
+  return cmk<T>(a.x - l.x * b.x + l.y * b.y, a.y - l.x * b.y - l.y * b.x);
+}
+template <typename T> __device__ __forceinline__ cx<T> cinv(cx<T> a) {
+  T d = a.x * a.x + a.y * a.y;
+  T r = T(1) / d;
+  return cmk<T>(a.x * r, -a.y * r);
+}
+template <typename T> __device__ __forceinline__ T cabs1(cx<T> a) { return fabs(a.x) + fabs(a.y); }
+
+// ---------------------------------------------------------------------------
+// Kernel parameters (by value)
+struct KdGf {
+  double time;
+  int k_idx, spin, orbital;
+};
+
+struct KdParams {
+  // g0 table (model.hpp:238-268): [n_orb][n_orb][n_pts] complex
+  const double2 *lesser;
+  const double2 *greater;
+  int n_pts, n_orb;
+  double t_min, dt, alpha;
+  // integrand (wick_direct.hpp:34-60)
+  KdGf A, B;
+  double cutoff;
+  int keep_u;
+  // warper chain (warpers.hpp:41-137)
+  const kd_warper_desc *warpers;
+  const kd_map1d *maps;
+  const double *tables;
+  int n_warpers;
+  // Sobol (qrng.hpp:49-114): bit-reversed matrices [dim][31], shifts [dim]
+  const uint32_t *Csr;
+  const uint32_t *shfr;
+  uint32_t k0; // points k0+1 ... k0+n_points
+  // per-point I/O
+  int mode;
+  const double *in_pts; // MODE_EVAL_U: u, MODE_EVAL_L: l
+  int64_t n_points;
+  double *out_vals; // n x 2
+  int *out_dom;
+  double *out_jac;
+  double *out_l; // MODE_SAMPLE
+  // accumulation (MODE_RUN): per block {re, im, n_dom, n_fallback}
+  double *partials;
+  // flow warper parameters (KD_WARP_FLOW)
+  const float *flow;
+  int flow_layers, flow_hidden;
+};
+
+// Compacted in-domain points between the two kernel stages (batch b of 64
+// points owns slots [64 b, 64 b + cnt[b])).
+struct PrepBuf {
+  double *u;   // [n][order]
+  double *jac; // [n]
+  int *idx;    // lane of the point inside its batch
+  int *cnt;    // [n / 64] in-domain points per batch
+};
+
+enum { MODE_EVAL_U = 0, MODE_EVAL_L = 1, MODE_RUN = 2, MODE_SAMPLE = 3 };
+
+// ---------------------------------------------------------------------------
+// g0 lookup: g0_keldysh_contour_t::operator() (model.hpp:244-265) with
+// compare_3way (common.cpp:47-57) and linear interpolation on the time mesh.
+struct CPt {
+  double time;
+  int k_idx, split, orb;
+};
+
+__device__ __forceinline__ int compare_3way(const CPt &a, const CPt &b) {
+  int kd = a.k_idx - b.k_idx;
+  if (kd != 0) return kd;
+  double dt = a.time - b.time;
+  if (dt != 0.0) return (1 - 2 * a.k_idx) * (signbit(dt) ? -1 : 1);
+  return (1 - 2 * a.k_idx) * (a.split - b.split);
+}
+
+__device__ __forceinline__ double2 table_eval(const KdParams &p, const double2 *tab, int oa, int ob,
+                                              double t) {
+  const double2 *g = tab + (size_t)(oa * p.n_orb + ob) * p.n_pts;
+  double a = (t - p.t_min) / p.dt;
+  long i = (long)floor(a);
+  i = i > p.n_pts - 2 ? p.n_pts - 2 : i;
+  i = i < 0 ? 0 : i;
+  double w = a - (double)i;
+  double2 g0 = g[i], g1 = g[i + 1];
+  double2 r;
+  r.x = (1.0 - w) * g0.x + w * g1.x;
+  r.y = (1.0 - w) * g0.y + w * g1.y;
+  return r;
+}
+
+__device__ __forceinline__ double2 g0_eval(const KdParams &p, const CPt &a, const CPt &b, bool internal) {
+  int ord = compare_3way(a, b);
+  if (ord > 0) return table_eval(p, p.greater, a.orb, b.orb, a.time - b.time);
+  if (ord < 0) return table_eval(p, p.lesser, a.orb, b.orb, a.time - b.time);
+  double2 v = table_eval(p, p.lesser, a.orb, b.orb, 0.0);
+  if (internal && a.orb == b.orb) v.y -= p.alpha;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Sobol point k (digitalseq_b2g.hpp:415-426 / set_state 455-471): the state
+// after k steps is the XOR of the columns selected by the Gray code of k.
+// k = 2^31 follows next()'s column rule (ctz(k) % 31 == 0).
+__device__ __forceinline__ double sobol_coord(const uint32_t *__restrict__ Csr, const uint32_t *__restrict__ shfr,
+                                              int j, uint32_t k) {
+  uint32_t g = k ^ (k >> 1);
+  uint32_t cur = 0;
+  const uint32_t *col = Csr + j * 31;
+  for (int i = 0; i < 31; i++) cur ^= col[i] & (0u - ((g >> i) & 1u));
+  if (k == 0x80000000u) cur = col[30] ^ col[0];
+  return (double)(cur ^ shfr[j]) * (1.0 / 4294967296.0);
+}
+
+// ---------------------------------------------------------------------------
+// GSL-compatible interval search (gsl_interp_bsearch over [0, n-1]).
+__device__ __forceinline__ int bsearch_knots(const double *xa, int n, double x) {
+  int ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    int i = (ihi + ilo) >> 1;
+    if (xa[i] > x)
+      ihi = i;
+    else
+      ilo = i;
+  }
+  return ilo;
+}
+
+// x = F^{-1}(l), returns 1/f(x)   (product_1d_simple.hpp:109-117 and makers 560-630,
+// interp nearest 198-217, interp hybrid 403-415)
+__device__ inline double map1d_reverse(const kd_map1d &m, const double *__restrict__ tab, double l, double *u) {
+  double x, f;
+  const double w = m.w, um = m.u_max;
+  switch (m.kind) {
+  case KD_MAP_UNIFORM:
+    x = um * l;
+    f = 1.0 / um;
+    break;
+  case KD_MAP_EXPONENTIAL:
+    x = um - w * log1p(expm1(um / w) * l);
+    f = exp(-x / w) / (w * m.norm);
+    break;
+  case KD_MAP_INVERSE:
+    x = w * expm1(l * m.norm);
+    f = 1.0 / (m.norm * (w + x));
+    break;
+  case KD_MAP_INVERSE_SQ:
+    x = m.norm * l * w / (1.0 - m.norm * l);
+    f = w / ((w + x) * (w + x) * m.norm);
+    break;
+  case KD_MAP_INVERSE_POW: {
+    const int pw = m.power;
+    double base = pow(w / (um + w), (double)(pw - 1));
+    double denom = pow(1.0 + l * (base - 1.0), 1.0 / (pw - 1.0));
+    x = w * (1.0 - denom) / denom;
+    f = m.norm / pow(w + x, (double)pw);
+    break;
+  }
+  case KD_MAP_NEAREST: {
+    const double *F = tab + m.F_off, *t = tab + m.t_off, *fk = tab + m.f_off;
+    double y = -l;
+    int i = bsearch_knots(F, m.n_inv, y);
+    double dx = F[i + 1] - F[i];
+    x = dx > 0.0 ? t[i] + (y - F[i]) / dx * (t[i + 1] - t[i]) : 0.0;
+    if (y < F[0] || y > F[m.n_inv - 1]) x = __builtin_nan("");
+    int j = bsearch_knots(t, m.n_pts, x);
+    f = 0.5 * (fk[j] + fk[j + 1]);
+    break;
+  }
+  case KD_MAP_HYBRID: {
+    const double *F = tab + m.F_off, *c = tab + m.c_off;
+    double y = -l;
+    int i = bsearch_knots(F, m.n_inv, y);
+    double d = y - F[i];
+    x = c[4 * i + 3] + d * (c[4 * i + 2] + d * (c[4 * i + 1] + d * c[4 * i + 0]));
+    if (y < F[0] || y > F[m.n_inv - 1]) x = __builtin_nan("");
+    double num = w, den = w + x, dd = den;
+    for (int q = 1; q < m.power; q++) num *= w;
+    for (int q = 1; q < m.power; q++) dd *= den;
+    f = (num / dd) / m.norm;
+    break;
+  }
+  default:
+    x = __builtin_nan("");
+    f = __builtin_nan("");
+  }
+  *u = x;
+  return 1.0 / f;
+}
+
+} // namespace kd

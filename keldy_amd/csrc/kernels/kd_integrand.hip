@@ -1,0 +1,431 @@
+// kd_integrand.hip -- the fused Keldysh-sum kernel for gfx950.
+//
+// One 64-lane wavefront per workgroup.  Per pass the wave takes 64 candidate
+// points (one per lane): Sobol point or caller-supplied point, warper chain in
+// registers, domain check (wick_direct.cpp:53-57).  In-domain points are
+// compacted with a ballot.  For each group of compacted points the wave stages
+// the (2n+1)^2 complex Wick matrix of g0 values in LDS
+// (wick_direct.cpp:92-101; the opposite-spin matrix equals its leading 2n x 2n
+// block because g0 is spin independent, model.cpp:463-465).  Each lane then
+// owns one or more Keldysh configurations sigma (wick_direct.cpp:110-149) and
+// computes
+//     (-1)^|sigma| det(S1_sigma) det(S2_sigma) = (-1)^|sigma| det(A)^2 s,
+// with A = S2_sigma, S1_sigma = [[A, b], [c, 0]] and s = -c A^{-1} b the
+// bordered Schur complement, by a register-resident row-wise LU of the
+// bordered matrix.  The LU runs without row exchanges; a pivot that lost more
+// than a set number of bits to cancellation flags the configuration, which is
+// then recomputed by a partial-pivoting FP64 LU.  The alternating sum over
+// sigma is reduced with wavefront shuffles in FP64 and accumulated in FP64.
+#include <type_traits>
+
+#include "kd_device.hpp"
+#include "kd_kernels.hpp"
+
+namespace kd {
+
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <typename T> struct PivotTol;
+template <> struct PivotTol<double> { static constexpr double v = 1.0 / 4096.0; };
+template <> struct PivotTol<float> { static constexpr float v = 1.0f / 256.0f; };
+
+// det(A)^2 * s for one Keldysh configuration, unpivoted row-wise LU in T.
+template <int N, typename T>
+__device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, int cfg, bool &bad) {
+  constexpr int D = 2 * N + 1;
+  int pr[N + 1], pc[N + 1];
+  static_for<0, N>([&](auto I) {
+    constexpr int i = I;
+    int pi = i + ((cfg >> i) & 1) * N;
+    pr[i] = pi * D;
+    pc[i] = pi;
+  });
+  pr[N] = 2 * N * D;
+  pc[N] = 2 * N;
+  cx<T> Uh[N + 1][N + 1];
+  cx<T> det = cmk<T>(T(1), T(0));
+  cx<T> s = cmk<T>(T(0), T(0));
+  bool flag = false;
+  static_for<0, N + 1>([&](auto I) {
+    constexpr int i = I;
+    cx<T> row[N + 1];
+    static_for<0, N + 1>([&](auto J) {
+      constexpr int j = J;
+      if constexpr (i == N && j == N) {
+        row[j] = cmk<T>(T(0), T(0));
+      } else {
+        double2 w = Wq[pr[i] + pc[j]];
+        row[j] = cmk<T>((T)w.x, (T)w.y);
+      }
+    });
+    T diag0 = T(0);
+    if constexpr (i < N) diag0 = cabs1<T>(row[i]);
+    constexpr int KE = i < N ? i : N;
+    static_for<0, KE>([&](auto K) {
+      constexpr int k = K;
+      cx<T> l = row[k];
+      static_for<k + 1, N + 1>([&](auto J) {
+        constexpr int j = J;
+        row[j] = cfms<T>(row[j], l, Uh[k][j]);
+      });
+    });
+    if constexpr (i < N) {
+      cx<T> piv = row[i];
+      T pm = cabs1<T>(piv);
+      flag = flag || !(pm >= PivotTol<T>::v * diag0) || pm == T(0);
+      det = cmul<T>(det, piv);
+      cx<T> inv = cinv<T>(piv);
+      static_for<i + 1, N + 1>([&](auto J) {
+        constexpr int j = J;
+        Uh[i][j] = cmul<T>(row[j], inv);
+      });
+    } else {
+      s = row[N];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  bad = flag;
+  cx<T> d2 = cmul<T>(det, det);
+  cx<T> v = cmul<T>(d2, s);
+  return make_double2((double)v.x, (double)v.y);
+}
+
+// Fallback: bordered LU with partial pivoting over the rows of A (pivot =
+// max |Re|+|Im|, first on ties: the LAPACK zgetf2 rule), FP64.  The row
+// permutation's sign cancels in det(A)^2.
+__device__ __noinline__ double2 config_value_pivoted(const double2 *__restrict__ Wq, int cfg, int N) {
+  const int D = 2 * N + 1, M = N + 1;
+  double2 a[(KD_MAX_ORDER + 1) * (KD_MAX_ORDER + 1)];
+  int pk[KD_MAX_ORDER + 1];
+  for (int i = 0; i < N; i++) pk[i] = i + ((cfg >> i) & 1) * N;
+  pk[N] = 2 * N;
+  for (int i = 0; i < M; i++)
+    for (int j = 0; j < M; j++) a[i * M + j] = Wq[pk[i] * D + pk[j]];
+  a[N * M + N] = make_double2(0.0, 0.0);
+  double2 det = make_double2(1.0, 0.0);
+  for (int k = 0; k < N; k++) {
+    int p = k;
+    double best = cabs1<double>(a[k * M + k]);
+    for (int r = k + 1; r < N; r++) {
+      double v = cabs1<double>(a[r * M + k]);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (best == 0.0) return make_double2(0.0, 0.0);
+    if (p != k)
+      for (int j = 0; j < M; j++) {
+        double2 t = a[k * M + j];
+        a[k * M + j] = a[p * M + j];
+        a[p * M + j] = t;
+      }
+    double2 piv = a[k * M + k];
+    det = cmul<double>(det, piv);
+    double2 inv = cinv<double>(piv);
+    for (int r = k + 1; r < M; r++) {
+      double2 l = cmul<double>(a[r * M + k], inv);
+      for (int j = k + 1; j < M; j++) a[r * M + j] = cfms<double>(a[r * M + j], l, a[k * M + j]);
+    }
+  }
+  double2 d2 = cmul<double>(det, det);
+  return cmul<double>(d2, a[N * M + N]);
+}
+
+// Warper chain, applied last-added first (warpers.hpp:72-85).
+template <int N>
+__device__ __forceinline__ double apply_warpers(const KdParams &p, double (&x)[N]) {
+  double jac = 1.0;
+  for (int q = p.n_warpers - 1; q >= 0; q--) {
+    const kd_warper_desc wd = p.warpers[q];
+    if (wd.kind == KD_WARP_PLASMA) {
+      // ui_from_vi (plasma_uv.hpp:46-51)
+      x[0] = wd.t_max - x[0];
+#pragma unroll
+      for (int i = 1; i < N; i++) x[i] = x[i - 1] - x[i];
+    } else if (wd.kind == KD_WARP_PRODUCT) {
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const kd_map1d m = p.maps[wd.map_offset + (wd.n_maps == 1 ? 0 : i)];
+        double xi;
+        jac *= map1d_reverse(m, p.tables, x[i], &xi);
+        x[i] = xi;
+      }
+    } else if (wd.kind == KD_WARP_FLOW) {
+      jac *= flow_reverse<N>(p, x);
+    }
+  }
+  return jac;
+}
+
+// Stage 1: candidate points.  One thread per point; a 64-point batch b is
+// one wavefront.  l (Sobol point or caller input) -> warper chain -> u, J and
+// the domain check (wick_direct.cpp:53-57).  In-domain points of batch b are
+// compacted, in lane order, to slots [64 b, 64 b + cnt[b]).
+template <int N>
+__global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
+  constexpr int NU = N > 0 ? N : 1;
+  __shared__ uint32_t sC[NU * 31];
+  __shared__ uint32_t sS[NU];
+  const bool sob = p.mode == MODE_RUN || p.mode == MODE_SAMPLE;
+  if (sob) {
+    for (int i = threadIdx.x; i < N * 31; i += blockDim.x) sC[i] = p.Csr[i];
+    if (threadIdx.x < N) sS[threadIdx.x] = p.shfr[threadIdx.x];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t n = p.n_points;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const bool spin_ok = p.A.spin == p.B.spin;
+  for (int64_t pi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pi - lane < n; pi += stride) {
+    // keep the warper descriptors' loads inside the loop (no hoisting into registers)
+    __asm__ volatile("" ::: "memory");
+    const bool valid = pi < n;
+    double x[NU];
+    double jac = 1.0;
+    bool dom = valid && spin_ok;
+    if (valid) {
+      if (sob) {
+        const uint32_t k = p.k0 + 1u + (uint32_t)pi;
+#pragma unroll
+        for (int i = 0; i < N; i++) x[i] = sobol_coord(sC, sS, i, k);
+        if (p.mode == MODE_SAMPLE) {
+#pragma unroll
+          for (int i = 0; i < N; i++) p.out_l[pi * N + i] = x[i];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) x[i] = p.in_pts[pi * N + i];
+      }
+      if constexpr (N > 0) {
+        if (p.mode != MODE_EVAL_U) jac = apply_warpers<N>(p, x);
+      }
+      if (p.keep_u) {
+#pragma unroll
+        for (int i = 0; i < N; i++) dom = dom && !(x[i] < 0.0);
+      }
+    }
+    const uint64_t mask = __ballot(dom);
+    const int64_t b = pi >> 6;
+    if (dom) {
+      const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+      const int64_t slot = b * 64 + pos;
+#pragma unroll
+      for (int i = 0; i < N; i++) buf.u[slot * N + i] = x[i];
+      buf.jac[slot] = jac;
+      buf.idx[slot] = lane;
+    }
+    if (lane == 0) buf.cnt[b] = __popcll(mask);
+    if (valid && !dom && p.mode != MODE_RUN) {
+      p.out_vals[2 * pi] = 0.0;
+      p.out_vals[2 * pi + 1] = 0.0;
+      if (p.out_dom) p.out_dom[pi] = 0;
+      if (p.out_jac) p.out_jac[pi] = jac;
+    }
+  }
+}
+
+// Stage 2: the Keldysh sum of the compacted points, one 64-lane workgroup per
+// batch (grid-stride).
+template <int N, typename T>
+__global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) {
+  constexpr int NC = 1 << N;            // Keldysh configurations per point
+  constexpr int G = NC < 64 ? NC : 64;  // lanes per point
+  constexpr int PPW = 64 / G;           // points per group
+  constexpr int CPL = NC / G;           // configurations per lane
+  constexpr int D = 2 * N + 1;
+  constexpr int NU = N > 0 ? N : 1;
+  __shared__ double2 W[PPW][D * D];
+  __shared__ double Us[PPW][NU];
+
+  const int lane = threadIdx.x;
+  const int64_t nbatch = (p.n_points + 63) / 64;
+  double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
+
+  for (int64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
+    const int cnt = buf.cnt[b];
+    for (int g0i = 0; g0i < cnt; g0i += PPW) {
+      const int ng = min(PPW, cnt - g0i);
+      // Opaque per-iteration copy of the lane id: keeps the lane-only index
+      // math of the matrix build from being hoisted into registers.
+      int lane = threadIdx.x;
+      __asm__ volatile("" : "+v"(lane));
+      const int64_t slot0 = b * 64 + g0i;
+      if (lane < ng * N) Us[lane / NU][lane % NU] = buf.u[slot0 * N + lane];
+      __syncthreads();
+      // Stage the Wick matrices of the group's points in LDS (wick_direct.cpp:92-101).
+      for (int e = lane; e < ng * D * D; e += 64) {
+        const int q = e / (D * D), rc = e - q * (D * D);
+        const int r = rc / D, c = rc - r * D;
+        CPt a, bb;
+        if (r < 2 * N) {
+          const int v = r < N ? r : r - N;
+          a = CPt{Us[q][v], r < N ? 0 : 1, v, 0};
+        } else {
+          a = CPt{p.A.time, p.A.k_idx, N, p.A.orbital};
+        }
+        if (c < 2 * N) {
+          const int v = c < N ? c : c - N;
+          bb = CPt{Us[q][v], c < N ? 0 : 1, v, 0};
+        } else {
+          bb = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
+        }
+        const bool corner = (r == 2 * N) && (c == 2 * N);
+        double2 gv = g0_eval(p, a, bb, !corner);
+        if (corner && N > 0) gv = make_double2(0.0, 0.0);
+        W[q][rc] = gv;
+      }
+      __syncthreads();
+
+      const int q = lane / G;
+      double sre = 0.0, sim = 0.0;
+      int nfb = 0;
+      if (q < ng) {
+        if constexpr (N == 0) {
+          const double2 v = W[q][0];
+          sre = v.x;
+          sim = v.y;
+        } else {
+          uint64_t badmask = 0;
+#pragma unroll 1
+          for (int m = 0; m < CPL; m++) {
+            const int cfg = (lane % G) + G * m;
+            bool bad;
+            const double2 v = config_value<N, T>(PPW == 1 ? W[0] : W[q], cfg, bad);
+            // branch-free: a divergent branch here doubles the register footprint
+            const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
+            sre += bad ? 0.0 : sg * v.x;
+            sim += bad ? 0.0 : sg * v.y;
+            badmask |= (uint64_t)bad << m;
+          }
+          // Flagged configurations: partial-pivoting FP64 LU, outside the hot loop.
+          while (badmask) {
+            const int m = __builtin_ctzll(badmask);
+            badmask &= badmask - 1;
+            const int cfg = (lane % G) + G * m;
+            const double2 v = config_value_pivoted(W[q], cfg, N);
+            const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
+            sre += sg * v.x;
+            sim += sg * v.y;
+            nfb++;
+          }
+        }
+      }
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) {
+        sre += __shfl_xor(sre, off, 64);
+        sim += __shfl_xor(sim, off, 64);
+      }
+      acc_fb += (double)nfb;
+      if ((lane % G) == 0 && q < ng) {
+        const int64_t slot = slot0 + q;
+        if (sqrt(sre * sre + sim * sim) < p.cutoff) {
+          sre = 0.0;
+          sim = 0.0;
+        }
+        const double jj = buf.jac[slot];
+        sre *= jj;
+        sim *= jj;
+        if (p.mode == MODE_RUN) {
+          acc_re += sre;
+          acc_im += sim;
+          acc_dom += 1.0;
+        } else {
+          const int64_t idx = b * 64 + buf.idx[slot];
+          p.out_vals[2 * idx] = sre;
+          p.out_vals[2 * idx + 1] = sim;
+          if (p.out_dom) p.out_dom[idx] = 1;
+          if (p.out_jac) p.out_jac[idx] = jj;
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  if (p.mode == MODE_RUN) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      acc_re += __shfl_xor(acc_re, off, 64);
+      acc_im += __shfl_xor(acc_im, off, 64);
+      acc_dom += __shfl_xor(acc_dom, off, 64);
+      acc_fb += __shfl_xor(acc_fb, off, 64);
+    }
+    if (lane == 0) {
+      double *o = p.partials + 4 * (int64_t)blockIdx.x;
+      o[0] = acc_re;
+      o[1] = acc_im;
+      o[2] = acc_dom;
+      o[3] = acc_fb;
+    }
+  }
+}
+
+// Deterministic sum of per-block partials into acc[0..3] (+=).
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double *__restrict__ part, int nblk,
+                                                              double *__restrict__ acc) {
+  __shared__ double sh[4][256];
+  const int t = threadIdx.x;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = t; i < nblk; i += 256)
+#pragma unroll
+    for (int c = 0; c < 4; c++) s[c] += part[4 * i + c];
+#pragma unroll
+  for (int c = 0; c < 4; c++) sh[c][t] = s[c];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int c = 0; c < 4; c++) sh[c][t] += sh[c][t + w];
+    __syncthreads();
+  }
+  if (t == 0)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] += sh[c][0];
+}
+
+template <int N, typename T>
+static hipError_t launch_n(const KdParams &p, const PrepBuf &buf, int grid_prep, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((prep_kernel<N>), dim3(grid_prep), dim3(256), 0, st, p, buf);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((integrand_kernel<N, T>), dim3(grid), dim3(64), 0, st, p, buf);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_t(int order, const KdParams &p, const PrepBuf &buf, int gp, int g, hipStream_t st) {
+  switch (order) {
+  case 0: return launch_n<0, T>(p, buf, gp, g, st);
+  case 1: return launch_n<1, T>(p, buf, gp, g, st);
+  case 2: return launch_n<2, T>(p, buf, gp, g, st);
+  case 3: return launch_n<3, T>(p, buf, gp, g, st);
+  case 4: return launch_n<4, T>(p, buf, gp, g, st);
+  case 5: return launch_n<5, T>(p, buf, gp, g, st);
+  case 6: return launch_n<6, T>(p, buf, gp, g, st);
+  case 7: return launch_n<7, T>(p, buf, gp, g, st);
+  case 8: return launch_n<8, T>(p, buf, gp, g, st);
+  case 9: return launch_n<9, T>(p, buf, gp, g, st);
+  case 10: return launch_n<10, T>(p, buf, gp, g, st);
+  case 11: return launch_n<11, T>(p, buf, gp, g, st);
+  case 12: return launch_n<12, T>(p, buf, gp, g, st);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_integrand(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid_prep,
+                            int grid, hipStream_t st) {
+  if (precision == KD_PREC_FP32_LU) return launch_t<float>(order, p, buf, grid_prep, grid, st);
+  return launch_t<double>(order, p, buf, grid_prep, grid, st);
+}
+
+hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, part, nblk, acc);
+  return hipGetLastError();
+}
+
+} // namespace kd

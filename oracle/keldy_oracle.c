@@ -1,0 +1,647 @@
+/*
+ * keldy_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY, see keldy_oracle.h).
+ *
+ * Serial FP64 restatement of the reference hot path:
+ *   Sobol          qrng.hpp:49-114, digitalseq_b2g.hpp:25-36,130-237,295-547,598-873
+ *   warpers        warpers.hpp:72-85, identity.hpp, plasma_uv.hpp:35-81,
+ *                  product_1d_simple.hpp:44-630, product_1d.hpp:43-310
+ *   g0 adapter     common.cpp:47-57, model.hpp:244-265
+ *   integrand      wick_direct.cpp:43-228
+ *   integrator     integrator.hpp:68-131
+ * Third-party pieces restated from their published algorithms:
+ *   std::mt19937_64           Matsumoto & Nishimura (2000), 64-bit MT parameters
+ *   JK2008 direction numbers  Joe & Kuo (2008), new-joe-kuo-6.21201 (s, a, m_i)
+ *   TRIQS determinant         LAPACK zgetf2 partial pivoting (pivot = max |Re|+|Im|)
+ *   GSL interp linear/steffen gsl_interp_bsearch + linear / Steffen (1990) splines
+ *   TRIQS gf<retime> eval     linear interpolation between mesh neighbours
+ *                             (the reference's tests never evaluate off-grid, so
+ *                             this choice is parity-unpinned against TRIQS)
+ */
+#include "keldy_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------------ */
+/* mt19937_64                                                                */
+/* ------------------------------------------------------------------------ */
+#define MT_NN 312
+#define MT_MM 156
+#define MT_MATRIX_A 0xB5026F5AA96619E9ULL
+#define MT_UM 0xFFFFFFFF80000000ULL
+#define MT_LM 0x7FFFFFFFULL
+
+void ko_mt64_seed(ko_mt64 *g, uint64_t seed) {
+  g->mt[0] = seed;
+  for (int i = 1; i < MT_NN; i++)
+    g->mt[i] = 6364136223846793005ULL * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t)i;
+  g->mti = MT_NN;
+}
+
+uint64_t ko_mt64_next(ko_mt64 *g) {
+  static const uint64_t mag01[2] = {0ULL, MT_MATRIX_A};
+  uint64_t x;
+  if (g->mti >= MT_NN) {
+    int i;
+    for (i = 0; i < MT_NN - MT_MM; i++) {
+      x = (g->mt[i] & MT_UM) | (g->mt[i + 1] & MT_LM);
+      g->mt[i] = g->mt[i + MT_MM] ^ (x >> 1) ^ mag01[(int)(x & 1ULL)];
+    }
+    for (; i < MT_NN - 1; i++) {
+      x = (g->mt[i] & MT_UM) | (g->mt[i + 1] & MT_LM);
+      g->mt[i] = g->mt[i + (MT_MM - MT_NN)] ^ (x >> 1) ^ mag01[(int)(x & 1ULL)];
+    }
+    x = (g->mt[MT_NN - 1] & MT_UM) | (g->mt[0] & MT_LM);
+    g->mt[MT_NN - 1] = g->mt[MT_MM - 1] ^ (x >> 1) ^ mag01[(int)(x & 1ULL)];
+    g->mti = 0;
+  }
+  x = g->mt[g->mti++];
+  x ^= (x >> 29) & 0x5555555555555555ULL;
+  x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+  x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+  x ^= (x >> 43);
+  return x;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sobol                                                                     */
+/* ------------------------------------------------------------------------ */
+/* Joe-Kuo (2008) primitive polynomials (degree s, coefficients a) and
+ * initial direction numbers m_1..m_s for dimensions 2..32 (dimension 1 is
+ * the van der Corput sequence). */
+static const int jk_s[KO_MAX_DIM] = {0, 1, 2, 3, 3, 4, 4, 5, 5, 5, 5, 5, 5, 6, 6, 6,
+                                     6, 6, 6, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7, 7};
+static const int jk_a[KO_MAX_DIM] = {0,  0,  1,  1,  2,  1,  4,  2,  4,  7,  11,
+                                     13, 14, 1,  13, 16, 19, 22, 25, 1,  4,  7,
+                                     8,  14, 19, 21, 28, 31, 32, 37, 41, 42};
+static const int jk_m[KO_MAX_DIM][7] = {
+    {0},
+    {1},
+    {1, 3},
+    {1, 3, 1},
+    {1, 1, 1},
+    {1, 1, 3, 3},
+    {1, 3, 5, 13},
+    {1, 1, 5, 5, 17},
+    {1, 1, 5, 5, 5},
+    {1, 1, 7, 11, 19},
+    {1, 1, 5, 1, 1},
+    {1, 1, 1, 3, 11},
+    {1, 3, 5, 5, 31},
+    {1, 3, 3, 9, 7, 49},
+    {1, 1, 1, 15, 21, 21},
+    {1, 3, 1, 13, 27, 49},
+    {1, 1, 1, 15, 7, 5},
+    {1, 3, 1, 15, 13, 25},
+    {1, 1, 5, 5, 19, 61},
+    {1, 3, 7, 11, 23, 15, 103},
+    {1, 3, 7, 13, 13, 15, 69},
+    {1, 1, 3, 13, 7, 35, 63},
+    {1, 3, 5, 9, 1, 25, 53},
+    {1, 3, 1, 13, 9, 35, 107},
+    {1, 3, 1, 5, 27, 61, 31},
+    {1, 1, 5, 11, 19, 41, 61},
+    {1, 3, 5, 3, 3, 13, 69},
+    {1, 1, 7, 13, 1, 19, 1},
+    {1, 3, 7, 5, 13, 19, 59},
+    {1, 1, 3, 9, 25, 29, 41},
+    {1, 3, 5, 13, 23, 1, 55},
+    {1, 3, 7, 3, 13, 59, 17}};
+
+/* Column c (0-based) of the generating matrix of dimension j, stored as an
+ * integer with the LSB in the top row: the bit-reversal over c+1 bits of the
+ * direction number m_{c+1}. (digitalseq_b2g.hpp:598-861 holds the same
+ * matrices as a literal table.) */
+int ko_jk2008_matrices(uint32_t *Cs, int dim, int m) {
+  if (dim < 1 || dim > KO_MAX_DIM || m < 1 || m > 32) return -1;
+  for (int j = 0; j < dim; j++) {
+    uint64_t mv[32];
+    int s = jk_s[j];
+    if (j == 0) {
+      for (int c = 0; c < m; c++) mv[c] = 1;
+    } else {
+      for (int c = 0; c < s && c < m; c++) mv[c] = (uint64_t)jk_m[j][c];
+      for (int c = s; c < m; c++) {
+        uint64_t v = mv[c - s] ^ (mv[c - s] << s);
+        for (int q = 1; q < s; q++)
+          if ((jk_a[j] >> (s - 1 - q)) & 1) v ^= mv[c - q] << q;
+        mv[c] = v;
+      }
+    }
+    for (int c = 0; c < m; c++) {
+      uint32_t r = 0;
+      for (int b = 0; b <= c; b++)
+        if ((mv[c] >> b) & 1) r |= 1u << (c - b);
+      Cs[j * m + c] = r;
+    }
+  }
+  return 0;
+}
+
+static uint32_t bitrev32(uint32_t v) {
+  v = ((v >> 1) & 0x55555555u) | ((v & 0x55555555u) << 1);
+  v = ((v >> 2) & 0x33333333u) | ((v & 0x33333333u) << 2);
+  v = ((v >> 4) & 0x0F0F0F0Fu) | ((v & 0x0F0F0F0Fu) << 4);
+  v = ((v >> 8) & 0x00FF00FFu) | ((v & 0x00FF00FFu) << 8);
+  return (v >> 16) | (v << 16);
+}
+
+/* qrng.hpp:77-109.  Note the reference passes the mt19937_64 engine by value
+ * both to std::generate_n (shifts) and to generate_random_linear_scramble, so
+ * the shift draws and the scramble draws both start from the freshly seeded
+ * engine. */
+int ko_sobol_init(ko_sobol *s, int dim, uint32_t state_seed, int do_shift, int do_scramble,
+                  int64_t rng_seed_shift) {
+  const int m = KO_SOBOL_M;
+  uint32_t Cs[KO_MAX_DIM * KO_SOBOL_M];
+  uint32_t shifts[KO_MAX_DIM];
+  if (ko_jk2008_matrices(Cs, dim, m) != 0) return -1;
+  memset(shifts, 0, sizeof(shifts));
+  uint32_t scr[KO_MAX_DIM * KO_SOBOL_M];
+  memcpy(scr, Cs, sizeof(uint32_t) * dim * m);
+  if (do_shift || do_scramble) {
+    ko_mt64 g;
+    ko_mt64_seed(&g, (uint64_t)rng_seed_shift);
+    for (int j = 0; j < dim; j++) shifts[j] = (uint32_t)ko_mt64_next(&g);
+    if (!do_shift) memset(shifts, 0, sizeof(shifts));
+    if (do_scramble) {
+      /* bit depth t of the JK2008 columns (bit_depth, digitalseq_b2g.hpp:130-138) */
+      uint32_t mx = 0;
+      for (int i = 0; i < dim * m; i++)
+        if (Cs[i] > mx) mx = Cs[i];
+      unsigned t = 31;
+      while (t && !((1u << (t - 1)) & mx)) t--;
+      ko_mt64 g2;
+      ko_mt64_seed(&g2, (uint64_t)rng_seed_shift);
+      uint32_t M[KO_MAX_DIM * 32];
+      for (int j = 0; j < dim; j++)
+        for (unsigned k = 0; k < t; k++) {
+          uint64_t r = ko_mt64_next(&g2);
+          uint32_t hb = 1u << (t - k - 1);
+          r &= (uint64_t)((hb - 1u) | hb);
+          r |= 1u;
+          r <<= k;
+          M[j * t + k] = (uint32_t)r;
+        }
+      for (int j = 0; j < dim; j++)
+        for (int c = 0; c < m; c++) {
+          uint32_t out = 0;
+          for (unsigned i = 0; i < t; i++)
+            if ((Cs[j * m + c] >> i) & 1u) out ^= M[j * t + i];
+          scr[j * m + c] = out;
+        }
+    }
+  }
+  s->dim = dim;
+  for (int i = 0; i < dim * m; i++) s->Csr[i] = bitrev32(scr[i]);
+  for (int j = 0; j < dim; j++) s->shfr[j] = bitrev32(shifts[j]);
+  ko_sobol_set_state(s, state_seed);
+  return 0;
+}
+
+/* digitalseq_b2g::set_state (digitalseq_b2g.hpp:455-471) */
+void ko_sobol_set_state(ko_sobol *s, uint32_t k) {
+  s->k = k;
+  uint32_t gk = (k >> 1) ^ k;
+  for (int j = 0; j < s->dim; j++) {
+    uint32_t c = 0;
+    if (k != 0)
+      for (int i = 0; i < KO_SOBOL_M; i++)
+        if ((1u << i) & gk) c ^= s->Csr[j * KO_SOBOL_M + i];
+    s->cur[j] = c;
+  }
+}
+
+static unsigned ctz32(uint32_t v) { return v ? (unsigned)__builtin_ctz(v) : 32u; }
+
+/* sobol::operator() + digitalseq_b2g::next (qrng.hpp:51-58, digitalseq_b2g.hpp:415-426) */
+int ko_sobol_next(ko_sobol *s, double *x) {
+  if (s->k >= (1u << KO_SOBOL_M)) return -1;
+  s->k += 1;
+  unsigned c = ctz32(s->k) % KO_SOBOL_M;
+  for (int j = 0; j < s->dim; j++) {
+    s->cur[j] ^= s->Csr[j * KO_SOBOL_M + c];
+    x[j] = (double)(s->cur[j] ^ s->shfr[j]) * (1.0 / 4294967296.0);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* g0 on the Keldysh contour                                                 */
+/* ------------------------------------------------------------------------ */
+/* common.cpp:47-57 */
+int ko_compare_3way(const ko_gf_index *a, const ko_gf_index *b) {
+  int kd = a->k_idx - b->k_idx;
+  if (kd != 0) return kd;
+  double dt = a->time - b->time;
+  if (dt != 0.0) return (1 - 2 * a->k_idx) * (2 * (!signbit(dt)) - 1);
+  return (1 - 2 * a->k_idx) * (a->timesplit - b->timesplit);
+}
+
+static void table_eval(const ko_g0 *g, const double *tab, int oa, int ob, double t, double out[2]) {
+  const double *p = tab + ((size_t)(oa * g->n_orb + ob) * g->n_pts) * 2;
+  double a = (t - g->t_min) / g->dt;
+  double fl = floor(a);
+  long i = (long)fl;
+  if (i > g->n_pts - 2) i = g->n_pts - 2;
+  if (i < 0) i = 0;
+  double w = a - (double)i;
+  out[0] = (1.0 - w) * p[2 * i] + w * p[2 * i + 2];
+  out[1] = (1.0 - w) * p[2 * i + 1] + w * p[2 * i + 3];
+}
+
+/* g0_keldysh_contour_t::operator() (model.hpp:244-265) */
+void ko_g0_eval(const ko_g0 *g, const ko_gf_index *a, const ko_gf_index *b, int internal_point,
+                double out[2]) {
+  if (a->spin != b->spin) {
+    out[0] = out[1] = 0.0;
+    return;
+  }
+  int order = ko_compare_3way(a, b);
+  if (order > 0) {
+    table_eval(g, g->greater, a->orbital, b->orbital, a->time - b->time, out);
+    return;
+  }
+  if (order < 0) {
+    table_eval(g, g->lesser, a->orbital, b->orbital, a->time - b->time, out);
+    return;
+  }
+  table_eval(g, g->lesser, a->orbital, b->orbital, 0.0, out);
+  if (a->orbital == b->orbital) out[1] -= (double)internal_point * g->alpha;
+}
+
+/* ------------------------------------------------------------------------ */
+/* warpers                                                                   */
+/* ------------------------------------------------------------------------ */
+static long interp_find(const double *xa, int n, double x) {
+  /* gsl_interp_bsearch over [0, n-1] */
+  long ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    long i = (ihi + ilo) / 2;
+    if (xa[i] > x)
+      ihi = i;
+    else
+      ilo = i;
+  }
+  return ilo;
+}
+
+static double linear_eval(const double *xa, const double *ya, int n, double x) {
+  if (x < xa[0] || x > xa[n - 1]) return NAN;
+  long i = interp_find(xa, n, x);
+  double dx = xa[i + 1] - xa[i];
+  if (!(dx > 0.0)) return 0.0;
+  return ya[i] + (x - xa[i]) / dx * (ya[i + 1] - ya[i]);
+}
+
+static double steffen_sign(double x, double y) {
+  if ((x < 0 && y > 0) || (x > 0 && y < 0)) return -x;
+  return x;
+}
+
+void ko_steffen_coeffs(const double *x, const double *y, int n, double *co) {
+  double *yp = (double *)malloc(sizeof(double) * n);
+  double h0 = x[1] - x[0];
+  yp[0] = (y[1] - y[0]) / h0;
+  for (int i = 1; i < n - 1; i++) {
+    double hi = x[i + 1] - x[i], him1 = x[i] - x[i - 1];
+    double si = (y[i + 1] - y[i]) / hi, sim1 = (y[i] - y[i - 1]) / him1;
+    double pi = (sim1 * hi + si * him1) / (him1 + hi);
+    double mn = fabs(sim1);
+    if (fabs(si) < mn) mn = fabs(si);
+    if (0.5 * fabs(pi) < mn) mn = 0.5 * fabs(pi);
+    yp[i] = (steffen_sign(1.0, sim1) + steffen_sign(1.0, si)) * mn;
+  }
+  yp[n - 1] = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+  for (int i = 0; i < n - 1; i++) {
+    double hi = x[i + 1] - x[i];
+    double si = (y[i + 1] - y[i]) / hi;
+    co[4 * i + 0] = (yp[i] + yp[i + 1] - 2 * si) / hi / hi;
+    co[4 * i + 1] = (3 * si - 2 * yp[i] - yp[i + 1]) / hi;
+    co[4 * i + 2] = yp[i];
+    co[4 * i + 3] = y[i];
+  }
+  free(yp);
+}
+
+double ko_steffen_eval(const double *x, const double *co, int n, double xv) {
+  if (xv < x[0] || xv > x[n - 1]) return NAN;
+  long i = interp_find(x, n, xv);
+  double d = xv - x[i];
+  return co[4 * i + 3] + d * (co[4 * i + 2] + d * (co[4 * i + 1] + d * co[4 * i + 0]));
+}
+
+static double hybrid_f_user(const ko_map1d *m, double t) {
+  double num = m->w, den = m->w + t;
+  for (int k = 1; k < m->power; k++) {
+    num *= m->w;
+  }
+  double d = den;
+  for (int k = 1; k < m->power; k++) d *= den;
+  return num / d;
+}
+
+/* 1D inverse-CDF map: x = F^{-1}(l), returns 1/f(x)
+ * (product_1d_simple.hpp:109-117, 320-328, 495-503 and the makers 560-630) */
+double ko_map1d_reverse(const ko_map1d *m, double l, double *u) {
+  double x, f;
+  double w = m->w, um = m->u_max;
+  switch (m->kind) {
+  case KO_MAP_UNIFORM:
+    x = um * l;
+    f = 1. / um;
+    break;
+  case KO_MAP_EXPONENTIAL: {
+    x = um - w * log1p(expm1(um / w) * l);
+    long double norm = -expm1l(-(long double)um / w);
+    f = (double)(exp(-x / w) / (w * norm));
+    break;
+  }
+  case KO_MAP_INVERSE: {
+    long double norm = log1pl((long double)um / w);
+    x = (double)(w * expm1l(l * norm));
+    f = (double)(1.0 / (norm * (w + x)));
+    break;
+  }
+  case KO_MAP_INVERSE_SQ: {
+    long double norm = (long double)um / (w + um);
+    x = (double)(norm * l * w / (1.0 - norm * l));
+    f = (double)(w / ((w + x) * (w + x) * norm));
+    break;
+  }
+  case KO_MAP_INVERSE_POW: {
+    int p = m->power;
+    long double umax = um;
+    long double denom = powl(1 + l * (powl(w / (umax + w), p - 1) - 1), (long double)(1.0 / (p - 1.0)));
+    x = (double)(w * (1 - denom) / denom);
+    long double norm = (p - 1) * powl(umax + w, p - 1) / (powl(umax / w + 1, p - 1) - 1);
+    f = (double)(norm / pow(w + x, p));
+    break;
+  }
+  case KO_MAP_NEAREST: {
+    x = linear_eval(m->F, m->t, m->n_inv, -l);
+    long i = interp_find(m->t, m->n_pts, x);
+    f = 0.5 * (m->f[i] + m->f[i + 1]);
+    break;
+  }
+  case KO_MAP_HYBRID: {
+    x = ko_steffen_eval(m->F, m->inv_c, m->n_inv, -l);
+    f = hybrid_f_user(m, x) / m->norm;
+    break;
+  }
+  default:
+    x = NAN;
+    f = NAN;
+  }
+  *u = x;
+  return 1.0 / f;
+}
+
+/* warper_train_t::map_reverse(li, size, 0) (warpers.hpp:72-85) */
+double ko_warp_reverse(const ko_warper *w, int count, int n, const double *l, double *u) {
+  double cur[KO_MAX_DIM];
+  double jac = 1.0;
+  memcpy(cur, l, sizeof(double) * n);
+  for (int q = count - 1; q >= 0; q--) {
+    const ko_warper *wq = &w[q];
+    if (wq->kind == KO_WARP_PLASMA) {
+      /* ui_from_vi (plasma_uv.hpp:46-51) */
+      cur[0] = wq->t_max - cur[0];
+      for (int i = 1; i < n; i++) cur[i] = cur[i - 1] - cur[i];
+    } else if (wq->kind == KO_WARP_PRODUCT) {
+      for (int i = 0; i < n; i++) {
+        double x;
+        double jr = ko_map1d_reverse(&wq->maps[wq->n_maps == 1 ? 0 : i], cur[i], &x);
+        cur[i] = x;
+        jac *= jr;
+      }
+    }
+  }
+  memcpy(u, cur, sizeof(double) * n);
+  return jac;
+}
+
+/* ------------------------------------------------------------------------ */
+/* integrand                                                                 */
+/* ------------------------------------------------------------------------ */
+void ko_det(double *ad, int m, double out[2]) {
+  cplx *a = (cplx *)ad;
+  cplx det = 1.0;
+  int sign = 1;
+  for (int k = 0; k < m; k++) {
+    int p = k;
+    double best = fabs(creal(a[k * m + k])) + fabs(cimag(a[k * m + k]));
+    for (int r = k + 1; r < m; r++) {
+      double v = fabs(creal(a[r * m + k])) + fabs(cimag(a[r * m + k]));
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (a[p * m + k] == 0.0) {
+      out[0] = out[1] = 0.0;
+      return;
+    }
+    if (p != k) {
+      for (int j = 0; j < m; j++) {
+        cplx t = a[k * m + j];
+        a[k * m + j] = a[p * m + j];
+        a[p * m + j] = t;
+      }
+      sign = -sign;
+    }
+    cplx inv = 1.0 / a[k * m + k];
+    for (int r = k + 1; r < m; r++) {
+      cplx l = a[r * m + k] * inv;
+      for (int j = k + 1; j < m; j++) a[r * m + j] -= l * a[k * m + j];
+    }
+    det *= a[k * m + k];
+  }
+  det *= sign;
+  out[0] = creal(det);
+  out[1] = cimag(det);
+}
+
+static cplx g0c(const ko_g0 *g, const ko_gf_index *a, const ko_gf_index *b, int internal) {
+  double o[2];
+  ko_g0_eval(g, a, b, internal, o);
+  return o[0] + I * o[1];
+}
+
+/* integrand_g_direct::operator() (wick_direct.cpp:43-158) */
+int ko_integrand_eval(const ko_integrand *f, int n, const double *times, int keep_u_hypercube,
+                      double out[2]) {
+  out[0] = out[1] = 0.0;
+  if (f->A.spin != f->B.spin) return 0;
+  if (keep_u_hypercube)
+    for (int i = 0; i < n; i++)
+      if (times[i] < 0.0) return 0;
+  ko_gf_index a = f->A, b = f->B;
+  a.timesplit = n;
+  b.timesplit = n;
+  if (n == 0) {
+    double o[2];
+    ko_g0_eval(&f->g0, &a, &b, 0, o);
+    out[0] = o[0];
+    out[1] = o[1];
+    return 1;
+  }
+  int n2 = 2 * n, ext = 2 * n, d1 = 2 * n + 1;
+  ko_gf_index c1[2 * KO_MAX_DIM], c2[2 * KO_MAX_DIM];
+  for (int i = 0; i < n; i++) {
+    ko_gf_index v = {times[i], 0, i, a.spin, 0};
+    c1[i] = v;
+    v.k_idx = 1;
+    c1[i + n] = v;
+    v.spin = 1 - a.spin;
+    v.k_idx = 0;
+    c2[i] = v;
+    v.k_idx = 1;
+    c2[i + n] = v;
+  }
+  cplx *W1 = (cplx *)malloc(sizeof(cplx) * d1 * d1);
+  cplx *W2 = (cplx *)malloc(sizeof(cplx) * n2 * n2);
+  W1[ext * d1 + ext] = g0c(&f->g0, &a, &b, 0);
+  for (int i = 0; i < n2; i++) {
+    W1[ext * d1 + i] = g0c(&f->g0, &a, &c1[i], 1);
+    W1[i * d1 + ext] = g0c(&f->g0, &c1[i], &b, 1);
+    for (int j = 0; j < n2; j++) {
+      W1[i * d1 + j] = g0c(&f->g0, &c1[i], &c1[j], 1);
+      W2[i * n2 + j] = g0c(&f->g0, &c2[i], &c2[j], 1);
+    }
+  }
+  cplx sum = 0.0;
+  cplx *M1 = (cplx *)malloc(sizeof(cplx) * (n + 1) * (n + 1));
+  cplx *M2 = (cplx *)malloc(sizeof(cplx) * n * n);
+  int pick[KO_MAX_DIM + 1];
+  uint64_t nconf = (uint64_t)1 << n;
+  for (uint64_t idx = 0; idx < nconf; idx++) {
+    for (int i = 0; i < n; i++) pick[i] = i + (int)((idx >> i) & 1u) * n;
+    pick[n] = ext;
+    for (int i = 0; i <= n; i++)
+      for (int j = 0; j <= n; j++) M1[i * (n + 1) + j] = W1[pick[i] * d1 + pick[j]];
+    M1[n * (n + 1) + n] = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) M2[i * n + j] = W2[pick[i] * n2 + pick[j]];
+    double d1v[2], d2v[2];
+    ko_det((double *)M1, n + 1, d1v);
+    ko_det((double *)M2, n, d2v);
+    int par = 1 - 2 * __builtin_parityll(idx);
+    sum += (double)par * (d1v[0] + I * d1v[1]) * (d2v[0] + I * d2v[1]);
+  }
+  free(W1);
+  free(W2);
+  free(M1);
+  free(M2);
+  if (cabs(sum) < f->cutoff) sum = 0.0;
+  out[0] = creal(sum);
+  out[1] = cimag(sum);
+  return 1;
+}
+
+/* integrand_g_direct::eval_no_det (wick_direct.cpp:161-228) */
+int ko_integrand_eval_no_det(const ko_integrand *f, int n, const double *times,
+                             int keep_u_hypercube, double out[2]) {
+  out[0] = out[1] = 0.0;
+  if (f->A.spin != f->B.spin) return 0;
+  if (keep_u_hypercube)
+    for (int i = 0; i < n; i++)
+      if (times[i] < 0.0) return 0;
+  ko_gf_index a = f->A, b = f->B;
+  a.timesplit = n;
+  b.timesplit = n;
+  const ko_g0 *g = &f->g0;
+  cplx res = 0.0;
+  if (n == 0) {
+    res = g0c(g, &a, &b, 0);
+  } else if (n == 1) {
+    for (int k0 = 0; k0 < 2; k0++) {
+      ko_gf_index up = {times[0], k0, 0, 0, 0}, dn = {times[0], k0, 0, 1, 0};
+      int sign = k0 == 0 ? 1 : -1;
+      res += -(double)sign * g0c(g, &dn, &dn, 1) * g0c(g, &a, &up, 1) * g0c(g, &up, &b, 1);
+    }
+  } else if (n == 2) {
+    for (int k0 = 0; k0 < 2; k0++)
+      for (int k1 = 0; k1 < 2; k1++) {
+        ko_gf_index u0 = {times[0], k0, 0, 0, 0}, u1 = {times[1], k1, 1, 0, 0};
+        ko_gf_index d0 = {times[0], k0, 0, 1, 0}, d1 = {times[1], k1, 1, 1, 0};
+        int sign = ((k0 + k1) % 2 == 0) ? 1 : -1;
+        res += (double)sign *
+               (g0c(g, &u0, &u0, 1) * g0c(g, &a, &u1, 1) * g0c(g, &u1, &b, 1) +
+                g0c(g, &a, &u0, 1) * g0c(g, &u1, &u1, 1) * g0c(g, &u0, &b, 1)) *
+               g0c(g, &d1, &d0, 1) * g0c(g, &d0, &d1, 1);
+        res += (double)sign *
+               (g0c(g, &a, &u0, 1) * g0c(g, &u0, &u1, 1) * g0c(g, &u1, &b, 1) +
+                g0c(g, &u1, &u0, 1) * g0c(g, &a, &u1, 1) * g0c(g, &u0, &b, 1)) *
+               (g0c(g, &d0, &d0, 1) * g0c(g, &d1, &d1, 1) - g0c(g, &d1, &d0, 1) * g0c(g, &d0, &d1, 1));
+      }
+  } else {
+    return -1;
+  }
+  if (cabs(res) < f->cutoff) res = 0.0;
+  out[0] = creal(res);
+  out[1] = cimag(res);
+  return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* integrator                                                                */
+/* ------------------------------------------------------------------------ */
+/* integrator::evaluate_warped_integrand (integrator.hpp:68-79) */
+int ko_eval_warped(const ko_integrand *f, const ko_warper *w, int nw, int n, const double *l,
+                   double out[2]) {
+  double u[KO_MAX_DIM];
+  double jac = ko_warp_reverse(w, nw, n, l, u);
+  int dom = ko_integrand_eval(f, n, u, 1, out);
+  out[0] *= jac;
+  out[1] *= jac;
+  return dom;
+}
+
+/* integrator::run (integrator.hpp:81-95) */
+int ko_run(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint64_t n_steps,
+           ko_accum *acc) {
+  double l[KO_MAX_DIM], v[2];
+  int n = s->dim;
+  for (uint64_t i = 0; i < n_steps; i++) {
+    if (ko_sobol_next(s, l) != 0) return -1;
+    int dom = ko_eval_warped(f, w, nw, n, l, v);
+    acc->n_in_domain += (uint64_t)dom;
+    acc->re += v[0];
+    acc->im += v[1];
+    acc->n_points++;
+  }
+  return 0;
+}
+
+int ko_run_parallel(const ko_integrand *f, const ko_warper *w, int nw, const ko_sobol *s0,
+                    uint64_t n_steps, int threads, ko_accum *acc) {
+  if (threads < 1) threads = 1;
+  ko_accum *part = (ko_accum *)calloc((size_t)threads, sizeof(ko_accum));
+  int err = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static, 1) reduction(| : err)
+#endif
+  for (int t = 0; t < threads; t++) {
+    uint64_t lo = n_steps * (uint64_t)t / (uint64_t)threads;
+    uint64_t hi = n_steps * (uint64_t)(t + 1) / (uint64_t)threads;
+    ko_sobol s = *s0;
+    ko_sobol_set_state(&s, s0->k + (uint32_t)lo);
+    if (ko_run(f, w, nw, &s, hi - lo, &part[t]) != 0) err |= 1;
+  }
+  for (int t = 0; t < threads; t++) {
+    acc->re += part[t].re;
+    acc->im += part[t].im;
+    acc->n_points += part[t].n_points;
+    acc->n_in_domain += part[t].n_in_domain;
+  }
+  free(part);
+  return err ? -1 : 0;
+}

@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark: wall time for the charge Q_8(t=10) to reach 1e-3 relative error
+on one MI355X.
+
+Workload (the configuration the metric is quoted on): single-orbital Anderson
+impurity, flat-band bath Gamma=1, T=0, V=0, eps_d=0, g0 tabulated analytically
+on 10^4 (->10001) time points in [-10, 10]; observable compute_charge_Q_direct
+at time 10, order 8.  Estimator: randomised QMC -- each step is one replica of
+`--points` scrambled Sobol points (rng_seed_shift = replica index) pushed
+through the warper chain and the Keldysh-sum kernel on the device.
+
+Plain warper chain (the reference's fitted product-1D recipe):
+    [plasma_uv(10), product_1d_simple_inverse(10, 1.0),
+     make_warper_from_proj(order 8, 500 bins, 2^14 pilot points, sigma 0.1)]
+
+value = (points needed for 1e-3 relative error) / (points per second).  The
+points needed come from a doubling search: 16 scrambled replicas at M points,
+M = 2^12, 2^13, ... until the relative spread of one replica's estimate is
+<= 1e-3, then log-log interpolation between the last two M.  Points per second
+come from the timed steps (`--points` per step).  The one-off set-up (g0
+tabulation and the pilot fit) is reported separately as setup_s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
+TARGET_REL = 1e-3
+
+
+def model_params():
+    return dict(beta=-1.0, bias_V=0.0, eps_d=0.0, Gamma=1.0, alpha=0.0, half_bandwidth=10.0,
+                bath_type="flatband", time_max=10.0, nr_time_points_gf=10000, ft_method="analytic")
+
+
+def plain_computer(order, t_max, pilot_points, precision):
+    """The plain estimator: the reference's fitted product-1D recipe
+    (projection_test.py:77-98): the pilot evaluates the warped integrand
+    without the positive-time restriction on a g0 window of order * t_max."""
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+
+    model = kio.G0Model(kio.G0ModelOmega(model_params()), False)
+    wide = dict(model_params(), time_max=order * t_max, nr_time_points_gf=order * 10000)
+    pilot = kio.ComputeChargeQDirect(kio.G0Model(kio.G0ModelOmega(wide), False), t_max, order, 0.0)
+    pilot.warper.emplace_back(kw.WarperPlasmaUvT(t_max))
+    pilot.warper.emplace_back(kw.make_product_1d_simple_inverse(t_max, 1.0))
+    proj = kw.make_warper_from_proj(pilot, order, 500, pilot_points, 0.1, keep_u_hypercube=False)
+    comp = kio.ComputeChargeQDirect(model, t_max, order, 0.0)
+    comp.set_precision(precision)
+    comp.warper.emplace_back(kw.WarperPlasmaUvT(t_max))
+    comp.warper.emplace_back(kw.make_product_1d_simple_inverse(t_max, 1.0))
+    comp.warper.emplace_back(proj)
+    return model, comp, proj
+
+
+def run_replicas(comp, first_seed, steps, points):
+    """Run `steps` RQMC replicas (scramble seeds first_seed, first_seed+1, ...);
+    returns per-replica estimates, in-domain counts, LU-kernel ms and the
+    per-replica wall times (device work included)."""
+    import numpy as np
+
+    est, dom, lu_ms, walls = [], [], [], []
+    for r in range(steps):
+        comp.reset_rng("sobol", 0, True, True, first_seed + r)
+        n0 = comp.reduce_nr_points_run()
+        s0 = comp.reduce_result() * n0 if n0 else 0j
+        d0 = comp.reduce_nr_points_in_domain()
+        t0 = time.perf_counter()
+        comp.run(points)
+        walls.append(time.perf_counter() - t0)
+        s1 = comp.reduce_result() * comp.reduce_nr_points_run()
+        est.append((s1 - s0) / points)
+        dom.append(comp.reduce_nr_points_in_domain() - d0)
+        lu_ms.append(comp.last_timing()[1])
+    return np.array(est), np.array(dom), np.array(lu_ms), np.array(walls)
+
+
+def points_for_target(comp, replicas, target, m0=1 << 12, m_max=1 << 22):
+    """Doubling search: RQMC spread over `replicas` scrambles at M points per
+    replica until the relative spread of one replica falls below `target`.
+    Returns (points needed by log-log interpolation, table of (M, rel spread),
+    median wall time of one replica at the first M that reached the target)."""
+    import numpy as np
+
+    table = []
+    m = m0
+    while True:
+        est, _, _, walls = run_replicas(comp, 10_000 + m, replicas, m)
+        rel = est.std(ddof=1) / abs(est.mean())
+        table.append((m, rel, float(np.median(walls))))
+        if rel <= target or m >= m_max:
+            break
+        m *= 2
+    if len(table) == 1:
+        m1, r1, _ = table[0]
+        return m1 * (r1 / target) ** 2, table, table[-1][2]
+    (m1, r1, _), (m2, r2, _) = table[-2], table[-1]
+    slope = np.log(r2 / r1) / np.log(m2 / m1)  # ~ -0.5 (MC) ... -1 (QMC)
+    n = m2 * (target / r2) ** (1.0 / slope) if slope < 0 else m2 * (r2 / target) ** 2
+    return float(n), table, table[-1][2]
+
+
+def cpu_baseline(model, proj, order, t_max, gpu_points_needed, seconds):
+    """The CPU oracle (a serial-per-point C port of the reference path) on the
+    same warper chain, timed on the host cores."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    cores = min(os.cpu_count() or 1, 16)
+    g0 = O.G0Table(model.g0_lesser[0, 0], model.g0_greater[0, 0], model.time_max)
+    f = O.charge_integrand(g0, t_max)
+    maps = []
+    for a in range(order):
+        t, f1, F, cut = proj.get_1d_warper(a).knots()
+        maps.append(O.Map1d("nearest", 1.0, t=np.array(t), f=np.array(f1), F=np.array(F), n_inv=cut + 1))
+    train = O.Train([O.plasma(t_max), O.product(O.Map1d("inverse", t_max, w=1.0)), O.product(maps)])
+    calib = 4096
+    s = O.Sobol(order, 0, True, True, 0)
+    t0 = time.perf_counter()
+    O.run(f, train, s, calib, threads=cores)
+    dt = time.perf_counter() - t0
+    n = int(max(calib, min(1 << 22, calib * seconds / max(dt, 1e-6))))
+    s = O.Sobol(order, 0, True, True, 0)
+    t0 = time.perf_counter()
+    val, npts, ndom = O.run(f, train, s, n, threads=cores)
+    dt = time.perf_counter() - t0
+    pps = n / dt
+    return {"pps": pps, "cores": cores, "sample_points": n, "seconds": dt,
+            "value": gpu_points_needed / pps, "sum": val, "n_in_domain": ndom}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16, help="timed RQMC replicas")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed replicas")
+    ap.add_argument("--points", type=int, default=1 << 22, help="Sobol points per replica")
+    ap.add_argument("--order", type=int, default=8)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--pilot-points", type=int, default=1 << 14)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(points would shard by contiguous Sobol index ranges with one 32-byte all-reduce).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    t_max = 10.0
+    t0 = time.perf_counter()
+    model, comp, proj = plain_computer(args.order, t_max, args.pilot_points, args.precision)
+    setup_s = time.perf_counter() - t0
+    if args.warmup:
+        run_replicas(comp, 1000, args.warmup, args.points)
+    est, dom, lu_ms, walls = run_replicas(comp, 0, args.steps, args.points)
+    wall = float(walls.sum())
+
+    mean = est.mean()
+    n_total = args.steps * args.points
+    pps = n_total / wall
+    n_needed, table, wall_at_target = points_for_target(comp, 16, TARGET_REL)
+    t_needed = n_needed / pps
+    charge = (-1j * (1j ** args.order) * mean).real
+    rel_rep = est.std(ddof=1) / abs(mean) if len(est) > 1 else float("nan")
+
+    fpp = comp.flops_per_point()
+    flops = float(dom.sum()) * fpp
+    lu_s = lu_ms.sum() * 1e-3
+    achieved = flops / lu_s / 1e12
+    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+    launches_per_step = max(1, -(-args.points // (1 << 21)))
+
+    out = {
+        "metric": "time to 1e-3 rel. error on charge at n=8: learned vs plain warper, 1 GPU",
+        "value": t_needed,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-lu/f64-sum",
+        "data": "synthetic (scrambled Sobol points; analytic flat-band g0)",
+        "config": {"workload": "charge Q_n, n=8, t=10, flatband Gamma=1 T=0 V=0 eps_d=0, "
+                               "plain fitted product-1D warper (estimator: plain)",
+                   "order": args.order, "points_per_replica": args.points, "replicas": args.steps,
+                   "pilot_points": args.pilot_points},
+        "estimator": {"warper": "plain", "charge_estimate": charge,
+                      "rel_err_per_replica_at_points": rel_rep,
+                      "points_needed_1e-3": n_needed, "points_per_s": pps,
+                      "rqmc_spread_table": [[m, r] for m, r, _ in table],
+                      "wall_s_one_replica_at_target": wall_at_target,
+                      "setup_s": setup_s,
+                      "in_domain_fraction": float(dom.sum()) / n_total},
+        "roofline": {"bound": "valu-fp64" if args.precision == "fp64" else "valu-fp32",
+                     "kernel": f"integrand_kernel<{args.order},{'double' if args.precision == 'fp64' else 'float'}>",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None, "flops_per_point": fpp,
+                     "lu_ms_per_launch": float(lu_ms.mean()) / launches_per_step},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(model, proj, args.order, t_max, n_needed, args.cpu_seconds)
+        out["cpu_baseline"] = {"value": cb["value"], "unit": "s", "cores": cb["cores"], "kind": "port",
+                               "sample": f"{cb['sample_points']} scrambled Sobol points (seed 0) of the same "
+                                         f"chain in {cb['seconds']:.1f} s ({cb['pps']:.0f} points/s)",
+                               "points_per_s": cb["pps"]}
+        out["gpu_vs_cpu"] = cb["value"] / t_needed
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

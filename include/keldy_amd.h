@@ -110,7 +110,8 @@ int kd_set_integrand(kd_ctx *ctx, int32_t order, const kd_gf_index *A, const kd_
 int kd_set_precision(kd_ctx *ctx, int32_t precision);
 
 /* Integrand at n_points vertex-time vectors u (row-major n_points x order, host memory).
- * out: n_points x (re, im); in_domain: n_points (may be NULL). */
+ * out: n_points x (re, im); in_domain: n_points (may be NULL).  A g0 lookup
+ * outside [-time_max, time_max] yields NaN (the reference throws). */
 int kd_eval_batch(kd_ctx *ctx, const double *u, int64_t n_points, int32_t keep_u_hypercube,
                   double *out, int32_t *in_domain);
 /* Same with device pointers (inputs already in HBM); asynchronous on the ctx stream. */
@@ -122,9 +123,12 @@ int kd_eval_batch_device(kd_ctx *ctx, const double *d_u, int64_t n_points, int32
 int kd_set_warper_chain(kd_ctx *ctx, const kd_warper_desc *warpers, int32_t n_warpers,
                         const kd_map1d *maps, int32_t n_maps, const double *tables,
                         int64_t n_tables);
-/* Warped integrand (times Jacobian) at n_points hypercube points l (host memory). */
-int kd_eval_warped_batch(kd_ctx *ctx, const double *l, int64_t n_points, double *out,
-                         int32_t *in_domain, double *jacobian);
+/* Warped integrand (times Jacobian) at n_points hypercube points l (host memory).
+ * keep_u_hypercube = 0 evaluates also at negative vertex times (as the
+ * reference's evaluate_warped_integrand(l, start, false)).  A g0 lookup
+ * outside [-time_max, time_max] yields NaN (the reference throws). */
+int kd_eval_warped_batch(kd_ctx *ctx, const double *l, int64_t n_points, int32_t keep_u_hypercube,
+                         double *out, int32_t *in_domain, double *jacobian);
 
 /* Sobol generator: Csr = bit-reversed (scrambled) generating matrices [dim][31],
  * shfr = bit-reversed digital shift [dim]; state k (next point is k+1). */
@@ -141,9 +145,10 @@ int kd_reset_accum(kd_ctx *ctx);
  * (advances the generator): l (n x order), value (n x 2). Device pointers. */
 int kd_sample_device(kd_ctx *ctx, uint64_t n_steps, double *d_l, double *d_value);
 
-/* Milliseconds of the last kd_run's integrand kernel(s), timed with HIP events
- * on the ctx stream, and the number of kernel launches it made. */
-int kd_last_kernel_ms(const kd_ctx *ctx, double *ms, int32_t *launches);
+/* Timing of the last kd_run / kd_eval_* / kd_sample_device call, from HIP
+ * events on the ctx stream: whole call, the Keldysh-sum (LU) kernels and the
+ * point-preparation kernels, and the number of kernel launches made. */
+int kd_last_timing(const kd_ctx *ctx, double *total_ms, double *lu_ms, double *prep_ms, int32_t *launches);
 /* Executed FP operations per point of the last configured order/precision
  * (the algorithm's own count, used for roofline figures). */
 int kd_flops_per_point(const kd_ctx *ctx, double *flops);

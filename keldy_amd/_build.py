@@ -33,6 +33,7 @@ HOST_SOURCES = [
     "host/quadrature.cpp",
     "host/warpers.cpp",
     "host/qrng.cpp",
+    "host/integrator.cpp",
     "bindings/py_keldy.cpp",
 ]
 
@@ -88,7 +89,7 @@ def build_host_ext():
            f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
     out = host_ext_path()
     tmp = out + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", *inc, *srcs,
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-fvisibility=hidden", *inc, *srcs,
            f"-L{LIBDIR}", "-lkeldy_amd", "-Wl,-rpath,$ORIGIN/lib", "-o", tmp]
     _run(cmd)
     os.replace(tmp, out)

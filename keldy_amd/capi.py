@@ -48,7 +48,7 @@ class Accum(C.Structure):
 SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_set_g0_table",
            "kd_set_integrand", "kd_set_precision", "kd_eval_batch", "kd_eval_batch_device",
            "kd_set_warper_chain", "kd_eval_warped_batch", "kd_set_sobol", "kd_get_sobol_state",
-           "kd_run", "kd_get_accum", "kd_reset_accum", "kd_sample_device", "kd_last_kernel_ms",
+           "kd_run", "kd_get_accum", "kd_reset_accum", "kd_sample_device", "kd_last_timing",
            "kd_flops_per_point"]
 
 _lib = None
@@ -75,14 +75,14 @@ def lib():
         L.kd_eval_batch_device.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
         L.kd_set_warper_chain.argtypes = [vp, C.POINTER(WarperDesc), C.c_int32, C.POINTER(Map1d),
                                           C.c_int32, dp, C.c_int64]
-        L.kd_eval_warped_batch.argtypes = [vp, dp, C.c_int64, dp, ip, dp]
+        L.kd_eval_warped_batch.argtypes = [vp, dp, C.c_int64, C.c_int32, dp, ip, dp]
         L.kd_set_sobol.argtypes = [vp, C.c_int32, up, up, C.c_uint32]
         L.kd_get_sobol_state.argtypes = [vp, up]
         L.kd_run.argtypes = [vp, C.c_uint64, C.POINTER(Accum)]
         L.kd_get_accum.argtypes = [vp, C.POINTER(Accum)]
         L.kd_reset_accum.argtypes = [vp]
         L.kd_sample_device.argtypes = [vp, C.c_uint64, vp, vp]
-        L.kd_last_kernel_ms.argtypes = [vp, dp, ip]
+        L.kd_last_timing.argtypes = [vp, dp, dp, dp, ip]
         L.kd_flops_per_point.argtypes = [vp, dp]
         _lib = L
     return _lib
@@ -166,13 +166,13 @@ class Context:
                                         dom.ctypes.data_as(C.POINTER(C.c_int32))))
         return out[:, 0] + 1j * out[:, 1], dom
 
-    def eval_warped_batch(self, l):
+    def eval_warped_batch(self, l, keep_u_hypercube=True):
         l = np.ascontiguousarray(l, dtype=np.float64)
         n = l.shape[0]
         out = np.empty((n, 2))
         dom = np.empty(n, dtype=np.int32)
         jac = np.empty(n)
-        self._check(lib().kd_eval_warped_batch(self.h, _dp(l), n, _dp(out),
+        self._check(lib().kd_eval_warped_batch(self.h, _dp(l), n, int(keep_u_hypercube), _dp(out),
                                                dom.ctypes.data_as(C.POINTER(C.c_int32)), _dp(jac)))
         return out[:, 0] + 1j * out[:, 1], dom, jac
 
@@ -194,11 +194,12 @@ class Context:
         self._check(lib().kd_get_sobol_state(self.h, C.byref(k)))
         return k.value
 
-    def last_kernel_ms(self):
-        ms = C.c_double()
+    def last_timing(self):
+        """(total_ms, lu_ms, prep_ms, launches) of the last device call."""
+        t, lu, pr = C.c_double(), C.c_double(), C.c_double()
         n = C.c_int32()
-        self._check(lib().kd_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
+        self._check(lib().kd_last_timing(self.h, C.byref(t), C.byref(lu), C.byref(pr), C.byref(n)))
+        return t.value, lu.value, pr.value, n.value
 
     def flops_per_point(self):
         f = C.c_double()

@@ -1,0 +1,388 @@
+// py_keldy.cpp -- Python bindings of the host API with the reference's Python
+// names (python/keldy/{common,warpers,impurity_oneband}_desc.py).
+#include <pybind11/complex.h>
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "../host/integrator.hpp"
+#include "../host/model.hpp"
+#include "../host/qrng.hpp"
+#include "../host/warpers.hpp"
+
+namespace py = pybind11;
+using namespace keldy;
+using namespace keldy::impurity_oneband;
+using namespace keldy::warpers;
+
+namespace {
+
+// model_param_t from a dict with the reference's keys and defaults
+// (impurity_oneband_desc.py:576-631).
+model_param_t params_from_dict(py::dict d) {
+  model_param_t p;
+  static const char *keys[] = {"beta",          "bias_V",         "eps_d",    "Gamma",           "alpha",
+                               "half_bandwidth", "bath_type",     "time_max", "nr_time_points_gf", "ft_method"};
+  for (auto item : d) {
+    const std::string k = py::str(item.first);
+    bool known = false;
+    for (auto kk : keys) known = known || k == kk;
+    if (!known) throw keldy_error("model_param_t: unknown key '" + k + "'");
+  }
+  auto get = [&](const char *k, auto &v) {
+    if (d.contains(k)) v = d[k].cast<std::decay_t<decltype(v)>>();
+  };
+  get("beta", p.beta);
+  get("bias_V", p.bias_V);
+  get("eps_d", p.eps_d);
+  get("Gamma", p.Gamma);
+  get("alpha", p.alpha);
+  get("half_bandwidth", p.half_bandwidth);
+  get("bath_type", p.bath_type);
+  get("time_max", p.time_max);
+  get("nr_time_points_gf", p.nr_time_points_gf);
+  get("ft_method", p.ft_method);
+  return p;
+}
+
+py::dict params_to_dict(model_param_t const &p) {
+  py::dict d;
+  d["beta"] = p.beta;
+  d["bias_V"] = p.bias_V;
+  d["eps_d"] = p.eps_d;
+  d["Gamma"] = p.Gamma;
+  d["alpha"] = p.alpha;
+  d["half_bandwidth"] = p.half_bandwidth;
+  d["bath_type"] = p.bath_type;
+  d["time_max"] = p.time_max;
+  d["nr_time_points_gf"] = p.nr_time_points_gf;
+  d["ft_method"] = p.ft_method;
+  return d;
+}
+
+py::array_t<dcomplex> table_array(g0_model const &m, std::vector<dcomplex> const &t) {
+  py::array_t<dcomplex> a({m.n_orb, m.n_orb, m.n_pts});
+  std::copy(t.begin(), t.end(), a.mutable_data());
+  return a;
+}
+
+template <typename W, typename C> void bind_map_methods(C &c) {
+  c.def("map_reverse", &W::map_reverse, py::arg("li_vec"))
+      .def("map_forward", &W::map_forward, py::arg("ui_vec"))
+      .def("ui_from_li", &W::ui_from_li, py::arg("li_vec"))
+      .def("li_from_ui", &W::li_from_ui, py::arg("ui_vec"))
+      .def("jacobian_reverse", &W::jacobian_reverse, py::arg("li_vec"))
+      .def("jacobian_forward", &W::jacobian_forward, py::arg("ui_vec"));
+}
+
+template <typename W> void bind_product_1d(py::module_ &m, const char *name) {
+  py::class_<W> c(m, name);
+  c.def(py::init<>())
+      .def("reserve", &W::reserve)
+      .def("size", &W::size)
+      .def("emplace_back", &W::emplace_back, py::arg("w"))
+      .def("__call__", &W::operator());
+  bind_map_methods<W>(c);
+}
+
+template <typename T> void bind_integrator(py::module_ &m, const char *name) {
+  py::class_<T> c(m, name);
+  c.def(py::init<g0_model, double, int, double>(), py::arg("model"), py::arg("time"), py::arg("order"),
+        py::arg("cutoff_integrand"))
+      .def(py::init([](py::dict params, double time, int order, double cutoff) {
+             return new T(params_from_dict(params), time, order, cutoff);
+           }),
+           py::arg("params"), py::arg("time"), py::arg("order"), py::arg("cutoff_integrand"))
+      .def_readwrite("warper", &T::warper)
+      .def(
+          "evaluate_warped_integrand",
+          [](T &s, std::vector<double> li, int start, bool keep) { return s.evaluate_warped_integrand(li, start, keep); },
+          py::arg("li_vec"), py::arg("start_domain_nr"), py::arg("keep_u_hypercube") = true)
+      .def(
+          "evaluate_warped_integrand", [](T &s, std::vector<double> li) { return s.evaluate_warped_integrand(li); },
+          py::arg("li_vec"))
+      .def(
+          "evaluate_warped_integrand_batch",
+          [](T &s, py::array_t<double, py::array::c_style | py::array::forcecast> l, bool keep) {
+            if (l.ndim() != 2 || l.shape(1) != s.get_dimension())
+              throw keldy_error("expected an array of shape (n, order)");
+            const long n = long(l.shape(0));
+            py::array_t<dcomplex> out(n);
+            py::array_t<int> dom(n);
+            py::array_t<double> jac(n);
+            {
+              py::gil_scoped_release rel;
+              s.evaluate_warped_integrand_batch(l.data(), n, out.mutable_data(), dom.mutable_data(),
+                                                jac.mutable_data(), keep);
+            }
+            return py::make_tuple(out, dom, jac);
+          },
+          py::arg("li"), py::arg("keep_u_hypercube") = true)
+      .def("run", &T::run, py::arg("nr_steps"), py::call_guard<py::gil_scoped_release>())
+      .def("reset_rng", &T::reset_rng, py::arg("rng_name"), py::arg("rng_state_seed"), py::arg("do_shift") = false,
+           py::arg("do_scramble") = false, py::arg("rng_seed_shift") = 0)
+      .def("reduce_result", &T::reduce_result)
+      .def("reduce_nr_points_run", &T::reduce_nr_points_run)
+      .def("reduce_nr_points_in_domain", &T::reduce_nr_points_in_domain)
+      .def("reduce_nr_fallback", &T::reduce_nr_fallback)
+      .def("get_integrand", &T::get_integrand)
+      .def("get_warper", &T::get_warper)
+      .def(
+          "set_precision",
+          [](T &s, std::string const &p) {
+            if (p == "fp64")
+              s.set_precision(KD_PREC_FP64);
+            else if (p == "fp32")
+              s.set_precision(KD_PREC_FP32_LU);
+            else
+              throw keldy_error("precision must be 'fp64' or 'fp32'");
+          },
+          py::arg("precision"))
+      .def("last_timing", &T::last_timing)
+      .def("flops_per_point", &T::flops_per_point);
+}
+
+} // namespace
+
+PYBIND11_MODULE(_keldy_host, mod) {
+  mod.doc() = "keldy_amd host API (C++), device path through libkeldy_amd.so";
+  py::register_exception<keldy_error>(mod, "KeldyError", PyExc_RuntimeError);
+
+  // ---------------------------------------------------------------- common
+  py::module_ common = mod.def_submodule("common");
+  py::class_<contour_pt_t>(common, "ContourPtT")
+      .def(py::init<>())
+      .def_readwrite("time", &contour_pt_t::time)
+      .def_property(
+          "k_idx", [](contour_pt_t const &c) { return int(c.k_idx); },
+          [](contour_pt_t &c, int k) { c.k_idx = keldysh_idx_t(k); })
+      .def_readwrite("timesplit_n", &contour_pt_t::timesplit_n);
+  common.def("compare_3way", &compare_3way, py::arg("a"), py::arg("b"));
+  py::class_<sobol>(common, "Sobol")
+      .def(py::init<int, int, int>(), py::arg("dim"), py::arg("rng_state_seed"), py::arg("log_max_points_") = 31)
+      .def(py::init<int, int, bool, bool, int, int>(), py::arg("dim"), py::arg("rng_state_seed"), py::arg("do_shift"),
+           py::arg("do_scramble"), py::arg("rng_seed_shift"), py::arg("log_max_points_") = 31)
+      .def("__call__", &sobol::operator())
+      .def("seed", &sobol::seed, py::arg("k"))
+      .def("discard", &sobol::discard, py::arg("nr_discard"))
+      .def_property_readonly("state", &sobol::state)
+      .def("generating_tables", [](sobol const &s) { return py::make_tuple(s.csr(), s.shfr()); });
+  common.def("jk2008_matrices", &jk2008_matrices, py::arg("dim"), py::arg("m") = 31);
+
+  // --------------------------------------------------------------- warpers
+  py::module_ wm = mod.def_submodule("warpers");
+  {
+    py::class_<warper_identity_t> c(wm, "WarperIdentityT");
+    c.def(py::init<>());
+    bind_map_methods<warper_identity_t>(c);
+  }
+  {
+    py::class_<warper_plasma_uv_t> c(wm, "WarperPlasmaUvT");
+    c.def(py::init<double>(), py::arg("t_max_"));
+    bind_map_methods<warper_plasma_uv_t>(c);
+  }
+  {
+    py::class_<warper_product_1d_simple_t> c(wm, "WarperProduct1dSimpleT");
+    c.def(py::init<>())
+        .def(py::init<std::function<double(double)>, std::function<double(double)>, std::function<double(double)>,
+                      double, bool>(),
+             py::arg("f1_"), py::arg("f1_integrated_"), py::arg("f1_integrated_inverse_"), py::arg("domain_u_max_"),
+             py::arg("do_domain_checks") = true)
+        .def("__call__", &warper_product_1d_simple_t::operator());
+    bind_map_methods<warper_product_1d_simple_t>(c);
+  }
+  {
+    py::class_<warper_product_1d_simple_interp_nearest_t> c(wm, "WarperProduct1dSimpleInterpNearestT");
+    c.def(py::init<>())
+        .def(py::init<std::function<double(double)> const &, double, int>(), py::arg("f1_"), py::arg("domain_u_max_"),
+             py::arg("nr_sample_points_"))
+        .def(py::init<std::vector<double>, std::vector<double>>(), py::arg("times_u_pts_"), py::arg("f1_pts_"))
+        .def("__call__", &warper_product_1d_simple_interp_nearest_t::operator())
+        .def("knots",
+             [](warper_product_1d_simple_interp_nearest_t const &w) {
+               return py::make_tuple(w.m.knot_times(), w.m.knot_f1(), w.m.knot_F(), w.m.cut_index());
+             },
+             "(times, normalised f1, normalised negated CDF, cut index) of the tabulated map");
+    bind_map_methods<warper_product_1d_simple_interp_nearest_t>(c);
+  }
+  {
+    py::class_<warper_product_1d_simple_interp_hybrid_t> c(wm, "WarperProduct1dSimpleInterpHybridT");
+    c.def(py::init<>())
+        .def(py::init<std::function<double(double)>, double, int>(), py::arg("f1_"), py::arg("domain_u_max_"),
+             py::arg("nr_sample_points_"))
+        .def("__call__", &warper_product_1d_simple_interp_hybrid_t::operator());
+    bind_map_methods<warper_product_1d_simple_interp_hybrid_t>(c);
+  }
+  bind_product_1d<warper_product_1d_t>(wm, "WarperProduct1dT");
+  bind_product_1d<warper_product_1d_interp_hybrid_t>(wm, "WarperProduct1dInterpHybridT");
+  {
+    py::class_<warper_product_1d_interp_nearest_t> c(wm, "WarperProduct1dInterpNearestT");
+    c.def(py::init<>())
+        .def("reserve", &warper_product_1d_interp_nearest_t::reserve)
+        .def("size", &warper_product_1d_interp_nearest_t::size)
+        .def("emplace_back", &warper_product_1d_interp_nearest_t::emplace_back, py::arg("w"))
+        .def("get_1d_warper", &warper_product_1d_interp_nearest_t::get_1d_warper, py::arg("axis"))
+        .def("__call__", &warper_product_1d_interp_nearest_t::operator());
+    bind_map_methods<warper_product_1d_interp_nearest_t>(c);
+  }
+  {
+    py::class_<warper_train_t> c(wm, "WarperTrainT");
+    c.def(py::init<>())
+        .def("emplace_back", &warper_train_t::emplace_back<warper_identity_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_plasma_uv_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_simple_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_simple_interp_nearest_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_simple_interp_hybrid_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_interp_nearest_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_interp_hybrid_t>, py::arg("w"))
+        .def("reserve", &warper_train_t::reserve)
+        .def("clear", &warper_train_t::clear)
+        .def("size", &warper_train_t::size)
+        .def("map_reverse", py::overload_cast<vec const &, int, int>(&warper_train_t::map_reverse, py::const_),
+             py::arg("li_vec"), py::arg("start_domain_nr"), py::arg("end_domain_nr"))
+        .def("map_forward", py::overload_cast<vec const &, int, int>(&warper_train_t::map_forward, py::const_),
+             py::arg("ui_vec"), py::arg("start_domain_nr"), py::arg("end_domain_nr"))
+        .def("map_reverse", py::overload_cast<vec const &>(&warper_train_t::map_reverse, py::const_),
+             py::arg("li_vec"))
+        .def("map_forward", py::overload_cast<vec const &>(&warper_train_t::map_forward, py::const_),
+             py::arg("ui_vec"))
+        .def("jacobian_reverse", &warper_train_t::jacobian_reverse, py::arg("li_vec"))
+        .def("jacobian_forward", &warper_train_t::jacobian_forward, py::arg("ui_vec"))
+        .def("ui_from_li", &warper_train_t::ui_from_li, py::arg("li_vec"))
+        .def("li_from_ui", &warper_train_t::li_from_ui, py::arg("ui_vec"));
+  }
+  wm.def("vi_from_ui", &vi_from_ui, py::arg("t_max"), py::arg("u_times"));
+  wm.def("ui_from_vi", &ui_from_vi, py::arg("t_max"), py::arg("v_times"));
+  wm.def("make_product_1d_simple_exponential", &make_product_1d_simple_exponential, py::arg("domain_u_max"),
+         py::arg("w_scale"));
+  wm.def("make_product_1d_simple_inverse", &make_product_1d_simple_inverse, py::arg("domain_u_max"),
+         py::arg("w_scale"));
+  wm.def("make_product_1d_simple_inverse_square", &make_product_1d_simple_inverse_square, py::arg("domain_u_max"),
+         py::arg("w_scale"));
+  wm.def("make_product_1d_simple_inverse_power", &make_product_1d_simple_inverse_power, py::arg("power"),
+         py::arg("domain_u_max"), py::arg("w_scale"));
+  wm.def("make_product_1d_simple_uniform", &make_product_1d_simple_uniform, py::arg("domain_u_max"));
+  wm.def("make_product_1d_inverse_cube_alternate", &make_product_1d_inverse_cube_alternate, py::arg("order"),
+         py::arg("time"), py::arg("warper_scale"));
+  wm.def("make_product_1d_inverse_cube_alternate_interp_nearest",
+         &make_product_1d_inverse_cube_alternate_interp_nearest, py::arg("order"), py::arg("time"),
+         py::arg("warper_scale"), py::arg("nr_sample_points_warper"));
+  wm.def("make_product_1d_inverse_cube_alternate_interp_hybrid", &make_product_1d_inverse_cube_alternate_interp_hybrid,
+         py::arg("order"), py::arg("time"), py::arg("warper_scale"), py::arg("nr_sample_points_warper"));
+  wm.def("make_warper_from_proj", &make_warper_from_proj, py::arg("warped_integrand"), py::arg("order"),
+         py::arg("num_bins"), py::arg("nr_samples"), py::arg("sigma"));
+  wm.def(
+      "make_warper_from_proj_samples",
+      [](py::array_t<double, py::array::c_style | py::array::forcecast> l,
+         py::array_t<double, py::array::c_style | py::array::forcecast> vals, int order, int num_bins, double sigma) {
+        if (l.ndim() != 2 || l.shape(1) != order || vals.ndim() != 1 || vals.shape(0) != l.shape(0))
+          throw keldy_error("expected l of shape (n, order) and values of shape (n,)");
+        std::vector<vec> pts(size_t(l.shape(0)), vec(size_t(order)));
+        for (ssize_t i = 0; i < l.shape(0); i++)
+          for (int a = 0; a < order; a++) pts[size_t(i)][size_t(a)] = l.at(i, a);
+        vec v(vals.data(), vals.data() + vals.shape(0));
+        return make_warper_from_proj_samples(pts, v, order, num_bins, sigma);
+      },
+      py::arg("l"), py::arg("abs_values"), py::arg("order"), py::arg("num_bins"), py::arg("sigma"));
+  wm.def("kernel_smoothing", &proj_details::kernel_smoothing, py::arg("y_in"), py::arg("sigma"));
+
+  // ------------------------------------------------------ impurity_oneband
+  py::module_ im = mod.def_submodule("impurity_oneband");
+  py::class_<gf_index_t>(im, "GfIndexT")
+      .def(py::init<>())
+      .def(py::init<double, int, int>(), py::arg("time_"), py::arg("spin_"), py::arg("k_idx_"))
+      .def(py::init<double, int, int, int>(), py::arg("time_"), py::arg("spin_"), py::arg("k_idx_"),
+           py::arg("timesplit_n_"))
+      .def(py::init<double, int, int, int, int>(), py::arg("time_"), py::arg("spin_"), py::arg("k_idx_"),
+           py::arg("timesplit_n_"), py::arg("orbital_"))
+      .def_readwrite("contour", &gf_index_t::contour)
+      .def_property(
+          "spin", [](gf_index_t const &g) { return int(g.spin); }, [](gf_index_t &g, int s) { g.spin = spin_t(s); })
+      .def_readwrite("orbital", &gf_index_t::orbital);
+  py::class_<g0_model_omega>(im, "G0ModelOmega")
+      .def(py::init<>())
+      .def(py::init([](py::dict p) { return g0_model_omega(params_from_dict(p)); }), py::arg("parameters"))
+      .def("get_param", [](g0_model_omega const &m) { return params_to_dict(m.get_param()); })
+      .def("get_param_alpha", &g0_model_omega::get_param_alpha)
+      .def("mu_left", &g0_model_omega::mu_left)
+      .def("mu_right", &g0_model_omega::mu_right)
+      .def("g0_dot_R", &g0_model_omega::g0_dot_R, py::arg("omega"))
+      .def("g0_dot_A", &g0_model_omega::g0_dot_A, py::arg("omega"))
+      .def(
+          "g0_dot_keldysh",
+          [](g0_model_omega const &m, int a, int b, dcomplex w) {
+            return m.g0_dot_keldysh(keldysh_idx_t(a), keldysh_idx_t(b), w);
+          },
+          py::arg("a"), py::arg("b"), py::arg("omega"))
+      .def("g0_dot_lesser", &g0_model_omega::g0_dot_lesser, py::arg("omega"))
+      .def("g0_dot_greater", &g0_model_omega::g0_dot_greater, py::arg("omega"))
+      .def("bath_hybrid_R_left", &g0_model_omega::bath_hybrid_R_left, py::arg("omega"))
+      .def("bath_hybrid_A_left", &g0_model_omega::bath_hybrid_A_left, py::arg("omega"))
+      .def("bath_hybrid_K_left", &g0_model_omega::bath_hybrid_K_left, py::arg("omega"))
+      .def("bath_hybrid_R_right", &g0_model_omega::bath_hybrid_R_right, py::arg("omega"))
+      .def("bath_hybrid_A_right", &g0_model_omega::bath_hybrid_A_right, py::arg("omega"))
+      .def("bath_hybrid_K_right", &g0_model_omega::bath_hybrid_K_right, py::arg("omega"))
+      .def("g0_rightlead_dot_lesser", &g0_model_omega::g0_rightlead_dot_lesser, py::arg("omega"))
+      .def("g0_rightlead_dot_greater", &g0_model_omega::g0_rightlead_dot_greater, py::arg("omega"));
+  py::class_<g0_model>(im, "G0Model")
+      .def(py::init<>())
+      .def(py::init<g0_model_omega, bool>(), py::arg("model_omega_"), py::arg("make_dot_lead_"),
+           py::call_guard<py::gil_scoped_release>())
+      .def_static(
+          "from_tables",
+          [](g0_model_omega mo, bool dot_lead, double time_max, py::array_t<dcomplex, py::array::c_style> les,
+             py::array_t<dcomplex, py::array::c_style> gre) {
+            const int n_orb = les.ndim() == 3 ? int(les.shape(0)) : 1;
+            std::vector<dcomplex> l(les.data(), les.data() + les.size()), g(gre.data(), gre.data() + gre.size());
+            return g0_model(mo, dot_lead, n_orb, time_max, std::move(l), std::move(g));
+          },
+          py::arg("model_omega"), py::arg("make_dot_lead"), py::arg("time_max"), py::arg("lesser"),
+          py::arg("greater"))
+      .def_readwrite("model_omega", &g0_model::model_omega)
+      .def_readonly("make_dot_lead", &g0_model::make_dot_lead)
+      .def_property_readonly("g0_lesser", [](g0_model const &m) { return table_array(m, m.lesser); })
+      .def_property_readonly("g0_greater", [](g0_model const &m) { return table_array(m, m.greater); })
+      .def_property_readonly("lesser_ft_error", [](g0_model const &m) { return table_array(m, m.lesser_ft_error); })
+      .def_property_readonly("greater_ft_error", [](g0_model const &m) { return table_array(m, m.greater_ft_error); })
+      .def_property_readonly("time_mesh",
+                             [](g0_model const &m) {
+                               py::array_t<double> t(m.n_pts);
+                               for (int k = 0; k < m.n_pts; k++) t.mutable_at(k) = m.time(k);
+                               return t;
+                             })
+      .def_readonly("time_max", &g0_model::time_max)
+      .def("lesser_at", [](g0_model const &m, double t, int a, int b) { return m.eval(m.lesser, a, b, t); },
+           py::arg("t"), py::arg("a") = 0, py::arg("b") = 0)
+      .def("greater_at", [](g0_model const &m, double t, int a, int b) { return m.eval(m.greater, a, b, t); },
+           py::arg("t"), py::arg("a") = 0, py::arg("b") = 0)
+      .def_static("hdf5_format", &g0_model::hdf5_format);
+  py::class_<g0_keldysh_contour_t>(im, "G0KeldyshContourT")
+      .def(py::init<g0_model>(), py::arg("model_"))
+      .def_readwrite("model", &g0_keldysh_contour_t::model)
+      .def("__call__", &g0_keldysh_contour_t::operator(), py::arg("a"), py::arg("b"), py::arg("internal_point") = true)
+      .def("get_time_max", &g0_keldysh_contour_t::get_time_max);
+  py::class_<integrand_g_direct>(im, "IntegrandGDirect")
+      .def(py::init<g0_keldysh_contour_t, gf_index_t, gf_index_t, double>(), py::arg("g0_"), py::arg("external_A_"),
+           py::arg("external_B_"), py::arg("cutoff_") = 0.)
+      .def("__call__", &integrand_g_direct::operator(), py::arg("times"), py::arg("keep_u_hypercube") = true)
+      .def("eval_no_det", &integrand_g_direct::eval_no_det, py::arg("times"), py::arg("keep_u_hypercube") = true)
+      .def(
+          "eval_batch",
+          [](integrand_g_direct const &f, py::array_t<double, py::array::c_style | py::array::forcecast> u,
+             bool keep) {
+            if (u.ndim() != 2) throw keldy_error("expected an array of shape (n, order)");
+            const long n = long(u.shape(0));
+            py::array_t<dcomplex> out(n);
+            py::array_t<int> dom(n);
+            {
+              py::gil_scoped_release rel;
+              f.eval_batch(u.data(), n, int(u.shape(1)), keep, out.mutable_data(), dom.mutable_data());
+            }
+            return py::make_tuple(out, dom);
+          },
+          py::arg("u"), py::arg("keep_u_hypercube") = true);
+  bind_integrator<compute_charge_Q_direct>(im, "ComputeChargeQDirect");
+  bind_integrator<compute_current_J_direct>(im, "ComputeCurrentJDirect");
+}

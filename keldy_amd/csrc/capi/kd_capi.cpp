@@ -20,6 +20,7 @@ struct kd_ctx {
   int n_cu = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> ev; // per-chunk timing events: prep start/end, LU start/end
   std::string err;
   // g0 table
   double2 *d_lesser = nullptr, *d_greater = nullptr;
@@ -56,8 +57,8 @@ struct kd_ctx {
   // compacted points between the two kernel stages
   void *d_prep = nullptr;
   size_t prep_cap = 0;
-  float last_ms = 0.f;
-  int last_launches = 0;
+  float last_ms = 0.f, last_lu_ms = 0.f, last_prep_ms = 0.f;
+  int last_launches = 0, last_chunks = 0;
 };
 
 namespace {
@@ -159,17 +160,29 @@ int ensure_prep(kd_ctx *c, int order, kd::PrepBuf *buf) {
   return KD_OK;
 }
 
+int ensure_events(kd_ctx *c, size_t n) {
+  while (c->ev.size() < n) {
+    hipEvent_t e;
+    KD_HIP(c, hipEventCreate(&e));
+    c->ev.push_back(e);
+  }
+  return KD_OK;
+}
+
 // Launch prep + integrand kernels over p.n_points points in chunks (and, in
-// MODE_RUN, the deterministic partial reduction into d_acc), timed with HIP
-// events on the ctx stream.
+// MODE_RUN, the deterministic partial reduction into d_acc).  Each kernel is
+// bracketed by HIP events on the ctx stream (kd_last_timing).
 int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) {
   kd::PrepBuf buf;
   int rc = ensure_prep(c, c->order, &buf);
   if (rc) return rc;
   const int N = c->order;
+  const int64_t nchunks = (p0.n_points + kChunk - 1) / kChunk;
+  if ((rc = ensure_events(c, size_t(4 * nchunks)))) return rc;
   int launches = 0;
   KD_HIP(c, hipEventRecord(c->ev0, c->stream));
-  for (int64_t off = 0; off < p0.n_points; off += kChunk) {
+  int64_t ci = 0;
+  for (int64_t off = 0; off < p0.n_points; off += kChunk, ci++) {
     kd::KdParams p = p0;
     const int64_t m = std::min<int64_t>(kChunk, p0.n_points - off);
     p.n_points = m;
@@ -186,7 +199,12 @@ int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) {
       if ((rc = ensure_partials(c, grid))) return rc;
       p.partials = c->d_part;
     }
-    KD_HIP(c, kd::launch_integrand(N, c->precision, p, buf, grid_prep, grid, c->stream));
+    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], c->stream));
+    KD_HIP(c, kd::launch_prep(N, p, buf, grid_prep, c->stream));
+    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
+    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
+    KD_HIP(c, kd::launch_lu(N, c->precision, p, buf, grid, c->stream));
+    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
     launches += 2;
     if (p.mode == kd::MODE_RUN) {
       KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->d_acc, c->stream));
@@ -195,14 +213,23 @@ int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) {
   }
   KD_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->last_launches = launches;
+  c->last_chunks = (int)nchunks;
   return KD_OK;
 }
 
 int finish_timing(kd_ctx *c) {
   KD_HIP(c, hipEventSynchronize(c->ev1));
-  float ms = 0.f;
+  float ms = 0.f, lu = 0.f, prep = 0.f, x = 0.f;
   KD_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  for (int i = 0; i < c->last_chunks; i++) {
+    KD_HIP(c, hipEventElapsedTime(&x, c->ev[4 * i + 0], c->ev[4 * i + 1]));
+    prep += x;
+    KD_HIP(c, hipEventElapsedTime(&x, c->ev[4 * i + 2], c->ev[4 * i + 3]));
+    lu += x;
+  }
   c->last_ms = ms;
+  c->last_lu_ms = lu;
+  c->last_prep_ms = prep;
   return KD_OK;
 }
 
@@ -258,6 +285,7 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_part);
   dfree(c->d_stage);
   dfree(c->d_prep);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -412,8 +440,8 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
   return KD_OK;
 }
 
-int kd_eval_warped_batch(kd_ctx *c, const double *l, int64_t n_points, double *out, int32_t *in_domain,
-                         double *jacobian) {
+int kd_eval_warped_batch(kd_ctx *c, const double *l, int64_t n_points, int32_t keep_u_hypercube, double *out,
+                         int32_t *in_domain, double *jacobian) {
   if (!c) return KD_ERR_ARG;
   int rc = need_integrand(c);
   if (rc) return rc;
@@ -430,6 +458,7 @@ int kd_eval_warped_batch(kd_ctx *c, const double *l, int64_t n_points, double *o
   if (nl) KD_HIP(c, hipMemcpyAsync(dl, l, sizeof(double) * nl, hipMemcpyHostToDevice, c->stream));
   kd::KdParams p = base_params(c);
   p.mode = kd::MODE_EVAL_L;
+  p.keep_u = keep_u_hypercube ? 1 : 0;
   p.in_pts = dl;
   p.n_points = n_points;
   p.out_vals = dout;
@@ -533,9 +562,11 @@ int kd_sample_device(kd_ctx *c, uint64_t n_steps, double *d_l, double *d_value) 
   return KD_OK;
 }
 
-int kd_last_kernel_ms(const kd_ctx *c, double *ms, int32_t *launches) {
-  if (!c || !ms) return KD_ERR_ARG;
-  *ms = c->last_ms;
+int kd_last_timing(const kd_ctx *c, double *total_ms, double *lu_ms, double *prep_ms, int32_t *launches) {
+  if (!c) return KD_ERR_ARG;
+  if (total_ms) *total_ms = c->last_ms;
+  if (lu_ms) *lu_ms = c->last_lu_ms;
+  if (prep_ms) *prep_ms = c->last_prep_ms;
   if (launches) *launches = c->last_launches;
   return KD_OK;
 }

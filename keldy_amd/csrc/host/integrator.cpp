@@ -1,0 +1,245 @@
+// integrator.cpp -- host side of the integrand and the QMC driver.
+// Reference: wick_direct.hpp:48-60 / wick_direct.cpp:43-228, integrator.hpp:68-131,
+// compute_obs.hpp:51-80.
+#include "integrator.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace keldy {
+
+device_ctx::device_ctx() {
+  kd_ctx *c = nullptr;
+  const int rc = kd_create(&c, 0);
+  if (rc == KD_ERR_NODEVICE) throw keldy_error("keldy_amd: no HIP device available (the integrand runs on the GPU)");
+  if (rc != KD_OK) throw keldy_error("keldy_amd: kd_create failed (" + std::to_string(rc) + ")");
+  ctx_ = std::shared_ptr<kd_ctx>(c, [](kd_ctx *p) { kd_destroy(p); });
+}
+
+void device_ctx::check(int rc) const {
+  if (rc != KD_OK) throw keldy_error(kd_last_error(ctx_.get()));
+}
+
+namespace impurity_oneband {
+
+namespace {
+kd_gf_index to_kd(gf_index_t const &g) {
+  kd_gf_index k{};
+  k.time = g.contour.time;
+  k.k_idx = int32_t(g.contour.k_idx);
+  k.spin = int32_t(g.spin);
+  k.orbital = int32_t(g.orbital);
+  return k;
+}
+} // namespace
+
+integrand_g_direct::integrand_g_direct(g0_keldysh_contour_t g0_in, gf_index_t A, gf_index_t B, double cutoff_)
+   : g0_(std::move(g0_in)), external_A(A), external_B(B), cutoff(cutoff_) {
+  if (external_A.contour.time < 0. || external_B.contour.time < 0.)
+    throw keldy_error("An external point has negative time.");
+  const double t_max = g0_.get_time_max();
+  if (external_A.contour.time > t_max || external_B.contour.time > t_max)
+    throw keldy_error("An external point is out of g0 time window.");
+  if ((external_A.orbital != 0 || external_B.orbital != 0) && !g0_.model.make_dot_lead)
+    throw keldy_error("Need g0_model with make_dot_lead == true to calculate off_diagonal gf.");
+}
+
+void integrand_g_direct::bind(device_ctx const &ctx, int order) const {
+  const g0_model &m = g0_.model;
+  ctx.check(kd_set_g0_table(ctx.get(), reinterpret_cast<const double *>(m.lesser.data()),
+                            reinterpret_cast<const double *>(m.greater.data()), m.n_orb, m.n_pts, m.time_max,
+                            m.model_omega.get_param_alpha()));
+  const kd_gf_index a = to_kd(external_A), b = to_kd(external_B);
+  ctx.check(kd_set_integrand(ctx.get(), order, &a, &b, cutoff));
+}
+
+// The device yields NaN for g0 lookups outside the tabulated window; the
+// reference's gf evaluation throws there, and so does the host API.
+void check_window(const dcomplex *out, long n) {
+  for (long i = 0; i < n; i++)
+    if (std::isnan(out[i].real()) || std::isnan(out[i].imag()))
+      throw keldy_error("g0 evaluated outside its time window [-time_max, time_max]");
+}
+
+void integrand_g_direct::eval_batch(const double *u, long n, int order, bool keep_u_hypercube, dcomplex *out,
+                                    int *in_domain) const {
+  if (order > KD_MAX_ORDER)
+    throw keldy_error("order " + std::to_string(order) + " exceeds the device maximum " + std::to_string(KD_MAX_ORDER));
+  if (!ctx_) ctx_ = std::make_shared<device_ctx>();
+  if (bound_order_ != order) {
+    bind(*ctx_, order);
+    bound_order_ = order;
+  }
+  ctx_->check(kd_eval_batch(ctx_->get(), u, n, keep_u_hypercube ? 1 : 0, reinterpret_cast<double *>(out),
+                            in_domain));
+  check_window(out, n);
+}
+
+std::pair<dcomplex, int> integrand_g_direct::operator()(std::vector<double> const &times, bool keep_u_hypercube) const {
+  dcomplex v;
+  int dom = 0;
+  eval_batch(times.data(), 1, int(times.size()), keep_u_hypercube, &v, &dom);
+  return {v, dom};
+}
+
+// Orders 0-2 by explicit Wick contractions (wick_direct.cpp:161-228).
+std::pair<dcomplex, int> integrand_g_direct::eval_no_det(std::vector<double> const &times, bool keep_u_hypercube) const {
+  if (external_A.spin != external_B.spin) return {0.0, 0};
+  if (keep_u_hypercube && std::any_of(times.begin(), times.end(), [](double t) { return t < 0.0; })) return {0.0, 0};
+  const int n = int(times.size());
+  gf_index_t a = external_A, b = external_B;
+  a.contour.timesplit_n = n;
+  b.contour.timesplit_n = n;
+  const g0_keldysh_contour_t &g = g0_;
+  if (n == 0) return {g(a, b, false), 1};
+  dcomplex res = 0.0;
+  if (n == 1) {
+    for (int k0 : {0, 1}) {
+      auto pt = [&](int sp) { return gf_index_t(times[0], sp, k0, 0); };
+      const double sign = k0 == 0 ? 1. : -1.;
+      res += -sign * g(pt(down), pt(down)) * g(a, pt(up)) * g(pt(up), b);
+    }
+  } else if (n == 2) {
+    for (int k0 : {0, 1})
+      for (int k1 : {0, 1}) {
+        auto pt = [&](int i, int sp) { return gf_index_t(times[i], sp, i == 0 ? k0 : k1, i); };
+        const double sign = ((k0 + k1) % 2 == 0) ? 1. : -1.;
+        res += sign *
+               (g(pt(0, up), pt(0, up)) * g(a, pt(1, up)) * g(pt(1, up), b) +
+                g(a, pt(0, up)) * g(pt(1, up), pt(1, up)) * g(pt(0, up), b)) *
+               g(pt(1, down), pt(0, down)) * g(pt(0, down), pt(1, down));
+        res += sign *
+               (g(a, pt(0, up)) * g(pt(0, up), pt(1, up)) * g(pt(1, up), b) +
+                g(pt(1, up), pt(0, up)) * g(a, pt(1, up)) * g(pt(0, up), b)) *
+               (g(pt(0, down), pt(0, down)) * g(pt(1, down), pt(1, down)) -
+                g(pt(1, down), pt(0, down)) * g(pt(0, down), pt(1, down)));
+      }
+  } else {
+    throw keldy_error("Not supported for order > 2");
+  }
+  if (std::abs(res) < cutoff) res = 0.;
+  return {res, 1};
+}
+
+// --------------------------------------------------------------------------
+integrator_direct::integrator_direct(integrand_g_direct integrand_, int dimension_, std::string rng_name,
+                                     int rng_state_seed)
+   : dimension(dimension_), integrand(std::move(integrand_)) {
+  if (dimension <= 0) throw keldy_error("Dimension of the integration space must be > 0.");
+  if (dimension > KD_MAX_ORDER)
+    throw keldy_error("order " + std::to_string(dimension) + " exceeds the device maximum " +
+                      std::to_string(KD_MAX_ORDER));
+  integrand.bind(ctx_, dimension);
+  reset_rng(std::move(rng_name), rng_state_seed);
+}
+
+void integrator_direct::set_precision(int precision) {
+  ctx_.check(kd_set_precision(ctx_.get(), precision));
+  precision_ = precision;
+}
+
+void integrator_direct::sync_warper() {
+  if (warper.stamp == uploaded_stamp_) return;
+  const warpers::device_chain c = warper.to_device();
+  ctx_.check(kd_set_warper_chain(ctx_.get(), c.warpers.data(), int32_t(c.warpers.size()), c.maps.data(),
+                                 int32_t(c.maps.size()), c.tables.data(), int64_t(c.tables.size())));
+  uploaded_stamp_ = warper.stamp;
+}
+
+std::pair<dcomplex, double> integrator_direct::evaluate_warped_integrand(std::vector<double> const &li_vec,
+                                                                       int start_domain_nr, bool keep_u_hypercube) {
+  if (start_domain_nr == int(warper.size())) {
+    dcomplex v;
+    int dom = 0;
+    double jac = 0;
+    evaluate_warped_integrand_batch(li_vec.data(), 1, &v, &dom, &jac, keep_u_hypercube);
+    return {v, double(dom)};
+  }
+  // partial chain: the host maps the point, the device evaluates the integrand
+  auto [ui, jac] = warper.map_reverse(li_vec, start_domain_nr, 0);
+  auto [v, dom] = integrand(ui, keep_u_hypercube);
+  return {v * jac, double(dom)};
+}
+
+void integrator_direct::evaluate_warped_integrand_batch(const double *l, long n, dcomplex *out, int *in_domain,
+                                                        double *jac, bool keep_u_hypercube) {
+  sync_warper();
+  ctx_.check(kd_eval_warped_batch(ctx_.get(), l, n, keep_u_hypercube ? 1 : 0, reinterpret_cast<double *>(out),
+                                  in_domain, jac));
+  check_window(out, n);
+}
+
+void integrator_direct::run(long nr_steps) {
+  if (nr_steps <= 0) return;
+  sync_warper();
+  if (rng_dirty_) {
+    ctx_.check(kd_set_sobol(ctx_.get(), rng_->dim(), rng_->csr().data(), rng_->shfr().data(), rng_->state()));
+    rng_dirty_ = false;
+  }
+  kd_accum acc{};
+  ctx_.check(kd_run(ctx_.get(), uint64_t(nr_steps), &acc));
+  n_points_ += uint64_t(nr_steps);
+  uint32_t k = 0;
+  ctx_.check(kd_get_sobol_state(ctx_.get(), &k));
+  rng_->set_state(k);
+}
+
+void integrator_direct::reset_rng(std::string rng_name, int rng_state_seed, bool do_shift, bool do_scramble,
+                                  int rng_seed_shift) {
+  if (rng_name == "sobol_unshifted") {
+    rng_ = std::make_unique<sobol>(dimension, rng_state_seed);
+  } else if (rng_name == "sobol") {
+    rng_ = std::make_unique<sobol>(dimension, rng_state_seed, do_shift, do_scramble, rng_seed_shift);
+  } else if (rng_name == "mt19937_64" || rng_name == "ranlux48" || rng_name == "taus88") {
+    throw keldy_error("rng '" + rng_name + "' is a pseudo-random generator; keldy_amd generates Sobol points on the "
+                      "device only ('sobol', 'sobol_unshifted')");
+  } else {
+    throw keldy_error("No other rng available.");
+  }
+  rng_dirty_ = true;
+}
+
+dcomplex integrator_direct::reduce_result() const {
+  kd_accum acc{};
+  ctx_.check(kd_get_accum(ctx_.get(), &acc));
+  return dcomplex(acc.re, acc.im) * (1.0 / double(n_points_));
+}
+
+uint64_t integrator_direct::reduce_nr_points_in_domain() const {
+  kd_accum acc{};
+  ctx_.check(kd_get_accum(ctx_.get(), &acc));
+  return acc.n_in_domain;
+}
+
+uint64_t integrator_direct::reduce_nr_fallback() const {
+  kd_accum acc{};
+  ctx_.check(kd_get_accum(ctx_.get(), &acc));
+  return acc.n_fallback;
+}
+
+std::vector<double> integrator_direct::last_timing() const {
+  double t = 0, lu = 0, pr = 0;
+  int32_t n = 0;
+  ctx_.check(kd_last_timing(ctx_.get(), &t, &lu, &pr, &n));
+  return {t, lu, pr, double(n)};
+}
+
+double integrator_direct::flops_per_point() const {
+  double f = 0;
+  ctx_.check(kd_flops_per_point(ctx_.get(), &f));
+  return f;
+}
+
+// --------------------------------------------------------------------------
+compute_charge_Q_direct::compute_charge_Q_direct(g0_model model, double time, int order, double cutoff_integrand)
+   : integrator_direct(integrand_g_direct(g0_keldysh_contour_t(std::move(model)), gf_index_t(time, up, forward),
+                                          gf_index_t(time, up, backward), cutoff_integrand),
+                       order, "sobol_unshifted", 0) {}
+
+compute_current_J_direct::compute_current_J_direct(g0_model model, double time, int order, double cutoff_integrand)
+   : integrator_direct(integrand_g_direct(g0_keldysh_contour_t(std::move(model)), gf_index_t(time, up, forward, 0, 1),
+                                          gf_index_t(time, up, backward, 0, 0), cutoff_integrand),
+                       order, "sobol_unshifted", 0) {}
+
+} // namespace impurity_oneband
+} // namespace keldy

@@ -1,0 +1,130 @@
+// integrator.hpp -- integrand and QMC driver of the host API, backed by the
+// HIP path through the C-ABI (include/keldy_amd.h).
+// Reference: c++/keldy/impurity_oneband/wick_direct.hpp:34-65,
+// c++/keldy/integrator.hpp:42-136, impurity_oneband/compute_obs.hpp:51-80.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/keldy_amd.h"
+#include "model.hpp"
+#include "qrng.hpp"
+#include "warpers.hpp"
+
+namespace keldy {
+
+/// Shared owner of one device context (one HIP stream on device 0).
+class device_ctx {
+ public:
+  device_ctx();
+  kd_ctx *get() const { return ctx_.get(); }
+  /// throw keldy_error with the C-ABI's message if rc != KD_OK
+  void check(int rc) const;
+
+ private:
+  std::shared_ptr<kd_ctx> ctx_;
+};
+
+namespace impurity_oneband {
+
+void check_window(const dcomplex *out, long n);
+
+/// Keldysh-sum integrand (wick_direct.hpp:34-65).  Evaluation runs on the GPU.
+class integrand_g_direct {
+ public:
+  using result_t = dcomplex;
+
+  integrand_g_direct(g0_keldysh_contour_t g0_, gf_index_t external_A_, gf_index_t external_B_,
+                     double cutoff_ = 0.);
+
+  /// (value, in_domain) at one vector of vertex times (wick_direct.cpp:43-158)
+  std::pair<result_t, int> operator()(std::vector<double> const &times, bool keep_u_hypercube = true) const;
+  /// values at many points, row-major [n][order]
+  void eval_batch(const double *u, long n, int order, bool keep_u_hypercube, dcomplex *out, int *in_domain) const;
+  /// closed forms of orders 0-2 by explicit Wick contractions (wick_direct.cpp:161-228)
+  std::pair<result_t, int> eval_no_det(std::vector<double> const &times, bool keep_u_hypercube = true) const;
+
+  g0_keldysh_contour_t const &g0() const { return g0_; }
+  gf_index_t const &A() const { return external_A; }
+  gf_index_t const &B() const { return external_B; }
+  double get_cutoff() const { return cutoff; }
+  /// upload table + externals + order to a device context
+  void bind(device_ctx const &ctx, int order) const;
+
+ private:
+  g0_keldysh_contour_t g0_;
+  gf_index_t external_A, external_B;
+  double cutoff;
+  mutable std::shared_ptr<device_ctx> ctx_; // lazily created for per-point calls
+  mutable int bound_order_ = -1;
+};
+
+/// QMC driver: Sobol points -> warper chain -> integrand -> accumulation,
+/// all on the device (integrator.hpp:42-136).
+class integrator_direct {
+ public:
+  warpers::warper_train_t warper{};
+
+  integrator_direct(integrand_g_direct integrand_, int dimension_, std::string rng_name, int rng_state_seed);
+
+  std::pair<dcomplex, double> evaluate_warped_integrand(std::vector<double> const &li_vec, int start_domain_nr,
+                                                        bool keep_u_hypercube = true);
+  std::pair<dcomplex, double> evaluate_warped_integrand(std::vector<double> const &li_vec) {
+    return evaluate_warped_integrand(li_vec, int(warper.size()), true);
+  }
+  /// batched warped evaluation with the full chain: values and in-domain flags
+  void evaluate_warped_integrand_batch(const double *l, long n, dcomplex *out, int *in_domain, double *jac,
+                                       bool keep_u_hypercube = true);
+
+  void run(long nr_steps);
+  void reset_rng(std::string rng_name, int rng_state_seed, bool do_shift = false, bool do_scramble = false,
+                 int rng_seed_shift = 0);
+  dcomplex reduce_result() const;
+  uint64_t reduce_nr_points_run() const { return n_points_; }
+  uint64_t reduce_nr_points_in_domain() const;
+  uint64_t reduce_nr_fallback() const;
+
+  integrand_g_direct get_integrand() const { return integrand; }
+  warpers::warper_train_t get_warper() const { return warper; }
+  int get_dimension() const { return dimension; }
+  void set_precision(int precision);
+  int get_precision() const { return precision_; }
+  /// timing of the last device call (total, LU kernels, prep kernels; ms)
+  std::vector<double> last_timing() const;
+  double flops_per_point() const;
+  device_ctx const &ctx() const { return ctx_; }
+  /// upload the warper chain if it changed since the last upload
+  void sync_warper();
+
+ protected:
+  int dimension = 1;
+  integrand_g_direct integrand;
+  device_ctx ctx_;
+  std::unique_ptr<sobol> rng_;
+  bool rng_dirty_ = true;
+  uint64_t n_points_ = 0;
+  int precision_ = KD_PREC_FP64;
+  uint64_t uploaded_stamp_ = 0;
+};
+
+/// Charge Q = G^<_{up,up}(t, t) (compute_obs.hpp:51-64).
+class compute_charge_Q_direct : public integrator_direct {
+ public:
+  compute_charge_Q_direct(g0_model model, double time, int order, double cutoff_integrand);
+  compute_charge_Q_direct(model_param_t params, double time, int order, double cutoff_integrand)
+     : compute_charge_Q_direct(g0_model{g0_model_omega{params}, false}, time, order, cutoff_integrand) {}
+};
+
+/// Current J ~ Re G^<_{lead,dot}(t, t) (compute_obs.hpp:67-80).
+class compute_current_J_direct : public integrator_direct {
+ public:
+  compute_current_J_direct(g0_model model, double time, int order, double cutoff_integrand);
+  compute_current_J_direct(model_param_t params, double time, int order, double cutoff_integrand)
+     : compute_current_J_direct(g0_model{g0_model_omega{params}, true}, time, order, cutoff_integrand) {}
+};
+
+} // namespace impurity_oneband
+} // namespace keldy

@@ -1,0 +1,639 @@
+// warpers.cpp -- host side of the warpers (reference: c++/keldy/warpers/*.hpp).
+#include "warpers.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <sstream>
+
+#include "qrng.hpp"
+
+namespace keldy::warpers {
+
+namespace {
+
+// gsl_interp_bsearch over [0, n-1]: largest i in [0, n-2] with xa[i] <= x
+size_t knot_find(const double *xa, size_t n, double x) {
+  size_t ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    const size_t i = (ihi + ilo) / 2;
+    if (xa[i] > x)
+      ihi = i;
+    else
+      ilo = i;
+  }
+  return ilo;
+}
+
+void check_domain(const double *xa, size_t n, double x) {
+  if (x < xa[0] || x > xa[n - 1]) throw keldy_error("interpolation error: point outside the tabulated domain");
+}
+
+// linear spline (gsl_interp_linear) through (xa, ya)
+double linear_eval(const double *xa, const double *ya, size_t n, double x) {
+  check_domain(xa, n, x);
+  const size_t i = knot_find(xa, n, x);
+  const double dx = xa[i + 1] - xa[i];
+  if (!(dx > 0.0)) return 0.0;
+  return ya[i] + (x - xa[i]) / dx * (ya[i + 1] - ya[i]);
+}
+
+double steffen_sign(double x, double y) { return ((x < 0 && y > 0) || (x > 0 && y < 0)) ? -x : x; }
+
+// Steffen (1990) monotone cubic: slopes from eq. (11), one-sided end slopes,
+// coefficients (a, b, c, d) per interval, y = d + dx (c + dx (b + dx a)).
+vec steffen_coeffs(const vec &x, const vec &y, size_t n) {
+  vec yp(n), co(4 * (n - 1));
+  yp[0] = (y[1] - y[0]) / (x[1] - x[0]);
+  for (size_t i = 1; i + 1 < n; i++) {
+    const double hi = x[i + 1] - x[i], him1 = x[i] - x[i - 1];
+    const double si = (y[i + 1] - y[i]) / hi, sim1 = (y[i] - y[i - 1]) / him1;
+    const double pi = (sim1 * hi + si * him1) / (him1 + hi);
+    yp[i] = (steffen_sign(1.0, sim1) + steffen_sign(1.0, si)) *
+            std::min(std::abs(sim1), std::min(std::abs(si), 0.5 * std::abs(pi)));
+  }
+  yp[n - 1] = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);
+  for (size_t i = 0; i + 1 < n; i++) {
+    const double hi = x[i + 1] - x[i];
+    const double si = (y[i + 1] - y[i]) / hi;
+    co[4 * i + 0] = (yp[i] + yp[i + 1] - 2 * si) / hi / hi;
+    co[4 * i + 1] = (3 * si - 2 * yp[i] - yp[i + 1]) / hi;
+    co[4 * i + 2] = yp[i];
+    co[4 * i + 3] = y[i];
+  }
+  return co;
+}
+
+double steffen_eval(const double *x, const double *co, size_t n, double xv) {
+  check_domain(x, n, xv);
+  const size_t i = knot_find(x, n, xv);
+  const double d = xv - x[i];
+  return co[4 * i + 3] + d * (co[4 * i + 2] + d * (co[4 * i + 1] + d * co[4 * i + 0]));
+}
+
+} // namespace
+
+// --------------------------------------------------------------------------
+// map1d
+map1d map1d::make_uniform(double u_max) {
+  if (!(0.0 < u_max)) throw keldy_error("coord_domain_u coordinates must be ordered.");
+  map1d m;
+  m.kind_ = uniform;
+  m.u_max_ = u_max;
+  return m;
+}
+
+map1d map1d::make_exponential(double u_max, double w) {
+  map1d m;
+  m.kind_ = exponential;
+  m.u_max_ = u_max;
+  m.w_ = w;
+  m.norm_ld_ = -std::expm1(-static_cast<long double>(u_max) / w);
+  return m;
+}
+
+map1d map1d::make_inverse(double u_max, double w) {
+  map1d m;
+  m.kind_ = inverse;
+  m.u_max_ = u_max;
+  m.w_ = w;
+  m.norm_ld_ = std::log1p(static_cast<long double>(u_max) / w);
+  return m;
+}
+
+map1d map1d::make_inverse_square(double u_max, double w) {
+  map1d m;
+  m.kind_ = inverse_square;
+  m.u_max_ = u_max;
+  m.w_ = w;
+  m.norm_ld_ = static_cast<long double>(u_max) / (w + u_max);
+  return m;
+}
+
+map1d map1d::make_inverse_power(int power, double u_max, double w) {
+  if (power == 1) return make_inverse(u_max, w);
+  map1d m;
+  m.kind_ = inverse_power;
+  m.power_ = power;
+  m.u_max_ = u_max;
+  m.w_ = w;
+  const long double um = u_max;
+  m.norm_ld_ = (power - 1) * std::pow(um + w, power - 1) / (std::pow(um / w + 1, power - 1) - 1);
+  return m;
+}
+
+map1d map1d::make_callable(std::function<double(double)> f1, std::function<double(double)> F,
+                           std::function<double(double)> Finv, double u_max, bool do_domain_checks) {
+  map1d m;
+  m.kind_ = callable;
+  m.u_max_ = u_max;
+  m.f1_fn_ = std::move(f1);
+  m.F_fn_ = std::move(F);
+  m.Finv_fn_ = std::move(Finv);
+  if (!(0.0 < u_max)) throw keldy_error("coord_domain_u coordinates must be ordered.");
+  // product_1d_simple.hpp:74-104
+  if (do_domain_checks) {
+    const double eps = 1e-12;
+    const bool ok = std::abs(m.F_fn_(0.0) - 0.0) < eps && std::abs(m.F_fn_(u_max) - 1.0) < eps &&
+                    std::abs(m.Finv_fn_(0.0) - 0.0) < eps && std::abs(m.Finv_fn_(1.0) - u_max) < eps;
+    if (!ok) throw keldy_error("f1_integrated and f1_integrated_inverse do not map domain boundaries correctly.");
+    for (double lp : {0., 0.125, 0.25, 0.375, 0.5, 0.625, 0.75, 0.875, 1.}) {
+      if (std::abs(m.F_fn_(m.Finv_fn_(lp)) - lp) > 1e-12) {
+        std::ostringstream os;
+        os << "Inconsistent functions: f1_integrated_inverse should be the inverse of f1_integrated ("
+           << m.F_fn_(m.Finv_fn_(lp)) << " != " << lp << ")";
+        throw keldy_error(os.str());
+      }
+    }
+  }
+  return m;
+}
+
+// warper_product_1d_simple_interp_nearest_t (product_1d_simple.hpp:219-287)
+map1d map1d::make_nearest(vec times, vec f1) {
+  const size_t n = times.size();
+  if (f1.size() != n) throw keldy_error("times_u_pts and f1_pts should have the same length.");
+  if (n < 3) throw keldy_error("Expect nr_sample_points to be >= 3. Given was " + std::to_string(n));
+  if (times[0] != 0.) throw keldy_error("times_u_pts should start at 0");
+  map1d m;
+  m.kind_ = nearest;
+  m.u_max_ = times[n - 1];
+  vec F(n, 0.0);
+  size_t cut = n - 1;
+  bool found = false;
+  for (size_t ii = n - 1; ii-- > 0;) {
+    F[ii] = F[ii + 1] - 0.5 * (f1[ii + 1] + f1[ii]) * (times[ii + 1] - times[ii]);
+    found = found || (F[ii] != 0);
+    if (!found) cut = ii;
+  }
+  if (!found) throw keldy_error("Model function is flat zero.");
+  const double norm = -F[0];
+  for (auto &v : F) v /= norm;
+  for (auto &v : f1) v /= norm;
+  F[0] = -1;
+  m.norm_ = norm;
+  m.t_ = std::move(times);
+  m.f_ = std::move(f1);
+  m.Fk_ = std::move(F);
+  m.cut_ = int(cut);
+  return m;
+}
+
+map1d map1d::make_nearest_from_function(const std::function<double(double)> &f1, double u_max, int n) {
+  if (n < 3) throw keldy_error("Expect nr_sample_points to be >= 3. Given was " + std::to_string(n));
+  vec t(n), f(n);
+  for (int i = 0; i < n; i++) t[i] = static_cast<double>(i) / (n - 1) * u_max;
+  t[n - 1] = u_max;
+  for (int i = 0; i < n; i++) f[i] = f1(t[i]);
+  return make_nearest(std::move(t), std::move(f));
+}
+
+// warper_product_1d_simple_interp_hybrid_t (product_1d_simple.hpp:424-491)
+map1d map1d::make_hybrid(std::function<double(double)> f1, double u_max, int n, int power, double w) {
+  if (n < 3) throw keldy_error("Expect nr_sample_points to be >= 3. Given was " + std::to_string(n));
+  map1d m;
+  m.kind_ = hybrid;
+  m.u_max_ = u_max;
+  m.power_ = power;
+  m.w_ = w;
+  const double delta = u_max / (n - 1);
+  vec t(n), F(n, 0.0);
+  for (int i = 0; i < n; i++) t[i] = delta * i;
+  t[n - 1] = u_max;
+  int cut = n - 1;
+  bool found = false;
+  double f2d = f1(t[n - 1]);
+  for (int i = n - 2; i >= 0; i--) {
+    const double f0d = f2d;
+    const double f1d = f1(0.5 * (t[i + 1] + t[i]));
+    f2d = f1(t[i]);
+    const double dl = delta / 6. * (f0d + 4 * f1d + f2d);
+    F[i] = F[i + 1] - dl;
+    if (dl < 1e-100)
+      throw keldy_error("dl=" + std::to_string(dl) + " < 1e-100 is too small for a cubic interpolator in double precision.");
+    found = found || (F[i] != 0);
+    if (!found) cut = i;
+  }
+  if (!found) throw keldy_error("Model function is flat zero.");
+  const double norm = -F[0];
+  for (auto &v : F) v /= norm;
+  F[0] = -1;
+  m.norm_ = norm;
+  m.f1_fn_ = std::move(f1);
+  m.t_ = std::move(t);
+  m.Fk_ = std::move(F);
+  m.cut_ = cut;
+  m.fwd_coef_ = steffen_coeffs(m.t_, m.Fk_, size_t(n));
+  vec Fi(m.Fk_.begin(), m.Fk_.begin() + cut + 1), ti(m.t_.begin(), m.t_.begin() + cut + 1);
+  m.inv_coef_ = steffen_coeffs(Fi, ti, size_t(cut + 1));
+  return m;
+}
+
+double map1d::f1(double u) const {
+  const double w = w_;
+  switch (kind_) {
+  case uniform: return 1.0 / u_max_;
+  case exponential: return double(std::exp(-u / w) / (w * norm_ld_));
+  case inverse: return double(1.0 / (norm_ld_ * (w + u)));
+  case inverse_square: return double(w / ((w + u) * (w + u) * norm_ld_));
+  case inverse_power: return double(norm_ld_ / std::pow(w + u, power_));
+  case callable: return f1_fn_(u);
+  case nearest: {
+    const size_t i = knot_find(t_.data(), t_.size(), u);
+    if (i == t_.size() - 1) return f_[i];
+    return 0.5 * (f_[i] + f_[i + 1]);
+  }
+  case hybrid: return f1_fn_(u) / norm_;
+  }
+  return 0.0;
+}
+
+double map1d::F(double u) const {
+  const double w = w_, um = u_max_;
+  switch (kind_) {
+  case uniform: return u / um;
+  case exponential: return std::expm1((um - u) / w) / std::expm1(um / w);
+  case inverse: return double(std::log1p(u / w) / norm_ld_);
+  case inverse_square: return double(u / ((w + u) * norm_ld_));
+  case inverse_power: {
+    const long double umax = um;
+    const long double n = std::pow(umax + w, power_ - 1) / (std::pow(umax / w + 1, power_ - 1) - 1);
+    return double(n * (std::pow(u / w + 1, power_ - 1) - 1) / std::pow(u + w, power_ - 1));
+  }
+  case callable: return F_fn_(u);
+  case nearest: return -linear_eval(t_.data(), Fk_.data(), t_.size(), u);
+  case hybrid: return -steffen_eval(t_.data(), fwd_coef_.data(), t_.size(), u);
+  }
+  return 0.0;
+}
+
+double map1d::Finv(double l) const {
+  const double w = w_, um = u_max_;
+  switch (kind_) {
+  case uniform: return um * l;
+  case exponential: return um - w * std::log1p(std::expm1(um / w) * l);
+  case inverse: return double(w * std::expm1(l * norm_ld_));
+  case inverse_square: return double(norm_ld_ * l * w / (1.0 - norm_ld_ * l));
+  case inverse_power: {
+    const long double umax = um;
+    const long double denom =
+        std::pow(1 + l * (std::pow(w / (umax + w), power_ - 1) - 1), 1.0 / (power_ - 1.0));
+    return double(w * (1 - denom) / denom);
+  }
+  case callable: return Finv_fn_(l);
+  case nearest: return linear_eval(Fk_.data(), t_.data(), size_t(cut_ + 1), -l);
+  case hybrid: return steffen_eval(Fk_.data(), inv_coef_.data(), size_t(cut_ + 1), -l);
+  }
+  return 0.0;
+}
+
+bool map1d::device_ok() const {
+  if (kind_ == callable) return false;
+  if (kind_ == hybrid) return power_ > 0;
+  return true;
+}
+
+kd_map1d map1d::to_device(std::vector<double> &tables) const {
+  if (!device_ok())
+    throw keldy_error(
+        "this warper is defined by host callables and has no device form; use the makers "
+        "(make_product_1d_simple_*, interp_nearest/interp_hybrid tables) to run it on the GPU");
+  kd_map1d d{};
+  d.power = power_;
+  d.u_max = u_max_;
+  d.w = w_;
+  d.norm = double(norm_ld_);
+  switch (kind_) {
+  case uniform: d.kind = KD_MAP_UNIFORM; break;
+  case exponential: d.kind = KD_MAP_EXPONENTIAL; break;
+  case inverse: d.kind = KD_MAP_INVERSE; break;
+  case inverse_square: d.kind = KD_MAP_INVERSE_SQ; break;
+  case inverse_power: d.kind = KD_MAP_INVERSE_POW; break;
+  case nearest:
+  case hybrid: {
+    d.kind = kind_ == nearest ? KD_MAP_NEAREST : KD_MAP_HYBRID;
+    d.n_pts = int32_t(t_.size());
+    d.n_inv = int32_t(cut_ + 1);
+    d.t_off = int64_t(tables.size());
+    tables.insert(tables.end(), t_.begin(), t_.end());
+    d.F_off = int64_t(tables.size());
+    tables.insert(tables.end(), Fk_.begin(), Fk_.end());
+    if (kind_ == nearest) {
+      d.f_off = int64_t(tables.size());
+      tables.insert(tables.end(), f_.begin(), f_.end());
+    } else {
+      d.c_off = int64_t(tables.size());
+      tables.insert(tables.end(), inv_coef_.begin(), inv_coef_.end());
+      d.norm = norm_;
+    }
+    break;
+  }
+  default: break;
+  }
+  return d;
+}
+
+// --------------------------------------------------------------------------
+void warper_identity_t::to_device(device_chain &c) const {
+  kd_warper_desc d{};
+  d.kind = KD_WARP_IDENTITY;
+  c.warpers.push_back(d);
+}
+
+// plasma_uv.hpp:35-51
+vec vi_from_ui(double t_max, vec const &u_times) {
+  vec v = u_times;
+  std::sort(v.begin(), v.end(), std::greater<>());
+  const int n = int(v.size());
+  for (int i = n - 1; i > 0; i--) v[i] = v[i - 1] - v[i];
+  if (n) v[0] = t_max - v[0];
+  return v;
+}
+
+vec ui_from_vi(double t_max, vec const &v_times) {
+  vec u = v_times;
+  if (u.empty()) return u;
+  u[0] = t_max - u[0];
+  for (size_t i = 1; i < u.size(); i++) u[i] = u[i - 1] - u[i];
+  return u;
+}
+
+void warper_plasma_uv_t::to_device(device_chain &c) const {
+  kd_warper_desc d{};
+  d.kind = KD_WARP_PLASMA;
+  d.t_max = t_max;
+  c.warpers.push_back(d);
+}
+
+// --------------------------------------------------------------------------
+map_pair warper_product_1d_simple_base::map_reverse(vec const &l) const {
+  vec x = l;
+  double j = 1.0;
+  for (auto &xi : x) {
+    xi = m.Finv(xi);
+    j *= 1.0 / m.f1(xi);
+  }
+  return {x, j};
+}
+
+map_pair warper_product_1d_simple_base::map_forward(vec const &u) const {
+  vec x = u;
+  double j = 1.0;
+  for (auto &xi : x) {
+    j *= m.f1(xi);
+    xi = m.F(xi);
+  }
+  return {x, j};
+}
+
+vec warper_product_1d_simple_base::ui_from_li(vec const &l) const {
+  vec x = l;
+  for (auto &xi : x) xi = m.Finv(xi);
+  return x;
+}
+
+vec warper_product_1d_simple_base::li_from_ui(vec const &u) const {
+  vec x = u;
+  for (auto &xi : x) xi = m.F(xi);
+  return x;
+}
+
+double warper_product_1d_simple_base::jacobian_reverse(vec const &l) const {
+  double r = 1.0;
+  for (double li : l) r *= 1.0 / m.f1(m.Finv(li));
+  return r;
+}
+
+double warper_product_1d_simple_base::jacobian_forward(vec const &u) const {
+  double r = 1.0;
+  for (double ui : u) r *= m.f1(ui);
+  return r;
+}
+
+double warper_product_1d_simple_base::operator()(vec const &u) const {
+  double r = 1.0;
+  for (double ui : u) r *= m.f1(ui) / m.normalization();
+  return r;
+}
+
+void warper_product_1d_simple_base::to_device(device_chain &c) const {
+  kd_warper_desc d{};
+  d.kind = KD_WARP_PRODUCT;
+  d.n_maps = 1;
+  d.map_offset = int64_t(c.maps.size());
+  c.maps.push_back(m.to_device(c.tables));
+  c.warpers.push_back(d);
+}
+
+// --------------------------------------------------------------------------
+warper_product_1d_simple_t make_product_1d_simple_exponential(double u_max, double w) {
+  return warper_product_1d_simple_t(map1d::make_exponential(u_max, w));
+}
+warper_product_1d_simple_t make_product_1d_simple_inverse(double u_max, double w) {
+  return warper_product_1d_simple_t(map1d::make_inverse(u_max, w));
+}
+warper_product_1d_simple_t make_product_1d_simple_inverse_square(double u_max, double w) {
+  return warper_product_1d_simple_t(map1d::make_inverse_square(u_max, w));
+}
+warper_product_1d_simple_t make_product_1d_simple_inverse_power(int power, double u_max, double w) {
+  return warper_product_1d_simple_t(map1d::make_inverse_power(power, u_max, w));
+}
+warper_product_1d_simple_t make_product_1d_simple_uniform(double u_max) {
+  return warper_product_1d_simple_t(map1d::make_uniform(u_max));
+}
+
+warper_product_1d_t make_product_1d_inverse_cube_alternate(int order, double time, double ws) {
+  warper_product_1d_t r;
+  r.reserve(order);
+  for (int n = 1; n <= order; ++n)
+    r.emplace_back(n % 2 == 0 ? make_product_1d_simple_inverse_power(3, time, ws)
+                              : make_product_1d_simple_inverse(time, ws));
+  return r;
+}
+
+warper_product_1d_interp_nearest_t make_product_1d_inverse_cube_alternate_interp_nearest(int order, double time,
+                                                                                       double ws, int n_pts) {
+  auto f1 = [ws](double t) -> double { return ws / (ws + t); };
+  auto f2 = [ws](double t) -> double { return ws * ws * ws / ((ws + t) * (ws + t) * (ws + t)); };
+  warper_product_1d_interp_nearest_t r;
+  r.reserve(order);
+  for (int n = 1; n <= order; ++n)
+    r.emplace_back(warper_product_1d_simple_interp_nearest_t(n % 2 == 0 ? std::function<double(double)>(f2)
+                                                                        : std::function<double(double)>(f1),
+                                                             time, n_pts));
+  return r;
+}
+
+warper_product_1d_interp_hybrid_t make_product_1d_inverse_cube_alternate_interp_hybrid(int order, double time,
+                                                                                     double ws, int n_pts) {
+  auto f1 = [ws](double t) -> double { return ws / (ws + t); };
+  auto f2 = [ws](double t) -> double { return ws * ws * ws / ((ws + t) * (ws + t) * (ws + t)); };
+  warper_product_1d_interp_hybrid_t r;
+  r.reserve(order);
+  for (int n = 1; n <= order; ++n) {
+    const bool even = n % 2 == 0;
+    r.emplace_back(warper_product_1d_simple_interp_hybrid_t(
+        map1d::make_hybrid(even ? std::function<double(double)>(f2) : std::function<double(double)>(f1), time, n_pts,
+                           even ? 3 : 1, ws)));
+  }
+  return r;
+}
+
+// --------------------------------------------------------------------------
+// pilot fit (make_warper_from_proj.hpp:52-153, gsl_fit_linear_wrap.hpp:66-113)
+namespace proj_details {
+
+std::pair<double, double> fit_wlinear(const double *x, const double *w, const double *y, size_t n) {
+  double W = 0, wm_x = 0, wm_y = 0, wm_dx2 = 0, wm_dxdy = 0;
+  for (size_t i = 0; i < n; i++) {
+    const double wi = w[i];
+    if (wi > 0) {
+      W += wi;
+      wm_x += (x[i] - wm_x) * (wi / W);
+      wm_y += (y[i] - wm_y) * (wi / W);
+    }
+  }
+  W = 0;
+  for (size_t i = 0; i < n; i++) {
+    const double wi = w[i];
+    if (wi > 0) {
+      const double dx = x[i] - wm_x, dy = y[i] - wm_y;
+      W += wi;
+      wm_dx2 += (dx * dx - wm_dx2) * (wi / W);
+      wm_dxdy += (dx * dy - wm_dxdy) * (wi / W);
+    }
+  }
+  const double b = wm_dxdy / wm_dx2;
+  const double a = wm_y - wm_x * b;
+  return {a, b};
+}
+
+vec kernel_smoothing(vec const &y_in, double sigma) {
+  const long N = long(y_in.size());
+  vec x(N), logy(N), out(N);
+  std::vector<long> dead;
+  for (long i = 0; i < N; i++) {
+    x[i] = double(i);
+    logy[i] = std::log(y_in[i]);
+    if (y_in[i] <= 0) dead.push_back(i);
+  }
+  const long H = std::max(std::min(long(3 * sigma), N), 1L);
+  const long K = 2 * H + 1;
+  vec kernel(K);
+  for (long j = 0; j < K; j++) kernel[j] = std::exp(-std::pow((double(j) - double(H)) / sigma, 2) / 2);
+  vec kc;
+  for (long i = 0; i < N; i++) {
+    const double *kp = kernel.data();
+    if (!dead.empty()) {
+      kc = kernel;
+      for (long j : dead) {
+        const long dp = j + H - i;
+        if (dp >= 0 && dp < K) kc[dp] = 0;
+      }
+      kp = kc.data();
+    }
+    const long start_ker = std::max(0L, H - i);
+    const long end_ker = std::min(N + H - i - 1, K - 1);
+    const long size_fit = end_ker - start_ker + 1;
+    const long start_x = start_ker - H + i;
+    auto c = fit_wlinear(x.data() + start_x, kp + start_ker, logy.data() + start_x, size_t(size_fit));
+    out[i] = std::exp(c.first + x[i] * c.second);
+  }
+  return out;
+}
+
+} // namespace proj_details
+
+warper_product_1d_interp_nearest_t make_warper_from_proj_samples(std::vector<vec> const &l, vec const &abs_values,
+                                                                 int order, int num_bins, double sigma) {
+  if (l.size() != abs_values.size()) throw keldy_error("sample and value counts differ");
+  if (num_bins < 2) throw keldy_error("num_bins must be >= 2");
+  const double bin_size = 1.0 / num_bins;
+  std::vector<vec> data(order, vec(num_bins, 0.0));
+  std::vector<std::vector<long>> count(order, std::vector<long>(num_bins, 0));
+  for (size_t s = 0; s < l.size(); s++) {
+    if (int(l[s].size()) != order) throw keldy_error("sample of wrong dimension");
+    for (int a = 0; a < order; a++) {
+      const double xv = l[s][a];
+      if (!(0.0 <= xv && xv <= 1.0)) continue;  // binner drops out-of-range values
+      const long b = (xv == 1.0) ? num_bins - 1 : long(xv / bin_size);
+      data[a][b] += abs_values[s];
+      count[a][b]++;
+    }
+  }
+  warper_product_1d_interp_nearest_t warper;
+  vec time_pts(num_bins + 1), f1_pts(num_bins + 1);
+  const double delta = 1.0 / num_bins;
+  for (int i = 0; i <= num_bins; i++) time_pts[i] = i * delta;
+  time_pts[num_bins] = 1.;
+  for (int a = 0; a < order; a++) {
+    for (int b = 0; b < num_bins; b++)
+      if (count[a][b] > 0) data[a][b] /= double(count[a][b]);
+    const vec sm = proj_details::kernel_smoothing(data[a], sigma / bin_size);
+    f1_pts[0] = 0;
+    for (int i = 0; i < num_bins; i++) f1_pts[i + 1] = 2 * sm[i] - f1_pts[i];
+    warper.emplace_back(warper_product_1d_simple_interp_nearest_t(time_pts, f1_pts));
+  }
+  return warper;
+}
+
+warper_product_1d_interp_nearest_t make_warper_from_proj(std::function<dcomplex(vec)> const &warped_integrand,
+                                                         int order, int num_bins, int nr_samples, double sigma) {
+  sobol g(order, 0);
+  std::vector<vec> l;
+  vec vals;
+  l.reserve(nr_samples);
+  vals.reserve(nr_samples);
+  for (int i = 0; i < nr_samples; i++) {
+    l.push_back(g());
+    vals.push_back(std::abs(warped_integrand(l.back())));
+  }
+  return make_warper_from_proj_samples(l, vals, order, num_bins, sigma);
+}
+
+// --------------------------------------------------------------------------
+map_pair warper_train_t::map_reverse(vec const &l, int start, int end) const {
+  if (start < 0 || end < 0 || start > int(size()) || end > int(size()) || end > start)
+    throw keldy_error("invalid warper domain range");
+  double j = 1.0;
+  vec x = l;
+  for (int q = start - 1; q >= end; q--) {
+    auto r = std::visit([&x](auto const &w) { return w.map_reverse(x); }, warpers[q]);
+    x = std::move(r.first);
+    j *= r.second;
+  }
+  return {x, j};
+}
+
+map_pair warper_train_t::map_forward(vec const &u, int start, int end) const {
+  if (start < 0 || end < 0 || start > int(size()) || end > int(size()) || start > end)
+    throw keldy_error("invalid warper domain range");
+  double j = 1.0;
+  vec x = u;
+  for (int q = start; q < end; q++) {
+    auto r = std::visit([&x](auto const &w) { return w.map_forward(x); }, warpers[q]);
+    x = std::move(r.first);
+    j *= r.second;
+  }
+  return {x, j};
+}
+
+vec warper_train_t::ui_from_li(vec const &l) const {
+  vec x = l;
+  for (auto it = warpers.crbegin(); it != warpers.crend(); ++it)
+    x = std::visit([&x](auto const &w) { return w.ui_from_li(x); }, *it);
+  return x;
+}
+
+vec warper_train_t::li_from_ui(vec const &u) const {
+  vec x = u;
+  for (auto const &w : warpers) x = std::visit([&x](auto const &ww) { return ww.li_from_ui(x); }, w);
+  return x;
+}
+
+device_chain warper_train_t::to_device() const {
+  device_chain c;
+  for (auto const &w : warpers) std::visit([&c](auto const &ww) { ww.to_device(c); }, w);
+  return c;
+}
+
+} // namespace keldy::warpers

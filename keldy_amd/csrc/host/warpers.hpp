@@ -5,6 +5,7 @@
 // that the QMC kernel applies in registers.
 #pragma once
 
+#include <atomic>
 #include <functional>
 #include <memory>
 #include <utility>
@@ -52,6 +53,11 @@ class map1d {
   kind_t kind() const { return kind_; }
   bool device_ok() const;
   kd_map1d to_device(std::vector<double> &tables) const;
+  // tabulated maps: knot times, normalised f1 knots, normalised negated CDF knots
+  vec const &knot_times() const { return t_; }
+  vec const &knot_f1() const { return f_; }
+  vec const &knot_F() const { return Fk_; }
+  int cut_index() const { return cut_; }
 
  private:
   kind_t kind_ = uniform;
@@ -233,39 +239,29 @@ vec kernel_smoothing(vec const &y_in, double sigma);
 std::pair<double, double> fit_wlinear(const double *x, const double *w, const double *y, size_t n);
 } // namespace proj_details
 
-/// Learned coupling-flow warper (device forward/backward kernels); parameters
-/// in the device layout of kd_flow_*.
-class warper_flow_t {
- public:
-  std::shared_ptr<void> handle; // owned by the flow trainer
-  int dim = 0;
-  map_pair map_reverse(vec const &l) const;
-  map_pair map_forward(vec const &u) const;
-  vec ui_from_li(vec const &l) const { return map_reverse(l).first; }
-  vec li_from_ui(vec const &u) const { return map_forward(u).first; }
-  double jacobian_reverse(vec const &l) const { return map_reverse(l).second; }
-  double jacobian_forward(vec const &u) const { return map_forward(u).second; }
-  void to_device(device_chain &c) const;
-};
-
 using warper_variant =
     std::variant<warper_identity_t, warper_plasma_uv_t, warper_product_1d_simple_interp_nearest_t,
                  warper_product_1d_simple_t, warper_product_1d_t, warper_product_1d_interp_nearest_t,
-                 warper_product_1d_interp_hybrid_t, warper_product_1d_simple_interp_hybrid_t, warper_flow_t>;
+                 warper_product_1d_interp_hybrid_t, warper_product_1d_simple_interp_hybrid_t>;
 
 /// Ordered chain of warpers; map_reverse applies them last-added first
 /// (warpers.hpp:41-137).
+inline uint64_t next_stamp() {
+  static std::atomic<uint64_t> counter{1};
+  return counter.fetch_add(1);
+}
+
 class warper_train_t {
  public:
   std::vector<warper_variant> warpers;
   template <typename W> void emplace_back(W w) {
     warpers.emplace_back(std::move(w));
-    version++;
+    stamp = next_stamp();
   }
   void reserve(size_t n) { warpers.reserve(n); }
   void clear() {
     warpers.clear();
-    version++;
+    stamp = next_stamp();
   }
   size_t size() const { return warpers.size(); }
   map_pair map_reverse(vec const &l, int start_domain_nr, int end_domain_nr) const;
@@ -277,7 +273,9 @@ class warper_train_t {
   vec ui_from_li(vec const &l) const;
   vec li_from_ui(vec const &u) const;
   device_chain to_device() const;
-  uint64_t version = 0; // bumped on every change (device re-upload)
+  // Unique per content change across all trains; the integrator re-uploads
+  // the device chain when the stamp of its train differs from the uploaded one.
+  uint64_t stamp = next_stamp();
 };
 
 } // namespace keldy::warpers

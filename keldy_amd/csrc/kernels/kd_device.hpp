@@ -106,6 +106,8 @@ __device__ __forceinline__ double2 table_eval(const KdParams &p, const double2 *
   const double2 *g = tab + (size_t)(oa * p.n_orb + ob) * p.n_pts;
   double a = (t - p.t_min) / p.dt;
   long i = (long)floor(a);
+  // outside the mesh by more than rounding: NaN (the reference's gf evaluation throws)
+  const bool out = !(a >= -1e-9) || !(a <= (double)(p.n_pts - 1) + 1e-9);
   i = i > p.n_pts - 2 ? p.n_pts - 2 : i;
   i = i < 0 ? 0 : i;
   double w = a - (double)i;
@@ -113,6 +115,7 @@ __device__ __forceinline__ double2 table_eval(const KdParams &p, const double2 *
   double2 r;
   r.x = (1.0 - w) * g0.x + w * g1.x;
   r.y = (1.0 - w) * g0.y + w * g1.y;
+  if (out) r.x = r.y = __builtin_nan("");
   return r;
 }
 

@@ -388,39 +388,50 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double *__re
     for (int c = 0; c < 4; c++) acc[c] += sh[c][0];
 }
 
+template <int N> static hipError_t launch_prep_n(const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((prep_kernel<N>), dim3(grid), dim3(256), 0, st, p, buf);
+  return hipGetLastError();
+}
+
 template <int N, typename T>
-static hipError_t launch_n(const KdParams &p, const PrepBuf &buf, int grid_prep, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((prep_kernel<N>), dim3(grid_prep), dim3(256), 0, st, p, buf);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+static hipError_t launch_lu_n(const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
   hipLaunchKernelGGL((integrand_kernel<N, T>), dim3(grid), dim3(64), 0, st, p, buf);
   return hipGetLastError();
 }
 
-template <typename T>
-static hipError_t launch_t(int order, const KdParams &p, const PrepBuf &buf, int gp, int g, hipStream_t st) {
-  switch (order) {
-  case 0: return launch_n<0, T>(p, buf, gp, g, st);
-  case 1: return launch_n<1, T>(p, buf, gp, g, st);
-  case 2: return launch_n<2, T>(p, buf, gp, g, st);
-  case 3: return launch_n<3, T>(p, buf, gp, g, st);
-  case 4: return launch_n<4, T>(p, buf, gp, g, st);
-  case 5: return launch_n<5, T>(p, buf, gp, g, st);
-  case 6: return launch_n<6, T>(p, buf, gp, g, st);
-  case 7: return launch_n<7, T>(p, buf, gp, g, st);
-  case 8: return launch_n<8, T>(p, buf, gp, g, st);
-  case 9: return launch_n<9, T>(p, buf, gp, g, st);
-  case 10: return launch_n<10, T>(p, buf, gp, g, st);
-  case 11: return launch_n<11, T>(p, buf, gp, g, st);
-  case 12: return launch_n<12, T>(p, buf, gp, g, st);
-  default: return hipErrorInvalidValue;
+#define KD_ORDER_SWITCH(ORDER, CALL)                                                                           \
+  switch (ORDER) {                                                                                             \
+  case 0: return CALL(0);                                                                                      \
+  case 1: return CALL(1);                                                                                      \
+  case 2: return CALL(2);                                                                                      \
+  case 3: return CALL(3);                                                                                      \
+  case 4: return CALL(4);                                                                                      \
+  case 5: return CALL(5);                                                                                      \
+  case 6: return CALL(6);                                                                                      \
+  case 7: return CALL(7);                                                                                      \
+  case 8: return CALL(8);                                                                                      \
+  case 9: return CALL(9);                                                                                      \
+  case 10: return CALL(10);                                                                                    \
+  case 11: return CALL(11);                                                                                    \
+  case 12: return CALL(12);                                                                                    \
+  default: return hipErrorInvalidValue;                                                                        \
   }
+
+hipError_t launch_prep(int order, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
+#define KD_PREP_CALL(n) launch_prep_n<n>(p, buf, grid, st)
+  KD_ORDER_SWITCH(order, KD_PREP_CALL)
+#undef KD_PREP_CALL
 }
 
-hipError_t launch_integrand(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid_prep,
-                            int grid, hipStream_t st) {
-  if (precision == KD_PREC_FP32_LU) return launch_t<float>(order, p, buf, grid_prep, grid, st);
-  return launch_t<double>(order, p, buf, grid_prep, grid, st);
+hipError_t launch_lu(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
+  if (precision == KD_PREC_FP32_LU) {
+#define KD_LU32_CALL(n) launch_lu_n<n, float>(p, buf, grid, st)
+    KD_ORDER_SWITCH(order, KD_LU32_CALL)
+#undef KD_LU32_CALL
+  }
+#define KD_LU64_CALL(n) launch_lu_n<n, double>(p, buf, grid, st)
+  KD_ORDER_SWITCH(order, KD_LU64_CALL)
+#undef KD_LU64_CALL
 }
 
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st) {

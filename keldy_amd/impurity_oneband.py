@@ -1,0 +1,19 @@
+"""keldy.impurity_oneband counterpart (python/keldy/impurity_oneband_desc.py).
+
+Model parameters are passed as a dict with the reference's keys (beta,
+bias_V, eps_d, Gamma, alpha, half_bandwidth, bath_type, time_max,
+nr_time_points_gf, ft_method) and defaults (model.hpp:37-50).
+"""
+from ._host import ext as _ext
+
+_m = _ext.impurity_oneband
+GfIndexT = _m.GfIndexT
+G0ModelOmega = _m.G0ModelOmega
+G0Model = _m.G0Model
+G0KeldyshContourT = _m.G0KeldyshContourT
+IntegrandGDirect = _m.IntegrandGDirect
+ComputeChargeQDirect = _m.ComputeChargeQDirect
+ComputeCurrentJDirect = _m.ComputeCurrentJDirect
+
+__all__ = ["GfIndexT", "G0ModelOmega", "G0Model", "G0KeldyshContourT", "IntegrandGDirect",
+           "ComputeChargeQDirect", "ComputeCurrentJDirect"]

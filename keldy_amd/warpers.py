@@ -1,0 +1,51 @@
+"""keldy.warpers counterpart (python/keldy/warpers_desc.py)."""
+import numpy as np
+
+from ._host import ext as _ext
+from .common import Sobol
+
+_w = _ext.warpers
+WarperIdentityT = _w.WarperIdentityT
+WarperPlasmaUvT = _w.WarperPlasmaUvT
+WarperProduct1dSimpleT = _w.WarperProduct1dSimpleT
+WarperProduct1dSimpleInterpNearestT = _w.WarperProduct1dSimpleInterpNearestT
+WarperProduct1dSimpleInterpHybridT = _w.WarperProduct1dSimpleInterpHybridT
+WarperProduct1dT = _w.WarperProduct1dT
+WarperProduct1dInterpNearestT = _w.WarperProduct1dInterpNearestT
+WarperProduct1dInterpHybridT = _w.WarperProduct1dInterpHybridT
+WarperTrainT = _w.WarperTrainT
+vi_from_ui = _w.vi_from_ui
+ui_from_vi = _w.ui_from_vi
+make_product_1d_simple_exponential = _w.make_product_1d_simple_exponential
+make_product_1d_simple_inverse = _w.make_product_1d_simple_inverse
+make_product_1d_simple_inverse_square = _w.make_product_1d_simple_inverse_square
+make_product_1d_simple_inverse_power = _w.make_product_1d_simple_inverse_power
+make_product_1d_simple_uniform = _w.make_product_1d_simple_uniform
+make_product_1d_inverse_cube_alternate = _w.make_product_1d_inverse_cube_alternate
+make_product_1d_inverse_cube_alternate_interp_nearest = _w.make_product_1d_inverse_cube_alternate_interp_nearest
+make_product_1d_inverse_cube_alternate_interp_hybrid = _w.make_product_1d_inverse_cube_alternate_interp_hybrid
+make_warper_from_proj_samples = _w.make_warper_from_proj_samples
+kernel_smoothing = _w.kernel_smoothing
+
+
+def make_warper_from_proj(warped_integrand, order, num_bins, nr_samples, sigma, keep_u_hypercube=False):
+    """Pilot-fitted product warper (make_warper_from_proj.hpp:118-153).
+
+    warped_integrand is either a callable l -> complex (evaluated once per
+    pilot point, as in the reference) or an integrator object
+    (ComputeChargeQDirect, ...) whose warped integrand, with its current warper
+    chain, is then evaluated for all pilot points in one device batch --
+    without the positive-time restriction by default, as the reference's own
+    recipe does (projection_test.py:86), so the g0 window must cover
+    order * t.
+    """
+    batch = getattr(warped_integrand, "evaluate_warped_integrand_batch", None)
+    if batch is None:
+        return _w.make_warper_from_proj(warped_integrand, order, num_bins, nr_samples, sigma)
+    g = Sobol(order, 0)
+    l = np.array([g() for _ in range(nr_samples)], dtype=np.float64).reshape(nr_samples, order)
+    vals, _, _ = batch(l, keep_u_hypercube)
+    return _w.make_warper_from_proj_samples(l, np.abs(vals), order, num_bins, sigma)
+
+
+__all__ = [n for n in dir() if n[0].isupper() or n.startswith(("make_", "vi_", "ui_", "kernel_"))]

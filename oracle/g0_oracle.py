@@ -79,20 +79,50 @@ def g0_dot_omega(w, lesser, p):
     return 1j * out.imag
 
 
-def g0_real_axis(t, lesser, p, epsabs=1e-13, epsrel=1e-11):
-    """(1/2pi) int e^{-i w t} g(w) dw for purely imaginary g(w) = i h(w)."""
-    h = lambda w: g0_dot_omega(w, lesser, p).imag
-    if t == 0.0:
-        val = 0.0
-        for a, b in ((-np.inf, 0.0), (0.0, np.inf)):
-            val += si.quad(h, a, b, epsabs=epsabs, epsrel=epsrel, limit=400)[0]
-        return 1j * val / (2 * np.pi)
-    # i h (cos wt - i sin wt) = h sin wt + i h cos wt
-    at = abs(t)
+def _fourier(h, t, epsabs, epsrel, finite=None):
+    """(1/2pi) int e^{-i w t} i h(w) dw for a real function h."""
+    if finite is not None:
+        a, b = finite
+        if t == 0.0:
+            return 1j * si.quad(h, a, b, epsabs=epsabs, epsrel=epsrel, limit=2000)[0] / (2 * np.pi)
+        c = si.quad(h, a, b, weight="cos", wvar=t, epsabs=epsabs, epsrel=epsrel, limit=2000)[0]
+        s = si.quad(h, a, b, weight="sin", wvar=t, epsabs=epsabs, epsrel=epsrel, limit=2000)[0]
+        return (s + 1j * c) / (2 * np.pi)
     c = s = 0.0
+    if abs(t) < 1.0:
+        for a, b in ((-np.inf, 0.0), (0.0, np.inf)):
+            c += si.quad(lambda w: h(w) * np.cos(w * t), a, b, epsabs=epsabs, epsrel=epsrel, limit=2000)[0]
+            s += si.quad(lambda w: h(w) * np.sin(w * t), a, b, epsabs=epsabs, epsrel=epsrel, limit=2000)[0]
+        return (s + 1j * c) / (2 * np.pi)
+    at = abs(t)
     for sgn in (1.0, -1.0):
         f = lambda w: h(sgn * w)
         c += si.quad(f, 0.0, np.inf, weight="cos", wvar=at, epsabs=epsabs, limlst=200)[0]
         s += sgn * si.quad(f, 0.0, np.inf, weight="sin", wvar=at, epsabs=epsabs, limlst=200)[0]
     s *= np.sign(t)
     return (s + 1j * c) / (2 * np.pi)
+
+
+def g0_real_axis(t, lesser, p, epsabs=1e-13, epsrel=1e-11):
+    """(1/2pi) int e^{-i w t} g(w) dw with g(w) = i h(w) purely imaginary.
+
+    Flat band: the T=0, eps_d=0, V=0 spectral function of the same Gamma,
+    2 Gamma theta(-+w)/(w^2+Gamma^2), is subtracted and added back through its
+    closed-form transform (the Ei formula of make_flat_band_analytic), which
+    leaves a remainder decaying like 1/w^3 for the quadrature.  Semicircle:
+    compact support [-D, D].
+    """
+    h = lambda w: g0_dot_omega(w, lesser, p).imag
+    G = p["Gamma"]
+    if p["bath_type"] == "flatband":
+        ref = flatband_analytic_lesser(t, G)
+        if lesser:
+            h_ref = lambda w: 2 * G / (w * w + G * G) if w < 0 else (G / (G * G) if w == 0 else 0.0)
+        else:
+            ref = np.conj(ref)
+            h_ref = lambda w: -2 * G / (w * w + G * G) if w > 0 else (-G / (G * G) if w == 0 else 0.0)
+        return complex(ref) + _fourier(lambda w: h(w) - h_ref(w), t, epsabs, epsrel)
+    if p["bath_type"] == "semicircle":
+        D = p["half_bandwidth"]
+        return _fourier(h, t, epsabs, epsrel, finite=(-D, D))
+    return _fourier(h, t, epsabs, epsrel)

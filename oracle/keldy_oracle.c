@@ -596,10 +596,10 @@ int ko_integrand_eval_no_det(const ko_integrand *f, int n, const double *times,
 /* ------------------------------------------------------------------------ */
 /* integrator::evaluate_warped_integrand (integrator.hpp:68-79) */
 int ko_eval_warped(const ko_integrand *f, const ko_warper *w, int nw, int n, const double *l,
-                   double out[2]) {
+                   int keep_u_hypercube, double out[2]) {
   double u[KO_MAX_DIM];
   double jac = ko_warp_reverse(w, nw, n, l, u);
-  int dom = ko_integrand_eval(f, n, u, 1, out);
+  int dom = ko_integrand_eval(f, n, u, keep_u_hypercube, out);
   out[0] *= jac;
   out[1] *= jac;
   return dom;
@@ -612,7 +612,7 @@ int ko_run(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint6
   int n = s->dim;
   for (uint64_t i = 0; i < n_steps; i++) {
     if (ko_sobol_next(s, l) != 0) return -1;
-    int dom = ko_eval_warped(f, w, nw, n, l, v);
+    int dom = ko_eval_warped(f, w, nw, n, l, 1, v);
     acc->n_in_domain += (uint64_t)dom;
     acc->re += v[0];
     acc->im += v[1];

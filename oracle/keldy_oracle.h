@@ -132,7 +132,7 @@ typedef struct {
 } ko_accum;
 
 int ko_eval_warped(const ko_integrand *f, const ko_warper *w, int nw, int n, const double *l,
-                   double out[2]);
+                   int keep_u_hypercube, double out[2]);
 /* integrator::run over Sobol points k = s->k+1 ... ; returns 0 or -1 on exhaustion */
 int ko_run(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint64_t n_steps,
            ko_accum *acc);

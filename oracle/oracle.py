@@ -93,7 +93,7 @@ def lib():
         L.ko_integrand_eval_no_det.argtypes = [C.POINTER(_Integrand), C.c_int, dp, C.c_int, dp]
         L.ko_det.argtypes = [dp, C.c_int, dp]
         L.ko_eval_warped.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int, C.c_int,
-                                     dp, dp]
+                                     dp, C.c_int, dp]
         L.ko_run.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int, C.POINTER(_Sobol),
                              C.c_uint64, C.POINTER(_Accum)]
         L.ko_run_parallel.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int,
@@ -342,11 +342,11 @@ def det(m):
     return complex(out[0], out[1])
 
 
-def eval_warped(integrand, train, l):
+def eval_warped(integrand, train, l, keep_u_hypercube=True):
     l = np.ascontiguousarray(l, dtype=np.float64)
     out = np.zeros(2)
     dom = lib().ko_eval_warped(C.byref(integrand.c), train.arr, len(train.warpers), len(l), _dp(l),
-                               _dp(out))
+                               int(keep_u_hypercube), _dp(out))
     return complex(out[0], out[1]), dom
 
 

@@ -1,0 +1,170 @@
+"""The host API (reference names) on the GPU: known answers of the reference's
+own tests and parity with the CPU oracle on the same points.
+
+Known answers: Horvatic-Zlatic charge series (compute_obs_test.cpp:93-181),
+orders 1-2 explicit Wick contractions (wick_direct_test.cpp:61-206).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HZ = [0.5, -0.15915494309189533577, 0.050660591821168885722, 0.0046743460474992114797,
+      -0.015882884762010443238, 0.0071422698935926388204, 0.0016484718800775438296]
+I_N = [1, 1j, -1, -1j]
+
+
+def _params(**kw):
+    p = dict(beta=-1.0, bias_V=0.0, eps_d=0.0, Gamma=1.0, time_max=30.0, nr_time_points_gf=10001,
+             alpha=0.0, bath_type="flatband", ft_method="analytic")
+    p.update(kw)
+    return p
+
+
+@pytest.fixture(scope="module")
+def model30():
+    from keldy_amd import impurity_oneband as kio
+    return kio.G0Model(kio.G0ModelOmega(_params()), False)
+
+
+def _oracle_for(model, t):
+    g0 = O.G0Table(model.g0_lesser[0, 0], model.g0_greater[0, 0], model.time_max)
+    return O.charge_integrand(g0, t)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6])
+def test_charge_series_inverse_square(model30, order):
+    """compute_obs_test.cpp FlatbandAnalyticCompare1: 1e4 unshifted Sobol points,
+    [plasma_uv(20), inverse_square(20, 1)], 1e-3 rel (n <= 4) / 1e-2 (n = 5, 6)."""
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t = 20.0
+    c = kio.ComputeChargeQDirect(model30, t, order, 0.0)
+    c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    c.warper.emplace_back(kw.make_product_1d_simple_inverse_square(t, 1.0))
+    c.run(10000)
+    res = c.reduce_result()
+    q = -1j * I_N[order % 4] * res
+    tol = 1e-3 if order <= 4 else 1e-2
+    assert abs(q - HZ[order]) <= tol * abs(HZ[order])
+    # same points through the oracle
+    tr = O.Train([O.plasma(t), O.product(O.Map1d("inverse_square", t, w=1.0))])
+    s, npo, ndom = O.run(_oracle_for(model30, t), tr, O.Sobol(order, 0), 10000, threads=4)
+    assert c.reduce_nr_points_run() == npo and c.reduce_nr_points_in_domain() == ndom
+    assert abs(res * npo - s) <= 1e-10 * abs(s)
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_charge_series_exponential(model30, order):
+    """FlatbandAnalyticCompare2: exponential(20, 2) warper, 5e4 points, 1e-2 rel."""
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t = 20.0
+    c = kio.ComputeChargeQDirect(model30, t, order, 0.0)
+    c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    c.warper.emplace_back(kw.make_product_1d_simple_exponential(t, 2.0))
+    c.run(50000)
+    q = -1j * I_N[order % 4] * c.reduce_result()
+    assert abs(q - HZ[order]) <= 1e-2 * abs(HZ[order])
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_integrand_vs_explicit_wick(model30, order):
+    """wick_direct_test.cpp:61-206: determinant sum == explicit contractions."""
+    from keldy_amd import impurity_oneband as kio
+    g0 = kio.G0KeldyshContourT(model30)
+    f = kio.IntegrandGDirect(g0, kio.GfIndexT(20.0, 0, 0), kio.GfIndexT(20.0, 0, 1), 0.0)
+    rng = np.random.default_rng(order)
+    for _ in range(20):
+        times = list(rng.uniform(0, 20.0, size=order))
+        v, d = f(times)
+        w, e = f.eval_no_det(times)
+        assert d == e == 1
+        assert abs(v - w) <= 1e-13 * abs(w) + 1e-16
+
+
+def test_integrand_call_and_domain(model30):
+    from keldy_amd import impurity_oneband as kio
+    g0 = kio.G0KeldyshContourT(model30)
+    f = kio.IntegrandGDirect(g0, kio.GfIndexT(20.0, 0, 0), kio.GfIndexT(20.0, 0, 1), 0.0)
+    assert f([]) == (0.5j, 1)                       # order 0: g^<(0) = i/2
+    assert f([1.0, -0.5]) == (0j, 0)                # negative time -> out of domain
+    v, d = f([1.0, -0.5], False)                    # keep_u_hypercube = False evaluates
+    assert d == 1 and v != 0
+    f_spin = kio.IntegrandGDirect(g0, kio.GfIndexT(20.0, 0, 0), kio.GfIndexT(20.0, 1, 1), 0.0)
+    assert f_spin([1.0]) == (0j, 0)                 # spin mismatch
+    big = kio.IntegrandGDirect(g0, kio.GfIndexT(20.0, 0, 0), kio.GfIndexT(20.0, 0, 1), 1e6)
+    assert big([3.0, 1.0, 2.0]) == (0j, 1)          # cutoff
+
+
+def test_errors(model30):
+    from keldy_amd import impurity_oneband as kio
+    g0 = kio.G0KeldyshContourT(model30)
+    with pytest.raises(RuntimeError, match="negative time"):
+        kio.IntegrandGDirect(g0, kio.GfIndexT(-1.0, 0, 0), kio.GfIndexT(1.0, 0, 1))
+    with pytest.raises(RuntimeError, match="out of g0 time window"):
+        kio.IntegrandGDirect(g0, kio.GfIndexT(31.0, 0, 0), kio.GfIndexT(1.0, 0, 1))
+    with pytest.raises(RuntimeError, match="make_dot_lead"):
+        kio.IntegrandGDirect(g0, kio.GfIndexT(1.0, 0, 0, 0, 1), kio.GfIndexT(1.0, 0, 1))
+    with pytest.raises(RuntimeError, match="must be > 0"):
+        kio.ComputeChargeQDirect(model30, 10.0, 0, 0.0)
+    c = kio.ComputeChargeQDirect(model30, 10.0, 2, 0.0)
+    with pytest.raises(RuntimeError, match="No other rng"):
+        c.reset_rng("unknown", 0)
+
+
+def test_run_continuation_and_reset(model30):
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t = 10.0
+    a = kio.ComputeChargeQDirect(model30, t, 4, 0.0)
+    b = kio.ComputeChargeQDirect(model30, t, 4, 0.0)
+    for c in (a, b):
+        c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+        c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+        c.reset_rng("sobol", 5, True, True, 2)
+    a.run(3000)
+    b.run(1000)
+    b.run(2000)
+    assert a.reduce_nr_points_run() == b.reduce_nr_points_run() == 3000
+    assert abs(a.reduce_result() - b.reduce_result()) <= 1e-13 * abs(a.reduce_result())
+
+
+def test_warped_single_vs_batch_and_partial_chain(model30):
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t = 10.0
+    c = kio.ComputeChargeQDirect(model30, t, 3, 0.0)
+    c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    c.warper.emplace_back(kw.make_product_1d_simple_inverse_square(t, 1.0))
+    l = O.Sobol(3, 0, True, True, 1).points(32)
+    vals, dom, jac = c.evaluate_warped_integrand_batch(l)
+    for i in range(8):
+        v, d = c.evaluate_warped_integrand(list(l[i]))
+        assert abs(v - vals[i]) <= 1e-14 * max(abs(v), 1e-300) and d == dom[i]
+    # partial chain: only the last-added warper applied, no domain check
+    u, j = c.warper.map_reverse(list(l[0]), 1, 0)
+    v, d = c.evaluate_warped_integrand(list(l[0]), 1, False)
+    w, e = c.get_integrand()(u, False)
+    assert abs(v - w * j) <= 1e-14 * abs(v)
+
+
+def test_pilot_fit_callable_vs_batch(model30):
+    """make_warper_from_proj through the per-point callback (reference form)
+    and through one device batch give the same fitted maps."""
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t = 10.0
+    c = kio.ComputeChargeQDirect(model30, t, 3, 0.0)
+    c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+    w1 = kw.make_warper_from_proj(lambda l: c.evaluate_warped_integrand(l)[0], 3, 50, 512, 0.1)
+    w2 = kw.make_warper_from_proj(c, 3, 50, 512, 0.1)
+    for a in range(3):
+        k1, k2 = w1.get_1d_warper(a).knots(), w2.get_1d_warper(a).knots()
+        np.testing.assert_allclose(k1[1], k2[1], rtol=1e-12)
+    c.warper.emplace_back(w2)
+    c.run(2000)
+    assert c.reduce_nr_points_run() == 2000

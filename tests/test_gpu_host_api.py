@@ -160,11 +160,13 @@ def test_pilot_fit_callable_vs_batch(model30):
     c = kio.ComputeChargeQDirect(model30, t, 3, 0.0)
     c.warper.emplace_back(kw.WarperPlasmaUvT(t))
     c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
-    w1 = kw.make_warper_from_proj(lambda l: c.evaluate_warped_integrand(l)[0], 3, 50, 512, 0.1)
-    w2 = kw.make_warper_from_proj(c, 3, 50, 512, 0.1)
-    for a in range(3):
-        k1, k2 = w1.get_1d_warper(a).knots(), w2.get_1d_warper(a).knots()
-        np.testing.assert_allclose(k1[1], k2[1], rtol=1e-12)
+    for keep in (True, False):
+        n_w = c.warper.size()
+        w1 = kw.make_warper_from_proj(lambda l: c.evaluate_warped_integrand(l, n_w, keep)[0], 3, 50, 512, 0.1)
+        w2 = kw.make_warper_from_proj(c, 3, 50, 512, 0.1, keep_u_hypercube=keep)
+        for a in range(3):
+            k1, k2 = w1.get_1d_warper(a).knots(), w2.get_1d_warper(a).knots()
+            np.testing.assert_allclose(k1[1], k2[1], rtol=1e-12)
     c.warper.emplace_back(w2)
     c.run(2000)
     assert c.reduce_nr_points_run() == 2000

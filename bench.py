@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Benchmark: wall time for the charge Q_8(t=10) to reach 1e-3 relative error
-on one MI355X.
+on one MI355X, learned flow warper vs plain warper.
 
 Workload (the configuration the metric is quoted on): single-orbital Anderson
 impurity, flat-band bath Gamma=1, T=0, V=0, eps_d=0, g0 tabulated analytically
@@ -13,12 +13,19 @@ Plain warper chain (the reference's fitted product-1D recipe):
     [plasma_uv(10), product_1d_simple_inverse(10, 1.0),
      make_warper_from_proj(order 8, 500 bins, 2^14 pilot points, sigma 0.1)]
 
-value = (points needed for 1e-3 relative error) / (points per second).  The
-points needed come from a doubling search: 16 scrambled replicas at M points,
-M = 2^12, 2^13, ... until the relative spread of one replica's estimate is
-<= 1e-3, then log-log interpolation between the last two M.  Points per second
-come from the timed steps (`--points` per step).  The one-off set-up (g0
-tabulation and the pilot fit) is reported separately as setup_s.
+Learned estimator: a 6-layer affine-coupling flow (hidden width 64, bf16 MFMA
+products) appended to the plain chain (applied first to the Sobol point),
+trained on the device for 2000 Adam steps of 2^14 points on 2^20 samples of
+|plain warped integrand| (scramble seed 1000003, disjoint from the replicas).
+
+For each estimator: time to 1e-3 = (points needed for 1e-3 relative error) /
+(points per second).  The points needed come from a doubling search: 16
+scrambled replicas at M points, M = 2^12, 2^13, ... until the relative spread
+of one replica's estimate is <= 1e-3, then log-log interpolation between the
+last two M.  Points per second come from the timed steps (`--points` per step,
+replica seeds 0..steps-1, the same Sobol points for both estimators).  value =
+the learned estimator's time (the plain one's with --no-flow).  One-off set-up
+(g0 tabulation, pilot fit, flow training) is reported separately.
 """
 import argparse
 import json
@@ -33,6 +40,21 @@ if ROOT not in sys.path:
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 TARGET_REL = 1e-3
+
+
+class _stdout_to_stderr:
+    """The host library prints the reference's model warnings on stdout (as
+    keldy does); keep them off the one-JSON-line stdout of this benchmark."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
 
 
 def model_params():
@@ -149,6 +171,11 @@ def main():
     ap.add_argument("--pilot-points", type=int, default=1 << 14)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-flow", action="store_true", help="plain estimator only")
+    ap.add_argument("--flow-steps", type=int, default=2000)
+    ap.add_argument("--flow-samples", type=int, default=1 << 20)
+    ap.add_argument("--flow-batch", type=int, default=1 << 14)
+    ap.add_argument("--flow-lr", type=float, default=1e-3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -160,65 +187,101 @@ def main():
 
     t_max = 10.0
     t0 = time.perf_counter()
-    model, comp, proj = plain_computer(args.order, t_max, args.pilot_points, args.precision)
+    with _stdout_to_stderr():
+        model, comp, proj = plain_computer(args.order, t_max, args.pilot_points, args.precision)
     setup_s = time.perf_counter() - t0
-    if args.warmup:
-        run_replicas(comp, 1000, args.warmup, args.points)
-    est, dom, lu_ms, walls = run_replicas(comp, 0, args.steps, args.points)
-    wall = float(walls.sum())
 
-    mean = est.mean()
-    n_total = args.steps * args.points
-    pps = n_total / wall
-    n_needed, table, wall_at_target = points_for_target(comp, 16, TARGET_REL)
-    t_needed = n_needed / pps
-    charge = (-1j * (1j ** args.order) * mean).real
-    rel_rep = est.std(ddof=1) / abs(mean) if len(est) > 1 else float("nan")
+    def measure(label):
+        if args.warmup:
+            run_replicas(comp, 1000, args.warmup, args.points)
+        est, dom, lu_ms, walls = run_replicas(comp, 0, args.steps, args.points)
+        wall = float(walls.sum())
+        n_total = args.steps * args.points
+        pps = n_total / wall
+        n_needed, table, wall_at_target = points_for_target(comp, 16, TARGET_REL)
+        mean = est.mean()
+        return {"warper": label, "est": est, "dom": dom, "lu_ms": lu_ms, "wall": wall, "pps": pps,
+                "n_needed": n_needed, "t_needed": n_needed / pps, "table": table,
+                "wall_at_target": wall_at_target, "mean": mean,
+                "charge": (-1j * (1j ** args.order) * mean).real,
+                "rel_rep": est.std(ddof=1) / abs(mean) if len(est) > 1 else float("nan"),
+                "in_domain_fraction": float(dom.sum()) / n_total,
+                "timing": comp.last_timing()}
+
+    def summary(m):
+        return {"warper": m["warper"], "time_to_1e-3_s": m["t_needed"], "charge_estimate": m["charge"],
+                "rel_err_per_replica_at_points": m["rel_rep"], "points_needed_1e-3": m["n_needed"],
+                "points_per_s": m["pps"], "ms_per_replica": m["wall"] / args.steps * 1e3,
+                "rqmc_spread_table": [[a, r] for a, r, _ in m["table"]],
+                "wall_s_one_replica_at_target": m["wall_at_target"],
+                "in_domain_fraction": m["in_domain_fraction"]}
+
+    plain = measure("plain")
+    learned = None
+    if not args.no_flow:
+        from keldy_amd import warpers as kw
+        t0 = time.perf_counter()
+        flow, losses = kw.train_flow_warper(comp, n_layers=6, nr_samples=args.flow_samples, steps=args.flow_steps,
+                                            batch=args.flow_batch, lr=args.flow_lr, lr_final=args.flow_lr * 1e-2)
+        train_s = time.perf_counter() - t0
+        comp.warper.emplace_back(flow)
+        learned = measure("learned flow")
+    head = learned or plain
 
     fpp = comp.flops_per_point()
-    flops = float(dom.sum()) * fpp
-    lu_s = lu_ms.sum() * 1e-3
+    flops = float(head["dom"].sum()) * fpp
+    lu_s = head["lu_ms"].sum() * 1e-3
     achieved = flops / lu_s / 1e12
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     launches_per_step = max(1, -(-args.points // (1 << 21)))
 
+    estimator = {"plain": summary(plain), "setup_s": setup_s}
+    if learned:
+        pe, le = plain["est"], learned["est"]
+        err = np.hypot(pe.std(ddof=1), le.std(ddof=1)) / np.sqrt(len(pe))
+        estimator.update({
+            "learned": summary(learned),
+            "speedup_learned_vs_plain": plain["t_needed"] / learned["t_needed"],
+            "variance_ratio_plain_over_learned_at_points": float(pe.var(ddof=1) / le.var(ddof=1)),
+            "unbiasedness": {"mean_plain": complex(pe.mean()).real, "mean_learned": complex(le.mean()).real,
+                             "difference_over_sigma": float(abs(le.mean() - pe.mean()) / err)},
+            "flow_training": {"steps": args.flow_steps, "samples": args.flow_samples, "batch": args.flow_batch,
+                              "lr": args.flow_lr, "seconds": train_s, "loss_first": float(losses[0]),
+                              "loss_last": float(np.mean(losses[-20:]))},
+        })
     out = {
         "metric": "time to 1e-3 rel. error on charge at n=8: learned vs plain warper, 1 GPU",
-        "value": t_needed,
+        "value": head["t_needed"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["wall"] / args.steps * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32-lu/f64-sum",
         "data": "synthetic (scrambled Sobol points; analytic flat-band g0)",
         "config": {"workload": "charge Q_n, n=8, t=10, flatband Gamma=1 T=0 V=0 eps_d=0, "
-                               "plain fitted product-1D warper (estimator: plain)",
+                               + ("learned 6-layer coupling flow on the plain fitted product-1D chain"
+                                  if learned else "plain fitted product-1D warper"),
                    "order": args.order, "points_per_replica": args.points, "replicas": args.steps,
                    "pilot_points": args.pilot_points},
-        "estimator": {"warper": "plain", "charge_estimate": charge,
-                      "rel_err_per_replica_at_points": rel_rep,
-                      "points_needed_1e-3": n_needed, "points_per_s": pps,
-                      "rqmc_spread_table": [[m, r] for m, r, _ in table],
-                      "wall_s_one_replica_at_target": wall_at_target,
-                      "setup_s": setup_s,
-                      "in_domain_fraction": float(dom.sum()) / n_total},
+        "estimator": estimator,
         "roofline": {"bound": "valu-fp64" if args.precision == "fp64" else "valu-fp32",
                      "kernel": f"integrand_kernel<{args.order},{'double' if args.precision == 'fp64' else 'float'}>",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": None, "flops_per_point": fpp,
-                     "lu_ms_per_launch": float(lu_ms.mean()) / launches_per_step},
+                     "lu_ms_per_launch": float(head["lu_ms"].mean()) / launches_per_step},
     }
     if not args.no_cpu:
-        cb = cpu_baseline(model, proj, args.order, t_max, n_needed, args.cpu_seconds)
+        # the CPU runs the plain estimator: its time to 1e-3 = plain points needed / CPU points/s
+        cb = cpu_baseline(model, proj, args.order, t_max, plain["n_needed"], args.cpu_seconds)
         out["cpu_baseline"] = {"value": cb["value"], "unit": "s", "cores": cb["cores"], "kind": "port",
-                               "sample": f"{cb['sample_points']} scrambled Sobol points (seed 0) of the same "
+                               "sample": f"{cb['sample_points']} scrambled Sobol points (seed 0) of the plain "
                                          f"chain in {cb['seconds']:.1f} s ({cb['pps']:.0f} points/s)",
                                "points_per_s": cb["pps"]}
-        out["gpu_vs_cpu"] = cb["value"] / t_needed
+        out["plain_gpu_vs_cpu"] = cb["value"] / plain["t_needed"]
     print(json.dumps(out))
 
 

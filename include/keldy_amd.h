@@ -37,9 +37,12 @@ extern "C" {
 #define KD_MAX_ORDER 12
 
 /* LU precision of the Keldysh sum; the sum itself and the QMC accumulation are
- * always FP64. */
+ * always FP64.  KD_PREC_FP32_LU keeps the FP64 LU from order 10 on (and for
+ * configurations flagged by the pivot check at any order); KD_PREC_FP32_LU_ALL
+ * uses FP32 at every order. */
 #define KD_PREC_FP64 0
 #define KD_PREC_FP32_LU 1
+#define KD_PREC_FP32_LU_ALL 2
 
 typedef struct kd_ctx kd_ctx;
 
@@ -152,6 +155,47 @@ int kd_last_timing(const kd_ctx *ctx, double *total_ms, double *lu_ms, double *p
 /* Executed FP operations per point of the last configured order/precision
  * (the algorithm's own count, used for roofline figures). */
 int kd_flops_per_point(const kd_ctx *ctx, double *flops);
+
+/* ---- launch shape and batched runs ---------------------------------------
+ * No counterpart in the reference (its integrator runs one order per object).
+ * kd_autotune times the Keldysh-sum kernel on sample_points Sobol points of
+ * the ctx's chain (from the current generator state, which does not advance;
+ * the accumulator is untouched) for every lanes-per-point x waves-per-CU
+ * candidate and keeps the fastest for this ctx.  kd_run_batch runs several
+ * ctxs (e.g. orders 1..10, uneven point counts) as ONE hipGraph launch: one
+ * graph branch per ctx (its persistent kernels on its own stream), forked from
+ * and joined to ctxs[0]'s stream; each ctx advances and accumulates as by
+ * kd_run.  ms = wall time of the graph launch. */
+int kd_autotune(kd_ctx *ctx, int64_t sample_points, int32_t *lanes, int32_t *waves_per_cu, double *ms_per_mpoint);
+int kd_set_launch_shape(kd_ctx *ctx, int32_t lanes, int32_t waves_per_cu);
+int kd_get_launch_shape(const kd_ctx *ctx, int32_t *lanes, int32_t *waves_per_cu);
+int kd_run_batch(kd_ctx *const *ctxs, const uint64_t *n_steps, int32_t count, kd_accum *out, double *ms);
+
+/* ---- learned coupling-flow warper (KD_WARP_FLOW) -------------------------
+ * No counterpart in the reference: an importance map learned from |integrand|
+ * samples, used as the last warper of a chain (applied first to the Sobol
+ * point).  Map, Jacobian, parameter layout and loss: keldy_amd/csrc/kernels/
+ * kd_flow_device.hpp.  One flow per ctx, of dimension `order` with n_layers
+ * affine-coupling layers of hidden width 64. */
+int kd_flow_n_params(int32_t order, int32_t n_layers, int64_t *n);
+/* identity-initialised flow (W3 = b3 = 0; He-uniform W1, W2 from seed); resets Adam */
+int kd_flow_init(kd_ctx *ctx, int32_t order, int32_t n_layers, uint64_t seed);
+int kd_flow_set_params(kd_ctx *ctx, int32_t order, int32_t n_layers, const float *params, int64_t n);
+int kd_flow_get_params(kd_ctx *ctx, float *params, int64_t n);
+/* x = T(l) and dx/dl at n points (host memory, row-major n x order) */
+int kd_flow_map(kd_ctx *ctx, const double *l, int64_t n, double *x, double *jac);
+/* loss = mean_p (w_p / w_mean) (-log q(x_p)) over the n <= 2^18 points and its
+ * gradient (n_params floats), q the flow's density; host memory */
+int kd_flow_loss_grad(kd_ctx *ctx, const double *x, const double *w, int64_t n, double w_mean, float *grad,
+                      double *loss);
+/* Training set: the next n_samples Sobol points of the ctx's generator (which
+ * advances) pushed through the warper chain WITHOUT the flow; keeps the points
+ * and |warped integrand| on the device; returns mean |warped integrand|. */
+int kd_flow_collect(kd_ctx *ctx, int64_t n_samples, double *w_mean);
+/* Adam steps on consecutive aligned batches of the training set (batch | n_samples,
+ * 64 | batch, batch <= 2^16), learning rate cosine-decayed from lr to lr_final; per-step loss
+ * into losses (steps doubles, may be NULL). */
+int kd_flow_train(kd_ctx *ctx, int32_t steps, int32_t batch, float lr, float lr_final, double *losses);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ from . import _build
 KD_OK = 0
 KD_ERR_ARG, KD_ERR_HIP, KD_ERR_STATE, KD_ERR_RANGE, KD_ERR_NODEVICE = -1, -2, -3, -4, -5
 KD_MAX_ORDER = 12
-KD_PREC_FP64, KD_PREC_FP32_LU = 0, 1
+KD_PREC_FP64, KD_PREC_FP32_LU, KD_PREC_FP32_LU_ALL = 0, 1, 2
 
 KD_MAP_UNIFORM, KD_MAP_EXPONENTIAL, KD_MAP_INVERSE, KD_MAP_INVERSE_SQ = 0, 1, 2, 3
 KD_MAP_INVERSE_POW, KD_MAP_NEAREST, KD_MAP_HYBRID = 4, 5, 6
@@ -49,7 +49,9 @@ SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_se
            "kd_set_integrand", "kd_set_precision", "kd_eval_batch", "kd_eval_batch_device",
            "kd_set_warper_chain", "kd_eval_warped_batch", "kd_set_sobol", "kd_get_sobol_state",
            "kd_run", "kd_get_accum", "kd_reset_accum", "kd_sample_device", "kd_last_timing",
-           "kd_flops_per_point"]
+           "kd_flops_per_point", "kd_flow_n_params", "kd_flow_init", "kd_flow_set_params",
+           "kd_flow_get_params", "kd_flow_map", "kd_flow_loss_grad", "kd_flow_collect", "kd_flow_train",
+           "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch"]
 
 _lib = None
 
@@ -84,6 +86,19 @@ def lib():
         L.kd_sample_device.argtypes = [vp, C.c_uint64, vp, vp]
         L.kd_last_timing.argtypes = [vp, dp, dp, dp, ip]
         L.kd_flops_per_point.argtypes = [vp, dp]
+        fp = C.POINTER(C.c_float)
+        L.kd_flow_n_params.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int64)]
+        L.kd_flow_init.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint64]
+        L.kd_flow_set_params.argtypes = [vp, C.c_int32, C.c_int32, fp, C.c_int64]
+        L.kd_flow_get_params.argtypes = [vp, fp, C.c_int64]
+        L.kd_flow_map.argtypes = [vp, dp, C.c_int64, dp, dp]
+        L.kd_flow_loss_grad.argtypes = [vp, dp, dp, C.c_int64, C.c_double, fp, dp]
+        L.kd_flow_collect.argtypes = [vp, C.c_int64, dp]
+        L.kd_flow_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_float, dp]
+        L.kd_autotune.argtypes = [vp, C.c_int64, ip, ip, dp]
+        L.kd_set_launch_shape.argtypes = [vp, C.c_int32, C.c_int32]
+        L.kd_get_launch_shape.argtypes = [vp, ip, ip]
+        L.kd_run_batch.argtypes = [C.POINTER(vp), C.POINTER(C.c_uint64), C.c_int32, C.POINTER(Accum), dp]
         _lib = L
     return _lib
 
@@ -205,6 +220,84 @@ class Context:
         f = C.c_double()
         self._check(lib().kd_flops_per_point(self.h, C.byref(f)))
         return f.value
+
+    # -- launch shape ----------------------------------------------------
+    def autotune(self, sample_points):
+        """-> (lanes per point, waves per CU, LU ms per 2^20 points)"""
+        l, w, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._check(lib().kd_autotune(self.h, sample_points, C.byref(l), C.byref(w), C.byref(ms)))
+        return l.value, w.value, ms.value
+
+    def set_launch_shape(self, lanes, waves_per_cu):
+        self._check(lib().kd_set_launch_shape(self.h, lanes, waves_per_cu))
+
+    def launch_shape(self):
+        l, w = C.c_int32(), C.c_int32()
+        self._check(lib().kd_get_launch_shape(self.h, C.byref(l), C.byref(w)))
+        return l.value, w.value
+
+    # -- learned flow warper ---------------------------------------------
+    def flow_init(self, order, n_layers, seed=0):
+        self._check(lib().kd_flow_init(self.h, order, n_layers, seed))
+        self.flow_shape = (order, n_layers)
+
+    def flow_set_params(self, order, n_layers, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(lib().kd_flow_set_params(self.h, order, n_layers, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                             p.size))
+        self.flow_shape = (order, n_layers)
+
+    def flow_get_params(self):
+        p = np.empty(flow_n_params(*self.flow_shape), dtype=np.float32)
+        self._check(lib().kd_flow_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size))
+        return p
+
+    def flow_map(self, l):
+        l = np.ascontiguousarray(l, dtype=np.float64)
+        x = np.empty_like(l)
+        jac = np.empty(l.shape[0])
+        self._check(lib().kd_flow_map(self.h, _dp(l), l.shape[0], _dp(x), _dp(jac)))
+        return x, jac
+
+    def flow_loss_grad(self, x, w, w_mean=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        g = np.empty(flow_n_params(*self.flow_shape), dtype=np.float32)
+        loss = C.c_double()
+        self._check(lib().kd_flow_loss_grad(self.h, _dp(x), _dp(w), x.shape[0],
+                                            float(w.mean() if w_mean is None else w_mean),
+                                            g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
+        return loss.value, g
+
+    def flow_collect(self, n_samples):
+        wm = C.c_double()
+        self._check(lib().kd_flow_collect(self.h, n_samples, C.byref(wm)))
+        return wm.value
+
+    def flow_train(self, steps, batch, lr, lr_final):
+        losses = np.empty(max(steps, 1))
+        self._check(lib().kd_flow_train(self.h, steps, batch, lr, lr_final, _dp(losses)))
+        return losses[:steps]
+
+
+def run_batch(ctxs, n_steps):
+    """One hipGraph launch over several contexts -> (list of Accum, ms)."""
+    n = len(ctxs)
+    hs = (C.c_void_p * n)(*[c.h for c in ctxs])
+    ns = (C.c_uint64 * n)(*[int(x) for x in n_steps])
+    acc = (Accum * n)()
+    ms = C.c_double()
+    rc = lib().kd_run_batch(hs, ns, n, acc, C.byref(ms))
+    if rc != KD_OK:
+        raise KdError(lib().kd_last_error(ctxs[0].h).decode() or f"keldy_amd error {rc}")
+    return list(acc), ms.value
+
+
+def flow_n_params(order, n_layers):
+    n = C.c_int64()
+    if lib().kd_flow_n_params(order, n_layers, C.byref(n)) != KD_OK:
+        raise KdError("bad flow shape")
+    return n.value
 
 
 def device_count():

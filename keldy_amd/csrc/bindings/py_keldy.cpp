@@ -87,7 +87,7 @@ template <typename W> void bind_product_1d(py::module_ &m, const char *name) {
 }
 
 template <typename T> void bind_integrator(py::module_ &m, const char *name) {
-  py::class_<T> c(m, name);
+  py::class_<T, integrator_direct> c(m, name);
   c.def(py::init<g0_model, double, int, double>(), py::arg("model"), py::arg("time"), py::arg("order"),
         py::arg("cutoff_integrand"))
       .def(py::init([](py::dict params, double time, int order, double cutoff) {
@@ -135,12 +135,34 @@ template <typename T> void bind_integrator(py::module_ &m, const char *name) {
               s.set_precision(KD_PREC_FP64);
             else if (p == "fp32")
               s.set_precision(KD_PREC_FP32_LU);
+            else if (p == "fp32_all")
+              s.set_precision(KD_PREC_FP32_LU_ALL);
             else
-              throw keldy_error("precision must be 'fp64' or 'fp32'");
+              throw keldy_error("precision must be 'fp64', 'fp32' (FP64 LU kept from order 10) or 'fp32_all'");
           },
           py::arg("precision"))
       .def("last_timing", &T::last_timing)
-      .def("flops_per_point", &T::flops_per_point);
+      .def("flops_per_point", &T::flops_per_point)
+      .def("autotune", &T::autotune, py::arg("sample_points") = 1 << 18, py::call_guard<py::gil_scoped_release>(),
+           "tune the Keldysh-sum kernel's launch shape: [lanes per point, waves per CU, LU ms per 2^20 points]")
+      .def(
+          "train_flow_warper",
+          [](T &s, int n_layers, long nr_samples, int steps, int batch, double lr, double lr_final, int seed_shift,
+             uint64_t init_seed) {
+            std::pair<warper_flow_t, std::vector<double>> r;
+            {
+              py::gil_scoped_release rel;
+              r = s.train_flow_warper(n_layers, nr_samples, steps, batch, lr, lr_final, seed_shift, init_seed);
+            }
+            py::array_t<double> losses(py::ssize_t(r.second.size()));
+            std::copy(r.second.begin(), r.second.end(), losses.mutable_data());
+            return py::make_tuple(r.first, losses);
+          },
+          py::arg("n_layers") = 6, py::arg("nr_samples") = 1 << 20, py::arg("steps") = 2000,
+          py::arg("batch") = 1 << 14, py::arg("lr") = 1e-3, py::arg("lr_final") = 1e-5,
+          py::arg("rng_seed_shift") = 1000003, py::arg("init_seed") = 0,
+          "Train a learned coupling-flow warper on |warped integrand| samples of the current chain "
+          "(returns (WarperFlowT, per-step loss)); append it last to the warper train.");
 }
 
 } // namespace
@@ -227,8 +249,24 @@ PYBIND11_MODULE(_keldy_host, mod) {
     bind_map_methods<warper_product_1d_interp_nearest_t>(c);
   }
   {
+    py::class_<warper_flow_t> c(wm, "WarperFlowT");
+    c.def(py::init([](int order, int n_layers, py::array_t<float, py::array::c_style | py::array::forcecast> p) {
+            return warper_flow_t(order, n_layers, std::vector<float>(p.data(), p.data() + p.size()));
+          }),
+          py::arg("order"), py::arg("n_layers"), py::arg("params"))
+        .def("order", &warper_flow_t::order)
+        .def("n_layers", &warper_flow_t::n_layers)
+        .def("params", [](warper_flow_t const &w) {
+          py::array_t<float> a(py::ssize_t(w.params().size()));
+          std::copy(w.params().begin(), w.params().end(), a.mutable_data());
+          return a;
+        });
+    bind_map_methods<warper_flow_t>(c);
+  }
+  {
     py::class_<warper_train_t> c(wm, "WarperTrainT");
     c.def(py::init<>())
+        .def("emplace_back", &warper_train_t::emplace_back<warper_flow_t>, py::arg("w"))
         .def("emplace_back", &warper_train_t::emplace_back<warper_identity_t>, py::arg("w"))
         .def("emplace_back", &warper_train_t::emplace_back<warper_plasma_uv_t>, py::arg("w"))
         .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_simple_t>, py::arg("w"))
@@ -383,6 +421,17 @@ PYBIND11_MODULE(_keldy_host, mod) {
             return py::make_tuple(out, dom);
           },
           py::arg("u"), py::arg("keep_u_hypercube") = true);
+  py::class_<integrator_direct>(im, "IntegratorDirect", "common base of the compute_* integrators");
   bind_integrator<compute_charge_Q_direct>(im, "ComputeChargeQDirect");
   bind_integrator<compute_current_J_direct>(im, "ComputeCurrentJDirect");
+  im.def(
+      "run_batch",
+      [](py::list integrators, std::vector<long> nr_steps) {
+        std::vector<integrator_direct *> its;
+        for (auto h : integrators) its.push_back(h.cast<integrator_direct *>());
+        py::gil_scoped_release rel;
+        return run_batch(its, nr_steps);
+      },
+      py::arg("integrators"), py::arg("nr_steps"),
+      "run several integrators (e.g. orders 1..N) in one hipGraph launch; returns its wall time in ms");
 }

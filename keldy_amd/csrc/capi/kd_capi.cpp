@@ -38,9 +38,22 @@ struct kd_ctx {
   double *d_tables = nullptr;
   int n_warpers = 0;
   std::vector<kd_warper_desc> h_warpers;
-  // flow parameters (KD_WARP_FLOW)
-  float *d_flow = nullptr;
-  int flow_layers = 0, flow_hidden = 0;
+  // learned flow warper (KD_WARP_FLOW): FP32 parameters, Adam moments,
+  // gradient, bf16 operand copies
+  int flow_order = -1, flow_layers = 0, flow_step = 0;
+  float *d_fw = nullptr, *d_fm = nullptr, *d_fv = nullptr, *d_fgrad = nullptr;
+  uint16_t *d_fpk = nullptr;
+  bool flow_in_chain = false, flow_suppressed = false;
+  double *d_fx = nullptr; // flow-stage output per chunk: x [kChunk][order], jac [kChunk]
+  size_t fx_cap = 0;
+  // flow training set and scratch
+  double *d_tx = nullptr, *d_tw = nullptr;
+  int64_t n_train = 0;
+  double w_mean = 0.0;
+  float *d_fpart = nullptr;
+  size_t fpart_cap = 0;
+  double *d_floss = nullptr; // [loss_part blocks | per-step losses]
+  size_t floss_cap = 0;
   // Sobol
   uint32_t *d_Csr = nullptr, *d_shfr = nullptr;
   int sobol_dim = 0;
@@ -59,6 +72,11 @@ struct kd_ctx {
   size_t prep_cap = 0;
   float last_ms = 0.f, last_lu_ms = 0.f, last_prep_ms = 0.f;
   int last_launches = 0, last_chunks = 0;
+  // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
+  // point and waves per CU of the persistent grid
+  int lu_lanes_max = 64, lu_waves_per_cu = 16;
+  float tune_ms_per_mpt = 0.f;
+  bool no_reduce = false; // autotuning launches leave the accumulator alone
 };
 
 namespace {
@@ -88,9 +106,13 @@ int ensure_stage(kd_ctx *c, size_t bytes) {
   return KD_OK;
 }
 
+// LU-kernel grid: one workgroup per group of points (64 / lanes points per
+// group), capped at lu_waves_per_cu waves per CU -- the kernel is persistent
+// over the rest.
 int grid_for(const kd_ctx *c, int64_t n_points) {
-  int64_t nb = (n_points + 63) / 64;
-  int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * 16;
+  const int64_t per_batch = kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
+  int64_t nb = (n_points + 63) / 64 * per_batch;
+  int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * c->lu_waves_per_cu;
   if (nb > cap) nb = cap;
   return (int)(nb < 1 ? 1 : nb);
 }
@@ -124,9 +146,6 @@ kd::KdParams base_params(const kd_ctx *c) {
   p.Csr = c->d_Csr;
   p.shfr = c->d_shfr;
   p.k0 = c->k;
-  p.flow = c->d_flow;
-  p.flow_layers = c->flow_layers;
-  p.flow_hidden = c->flow_hidden;
   return p;
 }
 
@@ -169,18 +188,57 @@ int ensure_events(kd_ctx *c, size_t n) {
   return KD_OK;
 }
 
-// Launch prep + integrand kernels over p.n_points points in chunks (and, in
-// MODE_RUN, the deterministic partial reduction into d_acc).  Each kernel is
-// bracketed by HIP events on the ctx stream (kd_last_timing).
-int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) {
+int ensure_flow_stage(kd_ctx *c, int order) {
+  const size_t bytes = sizeof(double) * (size_t)kChunk * (size_t)(order + 1);
+  if (bytes <= c->fx_cap) return KD_OK;
+  dfree(c->d_fx);
+  c->fx_cap = 0;
+  KD_HIP(c, hipMalloc(&c->d_fx, bytes));
+  c->fx_cap = bytes;
+  return KD_OK;
+}
+
+kd::FlowParams flow_params(const kd_ctx *c) {
+  return kd::FlowParams{c->d_fw, c->d_fpk, c->flow_layers, kd::flow_s_scale(c->flow_layers)};
+}
+
+int flow_grid(const kd_ctx *c, int64_t n) {
+  const int64_t nb = (n + 63) / 64, cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * 16;
+  return (int)std::max<int64_t>(1, std::min(nb, cap));
+}
+
+int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed);
+int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) { return launch_chunks(c, p0, true); }
+
+// Allocate every buffer a launch of p0 needs (before a graph capture).
+int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
+  kd::PrepBuf buf;
+  int rc = ensure_prep(c, c->order, &buf);
+  if (rc) return rc;
+  if (c->flow_in_chain && p0.mode != kd::MODE_EVAL_U && c->order > 0 && (rc = ensure_flow_stage(c, c->order)))
+    return rc;
+  return ensure_partials(c, grid_for(c, std::min<int64_t>(kChunk, p0.n_points)));
+}
+
+// Prep + integrand kernels over p0.n_points points in chunks (and, in
+// MODE_RUN, the deterministic partial reduction into d_acc); with `timed`
+// each kernel is bracketed by HIP events on the ctx stream (kd_last_timing).
+int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
   kd::PrepBuf buf;
   int rc = ensure_prep(c, c->order, &buf);
   if (rc) return rc;
   const int N = c->order;
+  // The flow warper, when last in the chain, is applied first: its own stage
+  // maps l -> x (and the Jacobian) before the rest of the chain.
+  const bool flow = c->flow_in_chain && !c->flow_suppressed && p0.mode != kd::MODE_EVAL_U && N > 0;
+  if (flow) {
+    if (c->flow_order != N) return fail(c, KD_ERR_STATE, "flow warper dimension differs from the integrand order");
+    if ((rc = ensure_flow_stage(c, N))) return rc;
+  }
   const int64_t nchunks = (p0.n_points + kChunk - 1) / kChunk;
-  if ((rc = ensure_events(c, size_t(4 * nchunks)))) return rc;
+  if (timed && (rc = ensure_events(c, size_t(4 * nchunks)))) return rc;
   int launches = 0;
-  KD_HIP(c, hipEventRecord(c->ev0, c->stream));
+  if (timed) KD_HIP(c, hipEventRecord(c->ev0, c->stream));
   int64_t ci = 0;
   for (int64_t off = 0; off < p0.n_points; off += kChunk, ci++) {
     kd::KdParams p = p0;
@@ -199,21 +257,40 @@ int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) {
       if ((rc = ensure_partials(c, grid))) return rc;
       p.partials = c->d_part;
     }
-    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], c->stream));
+    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], c->stream));
+    if (flow) {
+      kd::FlowMapIO io{};
+      io.Csr = p.Csr;
+      io.shfr = p.shfr;
+      io.k0 = p.k0;
+      io.in_l = p.mode == kd::MODE_EVAL_L ? p.in_pts : nullptr;
+      io.n_points = m;
+      io.out_x = c->d_fx;
+      io.out_jac = c->d_fx + (size_t)kChunk * N;
+      io.out_l = p.mode == kd::MODE_SAMPLE ? p.out_l : nullptr;
+      KD_HIP(c, kd::launch_flow_map(N, flow_params(c), io, flow_grid(c, m), c->stream));
+      launches += 1;
+      p.in_pts = io.out_x;
+      p.in_jac = io.out_jac;
+      p.out_l = nullptr;
+      p.n_warpers = c->n_warpers - 1;
+    } else if (c->flow_in_chain) {
+      p.n_warpers = c->n_warpers - 1; // flow suppressed: the chain without it
+    }
     KD_HIP(c, kd::launch_prep(N, p, buf, grid_prep, c->stream));
-    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
-    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
-    KD_HIP(c, kd::launch_lu(N, c->precision, p, buf, grid, c->stream));
-    KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
+    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
+    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
+    KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, p, buf, grid, c->stream));
+    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
     launches += 2;
-    if (p.mode == kd::MODE_RUN) {
+    if (p.mode == kd::MODE_RUN && !c->no_reduce) {
       KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->d_acc, c->stream));
       launches += 1;
     }
   }
-  KD_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (timed) KD_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->last_launches = launches;
-  c->last_chunks = (int)nchunks;
+  c->last_chunks = timed ? (int)nchunks : 0;
   return KD_OK;
 }
 
@@ -278,7 +355,16 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_warpers);
   dfree(c->d_maps);
   dfree(c->d_tables);
-  dfree(c->d_flow);
+  dfree(c->d_fw);
+  dfree(c->d_fm);
+  dfree(c->d_fv);
+  dfree(c->d_fgrad);
+  dfree(c->d_fpk);
+  dfree(c->d_fx);
+  dfree(c->d_tx);
+  dfree(c->d_tw);
+  dfree(c->d_fpart);
+  dfree(c->d_floss);
   dfree(c->d_Csr);
   dfree(c->d_shfr);
   dfree(c->d_acc);
@@ -343,7 +429,7 @@ int kd_set_integrand(kd_ctx *c, int32_t order, const kd_gf_index *A, const kd_gf
 
 int kd_set_precision(kd_ctx *c, int32_t precision) {
   if (!c) return KD_ERR_ARG;
-  if (precision != KD_PREC_FP64 && precision != KD_PREC_FP32_LU)
+  if (precision != KD_PREC_FP64 && precision != KD_PREC_FP32_LU && precision != KD_PREC_FP32_LU_ALL)
     return fail(c, KD_ERR_ARG, "unknown precision");
   c->precision = precision;
   return KD_OK;
@@ -401,7 +487,9 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
       if (d.n_maps < 1 || d.map_offset < 0 || d.map_offset + d.n_maps > n_maps)
         return fail(c, KD_ERR_ARG, "product warper map range out of bounds");
     } else if (d.kind == KD_WARP_FLOW) {
-      if (!c->d_flow) return fail(c, KD_ERR_STATE, "flow warper used before its parameters were set");
+      if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow warper used before its parameters were set");
+      if (i != n_w - 1)
+        return fail(c, KD_ERR_ARG, "the flow warper must be the last warper of the chain (it is applied first)");
     } else if (d.kind != KD_WARP_IDENTITY && d.kind != KD_WARP_PLASMA) {
       return fail(c, KD_ERR_ARG, "unknown warper kind " + std::to_string(d.kind));
     }
@@ -437,6 +525,7 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
   }
   c->n_warpers = n_w;
   c->h_warpers.assign(w, w + n_w);
+  c->flow_in_chain = n_w > 0 && w[n_w - 1].kind == KD_WARP_FLOW;
   return KD_OK;
 }
 
@@ -580,6 +669,425 @@ int kd_flops_per_point(const kd_ctx *c, double *flops) {
   const double muls = N * (N + 1) / 2.0 + N;
   const double per_cfg = 8 * macs + 6 * muls + 7 * N + 14;
   *flops = std::ldexp(per_cfg, (int)N);
+  return KD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned flow warper
+
+static uint64_t splitmix64(uint64_t &x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int kd_flow_n_params(int32_t order, int32_t n_layers, int64_t *n) {
+  if (!n || order < 1 || order > KD_MAX_ORDER || n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS) return KD_ERR_ARG;
+  *n = kd::flow_n_params(order, n_layers);
+  return KD_OK;
+}
+
+static int flow_alloc(kd_ctx *c, int32_t order, int32_t n_layers) {
+  if (order < 1 || order > KD_MAX_ORDER) return fail(c, KD_ERR_ARG, "flow dimension must be in [1, 12]");
+  if (n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS) return fail(c, KD_ERR_ARG, "flow layers must be in [1, 16]");
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->flow_order != order || c->flow_layers != n_layers || !c->d_fw) {
+    dfree(c->d_fw);
+    dfree(c->d_fm);
+    dfree(c->d_fv);
+    dfree(c->d_fgrad);
+    dfree(c->d_fpk);
+    c->flow_order = -1;
+    const size_t np = (size_t)kd::flow_n_params(order, n_layers);
+    KD_HIP(c, hipMalloc(&c->d_fw, np * sizeof(float)));
+    KD_HIP(c, hipMalloc(&c->d_fm, np * sizeof(float)));
+    KD_HIP(c, hipMalloc(&c->d_fv, np * sizeof(float)));
+    KD_HIP(c, hipMalloc(&c->d_fgrad, np * sizeof(float)));
+    KD_HIP(c, hipMalloc(&c->d_fpk, sizeof(uint16_t) * (size_t)kd::kFlowPackPerLayer * n_layers));
+    c->flow_order = order;
+    c->flow_layers = n_layers;
+  }
+  return KD_OK;
+}
+
+// upload params, reset Adam, rebuild the bf16 copies
+static int flow_upload(kd_ctx *c, const float *params) {
+  const size_t np = (size_t)kd::flow_n_params(c->flow_order, c->flow_layers);
+  KD_HIP(c, hipMemcpy(c->d_fw, params, np * sizeof(float), hipMemcpyHostToDevice));
+  KD_HIP(c, hipMemsetAsync(c->d_fm, 0, np * sizeof(float), c->stream));
+  KD_HIP(c, hipMemsetAsync(c->d_fv, 0, np * sizeof(float), c->stream));
+  KD_HIP(c, kd::launch_flow_pack(c->d_fw, c->d_fpk, c->flow_order, c->flow_layers, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  c->flow_step = 0;
+  return KD_OK;
+}
+
+int kd_flow_init(kd_ctx *c, int32_t order, int32_t n_layers, uint64_t seed) {
+  if (!c) return KD_ERR_ARG;
+  int rc = flow_alloc(c, order, n_layers);
+  if (rc) return rc;
+  // He-uniform W1, W2 (fan-in nA and H); zero biases; W3 = b3 = 0: the identity map
+  std::vector<float> w((size_t)kd::flow_n_params(order, n_layers), 0.f);
+  uint64_t st = seed;
+  auto unif = [&]() { return (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0; };
+  for (int k = 0; k < n_layers; k++) {
+    const int nA = kd::flow_nA(order, k), H = KD_FLOW_HIDDEN;
+    float *W1 = w.data() + kd::flow_layer_offset(order, k);
+    float *W2 = W1 + H * nA + H;
+    const double s1 = std::sqrt(6.0 / std::max(nA, 1)), s2 = std::sqrt(6.0 / H);
+    for (int i = 0; i < H * nA; i++) W1[i] = (float)(unif() * s1);
+    for (int i = 0; i < H * H; i++) W2[i] = (float)(unif() * s2);
+  }
+  return flow_upload(c, w.data());
+}
+
+int kd_flow_set_params(kd_ctx *c, int32_t order, int32_t n_layers, const float *params, int64_t n) {
+  if (!c) return KD_ERR_ARG;
+  if (!params) return fail(c, KD_ERR_ARG, "null buffer");
+  if (order < 1 || order > KD_MAX_ORDER || n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS ||
+      n != kd::flow_n_params(order, n_layers))
+    return fail(c, KD_ERR_ARG, "flow parameter count does not match (order, layers)");
+  for (int64_t i = 0; i < n; i++)
+    if (!std::isfinite(params[i])) return fail(c, KD_ERR_ARG, "non-finite flow parameter");
+  int rc = flow_alloc(c, order, n_layers);
+  if (rc) return rc;
+  return flow_upload(c, params);
+}
+
+int kd_flow_get_params(kd_ctx *c, float *params, int64_t n) {
+  if (!c) return KD_ERR_ARG;
+  if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow not initialised (kd_flow_init)");
+  if (!params || n != kd::flow_n_params(c->flow_order, c->flow_layers))
+    return fail(c, KD_ERR_ARG, "flow parameter buffer size does not match");
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  KD_HIP(c, hipMemcpy(params, c->d_fw, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return KD_OK;
+}
+
+int kd_flow_map(kd_ctx *c, const double *l, int64_t n, double *x, double *jac) {
+  if (!c) return KD_ERR_ARG;
+  if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow not initialised (kd_flow_init)");
+  if (n < 0) return fail(c, KD_ERR_ARG, "negative point count");
+  if (n == 0) return KD_OK;
+  if (!l || !x || !jac) return fail(c, KD_ERR_ARG, "null buffer");
+  const int N = c->flow_order;
+  const size_t nl = (size_t)n * N;
+  KD_HIP(c, hipSetDevice(c->device));
+  int rc = ensure_stage(c, sizeof(double) * (2 * nl + (size_t)n) + 64);
+  if (rc) return rc;
+  double *dl = (double *)c->d_stage, *dx = dl + nl, *dj = dx + nl;
+  KD_HIP(c, hipMemcpyAsync(dl, l, sizeof(double) * nl, hipMemcpyHostToDevice, c->stream));
+  kd::FlowMapIO io{};
+  io.in_l = dl;
+  io.n_points = n;
+  io.out_x = dx;
+  io.out_jac = dj;
+  KD_HIP(c, kd::launch_flow_map(N, flow_params(c), io, flow_grid(c, n), c->stream));
+  KD_HIP(c, hipMemcpyAsync(x, dx, sizeof(double) * nl, hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipMemcpyAsync(jac, dj, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  return KD_OK;
+}
+
+static int flow_scratch(kd_ctx *c, int blocks, int extra_losses) {
+  const size_t np = (size_t)kd::flow_n_params(c->flow_order, c->flow_layers);
+  const size_t pb = np * (size_t)blocks * sizeof(float);
+  if (pb > c->fpart_cap) {
+    dfree(c->d_fpart);
+    c->fpart_cap = 0;
+    KD_HIP(c, hipMalloc(&c->d_fpart, pb));
+    c->fpart_cap = pb;
+  }
+  const size_t lb = sizeof(double) * ((size_t)blocks + (size_t)extra_losses + 1);
+  if (lb > c->floss_cap) {
+    dfree(c->d_floss);
+    c->floss_cap = 0;
+    KD_HIP(c, hipMalloc(&c->d_floss, lb));
+    c->floss_cap = lb;
+  }
+  return KD_OK;
+}
+
+// gradient of the weighted loss over points [offset, offset + 64 blocks) of
+// (x, w) into d_fgrad; the loss into d_floss[blocks + loss_slot]
+static int flow_grad_launch(kd_ctx *c, const double *dx, const double *dw, int64_t n, int64_t offset, int blocks,
+                            double inv_norm, int loss_slot) {
+  kd::FlowTrainIO io{};
+  io.x = dx;
+  io.w = dw;
+  io.n_points = n;
+  io.offset = offset;
+  io.inv_norm = inv_norm;
+  io.part = c->d_fpart;
+  io.loss_part = c->d_floss;
+  io.n_params = kd::flow_n_params(c->flow_order, c->flow_layers);
+  KD_HIP(c, kd::launch_flow_train(c->flow_order, flow_params(c), io, blocks, c->stream));
+  KD_HIP(c, kd::launch_flow_grad_reduce(c->d_fpart, blocks, io.n_params, c->d_fgrad, c->d_floss,
+                                        c->d_floss + blocks + loss_slot, c->stream));
+  return KD_OK;
+}
+
+int kd_flow_loss_grad(kd_ctx *c, const double *x, const double *w, int64_t n, double w_mean, float *grad,
+                      double *loss) {
+  if (!c) return KD_ERR_ARG;
+  if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow not initialised (kd_flow_init)");
+  if (n <= 0 || n > (int64_t(1) << 18)) return fail(c, KD_ERR_ARG, "point count must be in [1, 2^18]");
+  if (!x || !w || !grad || !loss) return fail(c, KD_ERR_ARG, "null buffer");
+  if (!(w_mean > 0.0)) return fail(c, KD_ERR_ARG, "w_mean must be > 0");
+  const int N = c->flow_order;
+  const size_t nx = (size_t)n * N;
+  KD_HIP(c, hipSetDevice(c->device));
+  int rc = ensure_stage(c, sizeof(double) * (nx + (size_t)n) + 64);
+  if (rc) return rc;
+  const int blocks = (int)((n + 63) / 64);
+  if ((rc = flow_scratch(c, blocks, 0))) return rc;
+  double *dx = (double *)c->d_stage, *dw = dx + nx;
+  KD_HIP(c, hipMemcpyAsync(dx, x, sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
+  KD_HIP(c, hipMemcpyAsync(dw, w, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if ((rc = flow_grad_launch(c, dx, dw, n, 0, blocks, 1.0 / (w_mean * (double)n), 0))) return rc;
+  const size_t np = (size_t)kd::flow_n_params(N, c->flow_layers);
+  KD_HIP(c, hipMemcpyAsync(grad, c->d_fgrad, sizeof(float) * np, hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipMemcpyAsync(loss, c->d_floss + blocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  return KD_OK;
+}
+
+int kd_flow_collect(kd_ctx *c, int64_t n_samples, double *w_mean) {
+  if (!c) return KD_ERR_ARG;
+  int rc = need_integrand(c);
+  if (rc) return rc;
+  if (n_samples < 64 || n_samples > (int64_t(1) << 24))
+    return fail(c, KD_ERR_ARG, "training sample count must be in [64, 2^24]");
+  const int N = c->order;
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  dfree(c->d_tx);
+  dfree(c->d_tw);
+  c->n_train = 0;
+  KD_HIP(c, hipMalloc(&c->d_tx, sizeof(double) * (size_t)n_samples * N));
+  KD_HIP(c, hipMalloc(&c->d_tw, sizeof(double) * (size_t)n_samples));
+  double *dvals = nullptr;
+  KD_HIP(c, hipMalloc(&dvals, sizeof(double) * 2 * (size_t)n_samples));
+  // samples of the chain without the flow warper
+  c->flow_suppressed = true;
+  rc = run_common(c, (uint64_t)n_samples, kd::MODE_SAMPLE, c->d_tx, dvals);
+  c->flow_suppressed = false;
+  if (!rc) rc = finish_timing(c);
+  const int grid = 1024;
+  double *dpart = nullptr;
+  if (!rc && hipMalloc(&dpart, sizeof(double) * grid) != hipSuccess) rc = fail(c, KD_ERR_HIP, "hipMalloc failed");
+  std::vector<double> part(grid);
+  if (!rc && kd::launch_flow_abs(dvals, n_samples, c->d_tw, dpart, grid, c->stream) != hipSuccess)
+    rc = fail(c, KD_ERR_HIP, "flow_abs launch failed");
+  if (!rc && hipMemcpyAsync(part.data(), dpart, sizeof(double) * grid, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = fail(c, KD_ERR_HIP, "copy failed");
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, KD_ERR_HIP, "sync failed");
+  (void)hipFree(dvals);
+  if (dpart) (void)hipFree(dpart);
+  if (rc) return rc;
+  double s = 0.0;
+  for (double v : part) s += v;
+  if (!std::isfinite(s)) return fail(c, KD_ERR_ARG, "non-finite integrand samples (g0 window too small?)");
+  if (!(s > 0.0)) return fail(c, KD_ERR_ARG, "all training samples are zero");
+  c->n_train = n_samples;
+  c->w_mean = s / (double)n_samples;
+  if (w_mean) *w_mean = c->w_mean;
+  return KD_OK;
+}
+
+int kd_flow_train(kd_ctx *c, int32_t steps, int32_t batch, float lr, float lr_final, double *losses) {
+  if (!c) return KD_ERR_ARG;
+  if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow not initialised (kd_flow_init)");
+  if (c->n_train <= 0) return fail(c, KD_ERR_STATE, "no training samples (kd_flow_collect)");
+  if (c->flow_order != c->order) return fail(c, KD_ERR_STATE, "flow dimension differs from the integrand order");
+  if (steps < 0) return fail(c, KD_ERR_ARG, "negative step count");
+  if (batch < 64 || batch % 64 || batch > (1 << 16) || c->n_train % batch)
+    return fail(c, KD_ERR_ARG, "batch must be a multiple of 64 that divides the training sample count");
+  if (!(lr > 0.f) || !(lr_final > 0.f)) return fail(c, KD_ERR_ARG, "learning rates must be > 0");
+  if (steps == 0) return KD_OK;
+  KD_HIP(c, hipSetDevice(c->device));
+  const int blocks = batch / 64;
+  int rc = flow_scratch(c, blocks, steps);
+  if (rc) return rc;
+  const int np = kd::flow_n_params(c->flow_order, c->flow_layers);
+  const double inv_norm = 1.0 / (c->w_mean * (double)batch);
+  KD_HIP(c, hipEventRecord(c->ev0, c->stream));
+  for (int s = 0; s < steps; s++) {
+    const int64_t off = ((int64_t)s * batch) % c->n_train;
+    if ((rc = flow_grad_launch(c, c->d_tx, c->d_tw, c->n_train, off, blocks, inv_norm, s))) return rc;
+    // cosine decay from lr to lr_final
+    const float lrs = lr_final + 0.5f * (lr - lr_final) * (1.f + std::cos(3.14159265358979f * (float)s / (float)steps));
+    c->flow_step++;
+    KD_HIP(c, kd::launch_flow_adam(c->d_fw, c->d_fm, c->d_fv, c->d_fgrad, np, lrs, 0.9f, 0.999f, 1e-8f,
+                                   c->flow_step, c->stream));
+    KD_HIP(c, kd::launch_flow_pack(c->d_fw, c->d_fpk, c->flow_order, c->flow_layers, c->stream));
+  }
+  KD_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (losses)
+    KD_HIP(c, hipMemcpyAsync(losses, c->d_floss + blocks, sizeof(double) * steps, hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  KD_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  c->last_lu_ms = 0.f;
+  c->last_prep_ms = 0.f;
+  c->last_launches = 4 * steps;
+  c->last_chunks = 0;
+  return KD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch-shape autotuner and the batched multi-integrator run
+
+int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves_per_cu, double *ms_per_mpoint) {
+  if (!c) return KD_ERR_ARG;
+  int rc = need_integrand(c);
+  if (rc) return rc;
+  if (!c->has_sobol) return fail(c, KD_ERR_STATE, "rng not set (kd_set_sobol)");
+  if (c->sobol_dim != c->order) return fail(c, KD_ERR_ARG, "Sobol dimension differs from the integrand order");
+  if (sample_points < 64 || sample_points > kChunk) return fail(c, KD_ERR_ARG, "sample size must be in [64, 2^21]");
+  if ((uint64_t)c->k + (uint64_t)sample_points > (1ull << 31))
+    return fail(c, KD_ERR_RANGE, "More sample points requested then generator can hold.");
+  KD_HIP(c, hipSetDevice(c->device));
+  const int lane_opts[3] = {64, 32, 16}, wave_opts[3] = {8, 16, 32};
+  const int old_l = c->lu_lanes_max, old_w = c->lu_waves_per_cu;
+  int best_l = old_l, best_w = old_w;
+  float best = 1e30f;
+  kd::KdParams p = base_params(c);
+  p.mode = kd::MODE_RUN;
+  p.n_points = sample_points;
+  c->no_reduce = true;
+  for (int li = 0; li < 3 && !rc; li++) {
+    const int L = lane_opts[li];
+    if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
+    for (int wi = 0; wi < 3 && !rc; wi++) {
+      c->lu_lanes_max = L;
+      c->lu_waves_per_cu = wave_opts[wi];
+      float t = 1e30f;
+      for (int rep = 0; rep < 3 && !rc; rep++) { // first repetition warms up
+        if ((rc = launch_chunks(c, p, true)) || (rc = finish_timing(c))) break;
+        if (rep > 0) t = std::min(t, c->last_lu_ms);
+      }
+      if (!rc && t < best) {
+        best = t;
+        best_l = L;
+        best_w = wave_opts[wi];
+      }
+    }
+  }
+  c->no_reduce = false;
+  if (rc) {
+    c->lu_lanes_max = old_l;
+    c->lu_waves_per_cu = old_w;
+    return rc;
+  }
+  c->lu_lanes_max = best_l;
+  c->lu_waves_per_cu = best_w;
+  c->tune_ms_per_mpt = best * 1048576.f / (float)sample_points;
+  if (lanes) *lanes = kd::lu_lanes(c->order, best_l);
+  if (waves_per_cu) *waves_per_cu = best_w;
+  if (ms_per_mpoint) *ms_per_mpoint = c->tune_ms_per_mpt;
+  return KD_OK;
+}
+
+int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
+  if (!c) return KD_ERR_ARG;
+  if (lanes != 16 && lanes != 32 && lanes != 64) return fail(c, KD_ERR_ARG, "lanes must be 16, 32 or 64");
+  if (waves_per_cu < 1 || waves_per_cu > 64) return fail(c, KD_ERR_ARG, "waves_per_cu must be in [1, 64]");
+  c->lu_lanes_max = lanes;
+  c->lu_waves_per_cu = waves_per_cu;
+  return KD_OK;
+}
+
+int kd_get_launch_shape(const kd_ctx *c, int32_t *lanes, int32_t *waves_per_cu) {
+  if (!c) return KD_ERR_ARG;
+  if (lanes) *lanes = kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
+  if (waves_per_cu) *waves_per_cu = c->lu_waves_per_cu;
+  return KD_OK;
+}
+
+int kd_run_batch(kd_ctx *const *ctxs, const uint64_t *n_steps, int32_t count, kd_accum *out, double *ms) {
+  if (!ctxs || !n_steps || count < 1) return KD_ERR_ARG;
+  kd_ctx *c0 = ctxs[0];
+  if (!c0) return KD_ERR_ARG;
+  for (int i = 0; i < count; i++) {
+    kd_ctx *c = ctxs[i];
+    if (!c) return fail(c0, KD_ERR_ARG, "null context in batch");
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == c) return fail(c0, KD_ERR_ARG, "a context appears twice in the batch");
+    if (c->device != c0->device) return fail(c0, KD_ERR_ARG, "batched contexts must share one device");
+    int rc = need_integrand(c);
+    if (rc) return fail(c0, rc, c->err);
+    if (!c->has_sobol) return fail(c0, KD_ERR_STATE, "rng not set (kd_set_sobol)");
+    if (c->sobol_dim != c->order) return fail(c0, KD_ERR_ARG, "Sobol dimension differs from the integrand order");
+    if ((uint64_t)c->k + n_steps[i] > (1ull << 31))
+      return fail(c0, KD_ERR_RANGE, "More sample points requested then generator can hold.");
+  }
+  KD_HIP(c0, hipSetDevice(c0->device));
+  std::vector<kd::KdParams> ps(count);
+  for (int i = 0; i < count; i++) {
+    kd_ctx *c = ctxs[i];
+    ps[i] = base_params(c);
+    ps[i].mode = kd::MODE_RUN;
+    ps[i].n_points = (int64_t)n_steps[i];
+    if (n_steps[i] == 0) continue;
+    int rc = prepare_launch(c, ps[i]);
+    if (rc) return fail(c0, rc, c->err);
+    KD_HIP(c0, hipStreamSynchronize(c->stream));
+  }
+  // one graph: fork from ctx 0's stream to every ctx stream, join back
+  std::vector<hipEvent_t> evs(count + 1, nullptr);
+  for (auto &e : evs) KD_HIP(c0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int rc = KD_OK;
+  hipStream_t origin = c0->stream;
+  if (hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = fail(c0, KD_ERR_HIP, "hipStreamBeginCapture failed");
+  } else {
+    if (hipEventRecord(evs[count], origin) != hipSuccess) rc = fail(c0, KD_ERR_HIP, "hipEventRecord failed");
+    for (int i = 0; i < count && !rc; i++) {
+      kd_ctx *c = ctxs[i];
+      if (n_steps[i] == 0) continue;
+      if (i > 0 && hipStreamWaitEvent(c->stream, evs[count], 0) != hipSuccess) {
+        rc = fail(c0, KD_ERR_HIP, "hipStreamWaitEvent failed");
+        break;
+      }
+      if ((rc = launch_chunks(c, ps[i], false))) {
+        rc = fail(c0, rc, c->err);
+        break;
+      }
+      if (i > 0 && (hipEventRecord(evs[i], c->stream) != hipSuccess ||
+                    hipStreamWaitEvent(origin, evs[i], 0) != hipSuccess))
+        rc = fail(c0, KD_ERR_HIP, "join failed");
+    }
+    if (hipStreamEndCapture(origin, &graph) != hipSuccess && !rc) rc = fail(c0, KD_ERR_HIP, "hipStreamEndCapture failed");
+  }
+  if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = fail(c0, KD_ERR_HIP, "hipGraphInstantiate failed");
+  float t_ms = 0.f;
+  if (!rc) {
+    if (hipEventRecord(c0->ev0, origin) != hipSuccess || hipGraphLaunch(exec, origin) != hipSuccess ||
+        hipEventRecord(c0->ev1, origin) != hipSuccess || hipStreamSynchronize(origin) != hipSuccess ||
+        hipEventElapsedTime(&t_ms, c0->ev0, c0->ev1) != hipSuccess)
+      rc = fail(c0, KD_ERR_HIP, "graph launch failed");
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (auto &e : evs) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  for (int i = 0; i < count; i++) {
+    kd_ctx *c = ctxs[i];
+    c->k += (uint32_t)n_steps[i];
+    c->n_points += n_steps[i];
+    c->last_ms = t_ms;
+    c->last_lu_ms = c->last_prep_ms = 0.f;
+    if (out && (rc = kd_get_accum(c, &out[i]))) return fail(c0, rc, c->err);
+  }
+  if (ms) *ms = t_ms;
   return KD_OK;
 }
 

@@ -141,9 +141,38 @@ void integrator_direct::set_precision(int precision) {
 void integrator_direct::sync_warper() {
   if (warper.stamp == uploaded_stamp_) return;
   const warpers::device_chain c = warper.to_device();
+  if (c.flow_layers > 0 && c.flow_stamp != uploaded_flow_stamp_) {
+    ctx_.check(kd_flow_set_params(ctx_.get(), c.flow_order, c.flow_layers, c.flow_params.data(),
+                                  int64_t(c.flow_params.size())));
+    uploaded_flow_stamp_ = c.flow_stamp;
+  }
   ctx_.check(kd_set_warper_chain(ctx_.get(), c.warpers.data(), int32_t(c.warpers.size()), c.maps.data(),
                                  int32_t(c.maps.size()), c.tables.data(), int64_t(c.tables.size())));
   uploaded_stamp_ = warper.stamp;
+}
+
+std::pair<warpers::warper_flow_t, std::vector<double>>
+integrator_direct::train_flow_warper(int n_layers, long nr_samples, int steps, int batch, double lr, double lr_final,
+                                     int rng_seed_shift, uint64_t init_seed) {
+  sync_warper();
+  const sobol gen(dimension, 0, true, true, rng_seed_shift);
+  ctx_.check(kd_set_sobol(ctx_.get(), gen.dim(), gen.csr().data(), gen.shfr().data(), gen.state()));
+  rng_dirty_ = true; // the next run re-uploads the integrator's own generator
+  double w_mean = 0.0;
+  ctx_.check(kd_flow_collect(ctx_.get(), int64_t(nr_samples), &w_mean));
+  ctx_.check(kd_flow_init(ctx_.get(), dimension, n_layers, init_seed));
+  std::vector<double> losses(size_t(std::max(steps, 0)));
+  ctx_.check(kd_flow_train(ctx_.get(), steps, batch, float(lr), float(lr_final), losses.data()));
+  int64_t np = 0;
+  ctx_.check(kd_flow_n_params(dimension, n_layers, &np));
+  std::vector<float> p(static_cast<size_t>(np));
+  ctx_.check(kd_flow_get_params(ctx_.get(), p.data(), np));
+  warpers::warper_flow_t w(dimension, n_layers, std::move(p));
+  // the device now holds these parameters; any flow already in the chain must
+  // be uploaded again before it is used
+  uploaded_flow_stamp_ = w.stamp();
+  uploaded_stamp_ = 0;
+  return {std::move(w), std::move(losses)};
 }
 
 std::pair<dcomplex, double> integrator_direct::evaluate_warped_integrand(std::vector<double> const &li_vec,
@@ -169,19 +198,52 @@ void integrator_direct::evaluate_warped_integrand_batch(const double *l, long n,
   check_window(out, n);
 }
 
-void integrator_direct::run(long nr_steps) {
-  if (nr_steps <= 0) return;
+void integrator_direct::prepare_device_run() {
   sync_warper();
   if (rng_dirty_) {
     ctx_.check(kd_set_sobol(ctx_.get(), rng_->dim(), rng_->csr().data(), rng_->shfr().data(), rng_->state()));
     rng_dirty_ = false;
   }
-  kd_accum acc{};
-  ctx_.check(kd_run(ctx_.get(), uint64_t(nr_steps), &acc));
+}
+
+void integrator_direct::finish_device_run(long nr_steps) {
   n_points_ += uint64_t(nr_steps);
   uint32_t k = 0;
   ctx_.check(kd_get_sobol_state(ctx_.get(), &k));
   rng_->set_state(k);
+}
+
+void integrator_direct::run(long nr_steps) {
+  if (nr_steps <= 0) return;
+  prepare_device_run();
+  kd_accum acc{};
+  ctx_.check(kd_run(ctx_.get(), uint64_t(nr_steps), &acc));
+  finish_device_run(nr_steps);
+}
+
+std::vector<double> integrator_direct::autotune(long sample_points) {
+  prepare_device_run();
+  int32_t lanes = 0, wpc = 0;
+  double ms = 0;
+  ctx_.check(kd_autotune(ctx_.get(), int64_t(sample_points), &lanes, &wpc, &ms));
+  return {double(lanes), double(wpc), ms};
+}
+
+double run_batch(std::vector<integrator_direct *> const &its, std::vector<long> const &steps) {
+  if (its.size() != steps.size()) throw keldy_error("run_batch: one point count per integrator");
+  if (its.empty()) return 0.0;
+  std::vector<kd_ctx *> ctxs;
+  std::vector<uint64_t> ns;
+  for (size_t i = 0; i < its.size(); i++) {
+    if (steps[i] < 0) throw keldy_error("run_batch: negative point count");
+    its[i]->prepare_device_run();
+    ctxs.push_back(its[i]->ctx().get());
+    ns.push_back(uint64_t(steps[i]));
+  }
+  double ms = 0;
+  its[0]->ctx().check(kd_run_batch(ctxs.data(), ns.data(), int32_t(ctxs.size()), nullptr, &ms));
+  for (size_t i = 0; i < its.size(); i++) its[i]->finish_device_run(steps[i]);
+  return ms;
 }
 
 void integrator_direct::reset_rng(std::string rng_name, int rng_state_seed, bool do_shift, bool do_scramble,

@@ -98,6 +98,22 @@ class integrator_direct {
   device_ctx const &ctx() const { return ctx_; }
   /// upload the warper chain if it changed since the last upload
   void sync_warper();
+  /// Keldysh-sum kernel launch shape tuned on sample_points points of the
+  /// current chain (generator and accumulator untouched): (lanes per point,
+  /// waves per CU, LU ms per 2^20 points).
+  std::vector<double> autotune(long sample_points);
+  /// device state before / after a run launched elsewhere (run_batch)
+  void prepare_device_run();
+  void finish_device_run(long nr_steps);
+  /// Learned flow warper (no counterpart in keldy): collects nr_samples points
+  /// of the current chain (scrambled Sobol, seed shift rng_seed_shift, the
+  /// flow warper excluded), then trains an n_layers coupling flow on the
+  /// device for `steps` Adam steps of `batch` points, learning rate cosine-
+  /// decayed from lr to lr_final.  Returns the warper (append it last) and the
+  /// per-step loss.
+  std::pair<warpers::warper_flow_t, std::vector<double>> train_flow_warper(int n_layers, long nr_samples, int steps,
+                                                                          int batch, double lr, double lr_final,
+                                                                          int rng_seed_shift, uint64_t init_seed);
 
  protected:
   int dimension = 1;
@@ -108,7 +124,13 @@ class integrator_direct {
   uint64_t n_points_ = 0;
   int precision_ = KD_PREC_FP64;
   uint64_t uploaded_stamp_ = 0;
+  uint64_t uploaded_flow_stamp_ = 0;
 };
+
+/// Several integrators (e.g. orders 1..N with uneven point counts) run as one
+/// hipGraph launch; each advances and accumulates as by its own run().
+/// Returns the launch's wall time in ms.  No counterpart in keldy.
+double run_batch(std::vector<integrator_direct *> const &integrators, std::vector<long> const &nr_steps);
 
 /// Charge Q = G^<_{up,up}(t, t) (compute_obs.hpp:51-64).
 class compute_charge_Q_direct : public integrator_direct {

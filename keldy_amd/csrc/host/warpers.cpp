@@ -1,6 +1,8 @@
 // warpers.cpp -- host side of the warpers (reference: c++/keldy/warpers/*.hpp).
 #include "warpers.hpp"
 
+#include "../kernels/kd_flow_device.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <sstream>
@@ -634,6 +636,126 @@ device_chain warper_train_t::to_device() const {
   device_chain c;
   for (auto const &w : warpers) std::visit([&c](auto const &ww) { ww.to_device(c); }, w);
   return c;
+}
+
+} // namespace keldy::warpers
+
+// --------------------------------------------------------------------------
+// learned flow warper (host evaluation of the device network)
+namespace keldy::warpers {
+
+namespace {
+constexpr int kH = KD_FLOW_HIDDEN;
+constexpr double kFlowClampHost = 1.0 / 1099511627776.0; // 2^-40
+int fl_nA(int n, int k) { return kd::flow_nA(n, k); }
+float bf16_round(float f) { return kd::bf2f(kd::f2bf(f)); }
+double log_sig_prime(double y) {
+  const double ay = std::abs(y);
+  return -ay - 2.0 * std::log1p(std::exp(-ay));
+}
+} // namespace
+
+warper_flow_t::warper_flow_t(int order, int n_layers, std::vector<float> params)
+   : order_(order), layers_(n_layers), params_(std::move(params)), stamp_(next_stamp()) {
+  if (order < 1 || order > KD_MAX_ORDER) throw keldy_error("flow dimension must be in [1, 12]");
+  if (n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS) throw keldy_error("flow layers must be in [1, 16]");
+  if (int64_t(params_.size()) != kd::flow_n_params(order, n_layers))
+    throw keldy_error("flow parameter count does not match (order, layers)");
+}
+
+void warper_flow_t::net(int k, const double *y, double *s, double *t) const {
+  const int n = order_, nA = fl_nA(n, k), nB = n - nA;
+  const float *W1 = params_.data() + kd::flow_layer_offset(n, k);
+  const float *b1 = W1 + kH * nA, *W2 = b1 + kH, *b2 = W2 + kH * kH, *W3 = b2 + kH, *b3 = W3 + 2 * nB * kH;
+  float a[KD_MAX_ORDER];
+  int na = 0;
+  for (int i = 0; i < n; i++)
+    if (((i + k) & 1) == 0) a[na++] = float(y[i]);
+  float h1[kH], h2[kH];
+  for (int j = 0; j < kH; j++) {
+    float z = b1[j];
+    for (int i = 0; i < nA; i++) z = std::fmaf(W1[j * nA + i], a[i], z);
+    h1[j] = bf16_round(std::max(z, 0.f));
+  }
+  for (int j = 0; j < kH; j++) {
+    double z = 0.0;
+    for (int i = 0; i < kH; i++) z += double(bf16_round(W2[j * kH + i])) * double(h1[i]);
+    h2[j] = bf16_round(std::max(float(z) + b2[j], 0.f));
+  }
+  for (int q = 0; q < 2 * nB; q++) {
+    double z = 0.0;
+    for (int i = 0; i < kH; i++) z += double(bf16_round(W3[q * kH + i])) * double(h2[i]);
+    const float o = float(z) + b3[q];
+    if (q < nB)
+      s[q] = double(kd::flow_s_scale(layers_) * std::tanh(o));
+    else
+      t[q - nB] = double(o);
+  }
+}
+
+map_pair warper_flow_t::map_reverse(vec const &l) const {
+  if (int(l.size()) != order_) throw keldy_error("flow warper: point dimension differs from the flow's");
+  double y[KD_MAX_ORDER], s[KD_MAX_ORDER], t[KD_MAX_ORDER];
+  double ldj = 0.0;
+  for (int i = 0; i < order_; i++) {
+    const double li = std::min(std::max(l[i], kFlowClampHost), 1.0 - kFlowClampHost);
+    y[i] = std::log(li) - std::log1p(-li);
+    ldj -= std::log(li) + std::log1p(-li);
+  }
+  for (int k = 0; k < layers_; k++) {
+    net(k, y, s, t);
+    int nb = 0;
+    for (int i = 0; i < order_; i++)
+      if (((i + k) & 1) == 1) {
+        y[i] = y[i] * std::exp(s[nb]) + t[nb];
+        ldj += s[nb];
+        nb++;
+      }
+  }
+  vec x(order_);
+  for (int i = 0; i < order_; i++) {
+    x[i] = 1.0 / (1.0 + std::exp(-y[i]));
+    ldj += log_sig_prime(y[i]);
+  }
+  return {x, std::exp(ldj)};
+}
+
+map_pair warper_flow_t::map_forward(vec const &xin) const {
+  if (int(xin.size()) != order_) throw keldy_error("flow warper: point dimension differs from the flow's");
+  double v[KD_MAX_ORDER], s[KD_MAX_ORDER], t[KD_MAX_ORDER];
+  double ldj = 0.0; // log dl/dx
+  for (int i = 0; i < order_; i++) {
+    const double xi = std::min(std::max(xin[i], kFlowClampHost), 1.0 - kFlowClampHost);
+    v[i] = std::log(xi) - std::log1p(-xi);
+    ldj -= std::log(xi) + std::log1p(-xi);
+  }
+  for (int k = layers_ - 1; k >= 0; k--) {
+    net(k, v, s, t);
+    int nb = 0;
+    for (int i = 0; i < order_; i++)
+      if (((i + k) & 1) == 1) {
+        v[i] = (v[i] - t[nb]) * std::exp(-s[nb]);
+        ldj -= s[nb];
+        nb++;
+      }
+  }
+  vec l(order_);
+  for (int i = 0; i < order_; i++) {
+    l[i] = 1.0 / (1.0 + std::exp(-v[i]));
+    ldj += log_sig_prime(v[i]);
+  }
+  return {l, std::exp(ldj)};
+}
+
+void warper_flow_t::to_device(device_chain &c) const {
+  if (params_.empty()) throw keldy_error("flow warper has no parameters");
+  kd_warper_desc d{};
+  d.kind = KD_WARP_FLOW;
+  c.warpers.push_back(d);
+  c.flow_order = order_;
+  c.flow_layers = layers_;
+  c.flow_params = params_;
+  c.flow_stamp = stamp_;
 }
 
 } // namespace keldy::warpers

@@ -25,6 +25,10 @@ struct device_chain {
   std::vector<kd_warper_desc> warpers;
   std::vector<kd_map1d> maps;
   std::vector<double> tables;
+  // learned flow warper (last in the chain), if any
+  int flow_order = 0, flow_layers = 0;
+  std::vector<float> flow_params;
+  uint64_t flow_stamp = 0;
 };
 
 /// One-dimensional inverse-CDF map u = F^{-1}(l) with density f = F'.
@@ -239,17 +243,48 @@ vec kernel_smoothing(vec const &y_in, double sigma);
 std::pair<double, double> fit_wlinear(const double *x, const double *w, const double *y, size_t n);
 } // namespace proj_details
 
-using warper_variant =
-    std::variant<warper_identity_t, warper_plasma_uv_t, warper_product_1d_simple_interp_nearest_t,
-                 warper_product_1d_simple_t, warper_product_1d_t, warper_product_1d_interp_nearest_t,
-                 warper_product_1d_interp_hybrid_t, warper_product_1d_simple_interp_hybrid_t>;
-
-/// Ordered chain of warpers; map_reverse applies them last-added first
-/// (warpers.hpp:41-137).
 inline uint64_t next_stamp() {
   static std::atomic<uint64_t> counter{1};
   return counter.fetch_add(1);
 }
+
+/// Learned affine-coupling flow on (0,1)^n: an importance map trained on
+/// |integrand| samples (no counterpart in keldy).  Map, Jacobian and parameter
+/// layout are those of the device kernels (kd_flow_device.hpp): y = logit(l),
+/// coupling layers y_B <- y_B exp(s(y_A)) + t(y_A), x = sigmoid(y).  The host
+/// methods evaluate the same network (bf16-rounded operands where the device
+/// uses bf16 MFMA, FP64 accumulation); runs use the device kernels.  Must be
+/// the last warper of a train (it is applied first to the Sobol point).
+class warper_flow_t {
+ public:
+  warper_flow_t() = default;
+  warper_flow_t(int order, int n_layers, std::vector<float> params);
+  map_pair map_reverse(vec const &l) const; // l -> x, dx/dl
+  map_pair map_forward(vec const &x) const; // x -> l, dl/dx
+  vec ui_from_li(vec const &l) const { return map_reverse(l).first; }
+  vec li_from_ui(vec const &u) const { return map_forward(u).first; }
+  double jacobian_reverse(vec const &l) const { return map_reverse(l).second; }
+  double jacobian_forward(vec const &u) const { return map_forward(u).second; }
+  void to_device(device_chain &c) const;
+  int order() const { return order_; }
+  int n_layers() const { return layers_; }
+  std::vector<float> const &params() const { return params_; }
+  uint64_t stamp() const { return stamp_; }
+
+ private:
+  void net(int k, const double *y, double *s, double *t) const;
+  int order_ = 0, layers_ = 0;
+  std::vector<float> params_;
+  uint64_t stamp_ = 0;
+};
+
+using warper_variant =
+    std::variant<warper_identity_t, warper_plasma_uv_t, warper_product_1d_simple_interp_nearest_t,
+                 warper_product_1d_simple_t, warper_product_1d_t, warper_product_1d_interp_nearest_t,
+                 warper_product_1d_interp_hybrid_t, warper_product_1d_simple_interp_hybrid_t, warper_flow_t>;
+
+/// Ordered chain of warpers; map_reverse applies them last-added first
+/// (warpers.hpp:41-137).
 
 class warper_train_t {
  public:

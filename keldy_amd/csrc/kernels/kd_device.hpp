@@ -69,9 +69,8 @@ struct KdParams {
   double *out_l; // MODE_SAMPLE
   // accumulation (MODE_RUN): per block {re, im, n_dom, n_fallback}
   double *partials;
-  // flow warper parameters (KD_WARP_FLOW)
-  const float *flow;
-  int flow_layers, flow_hidden;
+  // optional per-point initial Jacobian (output of the flow stage), or null
+  const double *in_jac;
 };
 
 // Compacted in-domain points between the two kernel stages (batch b of 64

@@ -156,8 +156,6 @@ __device__ __forceinline__ double apply_warpers(const KdParams &p, double (&x)[N
         jac *= map1d_reverse(m, p.tables, x[i], &xi);
         x[i] = xi;
       }
-    } else if (wd.kind == KD_WARP_FLOW) {
-      jac *= flow_reverse<N>(p, x);
     }
   }
   return jac;
@@ -172,7 +170,8 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
   constexpr int NU = N > 0 ? N : 1;
   __shared__ uint32_t sC[NU * 31];
   __shared__ uint32_t sS[NU];
-  const bool sob = p.mode == MODE_RUN || p.mode == MODE_SAMPLE;
+  // Sobol points unless the caller (or the flow stage) supplies the points
+  const bool sob = (p.mode == MODE_RUN || p.mode == MODE_SAMPLE) && p.in_pts == nullptr;
   if (sob) {
     for (int i = threadIdx.x; i < N * 31; i += blockDim.x) sC[i] = p.Csr[i];
     if (threadIdx.x < N) sS[threadIdx.x] = p.shfr[threadIdx.x];
@@ -190,6 +189,7 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
     double jac = 1.0;
     bool dom = valid && spin_ok;
     if (valid) {
+      if (p.in_jac) jac = p.in_jac[pi];
       if (sob) {
         const uint32_t k = p.k0 + 1u + (uint32_t)pi;
 #pragma unroll
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
         for (int i = 0; i < N; i++) x[i] = p.in_pts[pi * N + i];
       }
       if constexpr (N > 0) {
-        if (p.mode != MODE_EVAL_U) jac = apply_warpers<N>(p, x);
+        if (p.mode != MODE_EVAL_U) jac *= apply_warpers<N>(p, x);
       }
       if (p.keep_u) {
 #pragma unroll
@@ -230,12 +230,12 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
   }
 }
 
-// Stage 2: the Keldysh sum of the compacted points, one 64-lane workgroup per
-// batch (grid-stride).
-template <int N, typename T>
+// Stage 2: the Keldysh sum of the compacted points; 64-lane workgroups,
+// grid-stride over groups of points.
+template <int N, typename T, int GL>
 __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) {
   constexpr int NC = 1 << N;            // Keldysh configurations per point
-  constexpr int G = NC < 64 ? NC : 64;  // lanes per point
+  constexpr int G = NC < GL ? NC : GL;  // lanes per point (GL: the tuned maximum)
   constexpr int PPW = 64 / G;           // points per group
   constexpr int CPL = NC / G;           // configurations per lane
   constexpr int D = 2 * N + 1;
@@ -247,9 +247,16 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
   const int64_t nbatch = (p.n_points + 63) / 64;
   double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
 
-  for (int64_t b = blockIdx.x; b < nbatch; b += gridDim.x) {
+  // Work item = one group of PPW compacted points of a 64-point batch, so that
+  // a launch of few points still spreads over every CU.
+  constexpr int GPB = 64 / PPW; // groups per batch
+  const int64_t nitems = nbatch * GPB;
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int64_t b = it / GPB;
+    const int g0i = (int)(it - b * GPB) * PPW;
     const int cnt = buf.cnt[b];
-    for (int g0i = 0; g0i < cnt; g0i += PPW) {
+    if (g0i >= cnt) continue;
+    {
       const int ng = min(PPW, cnt - g0i);
       // Opaque per-iteration copy of the lane id: keeps the lane-only index
       // math of the matrix build from being hoisted into registers.
@@ -394,8 +401,14 @@ template <int N> static hipError_t launch_prep_n(const KdParams &p, const PrepBu
 }
 
 template <int N, typename T>
-static hipError_t launch_lu_n(const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((integrand_kernel<N, T>), dim3(grid), dim3(64), 0, st, p, buf);
+static hipError_t launch_lu_n(int lanes, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
+  // lane-count variants only where they differ (2^N > lanes)
+  if (lanes == 16 && N >= 5)
+    hipLaunchKernelGGL((integrand_kernel<N, T, 16>), dim3(grid), dim3(64), 0, st, p, buf);
+  else if (lanes == 32 && N >= 6)
+    hipLaunchKernelGGL((integrand_kernel<N, T, 32>), dim3(grid), dim3(64), 0, st, p, buf);
+  else
+    hipLaunchKernelGGL((integrand_kernel<N, T, 64>), dim3(grid), dim3(64), 0, st, p, buf);
   return hipGetLastError();
 }
 
@@ -423,13 +436,14 @@ hipError_t launch_prep(int order, const KdParams &p, const PrepBuf &buf, int gri
 #undef KD_PREP_CALL
 }
 
-hipError_t launch_lu(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
-  if (precision == KD_PREC_FP32_LU) {
-#define KD_LU32_CALL(n) launch_lu_n<n, float>(p, buf, grid, st)
+hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, const PrepBuf &buf, int grid,
+                     hipStream_t st) {
+  if (lu_is_fp32(order, precision)) {
+#define KD_LU32_CALL(n) launch_lu_n<n, float>(lanes, p, buf, grid, st)
     KD_ORDER_SWITCH(order, KD_LU32_CALL)
 #undef KD_LU32_CALL
   }
-#define KD_LU64_CALL(n) launch_lu_n<n, double>(p, buf, grid, st)
+#define KD_LU64_CALL(n) launch_lu_n<n, double>(lanes, p, buf, grid, st)
   KD_ORDER_SWITCH(order, KD_LU64_CALL)
 #undef KD_LU64_CALL
 }

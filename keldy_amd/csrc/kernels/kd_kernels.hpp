@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../../include/keldy_amd.h"
 #include "kd_device.hpp"
 #include "kd_flow_device.hpp"
 
@@ -10,9 +11,51 @@ namespace kd {
 
 // prep_kernel<order>: grid blocks of 256 threads over p.n_points points.
 hipError_t launch_prep(int order, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
-// integrand_kernel<order, T> (the Keldysh sum): grid blocks of 64 threads.
-hipError_t launch_lu(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
+// LU precision actually used: FP32 for KD_PREC_FP32_LU below order 10 and
+// for KD_PREC_FP32_LU_ALL; FP64 otherwise.
+inline bool lu_is_fp32(int order, int precision) {
+  return precision == KD_PREC_FP32_LU_ALL || (precision == KD_PREC_FP32_LU && order < 10);
+}
+// lanes per point of the Keldysh-sum kernel for a tuned maximum
+inline int lu_lanes(int order, int lanes_max) {
+  const int nc = 1 << order;
+  return nc < lanes_max ? nc : lanes_max;
+}
+// integrand_kernel<order, T, lanes> (the Keldysh sum): grid blocks of 64
+// threads; lanes = the maximum lanes per point (64, 32 or 16).
+hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, const PrepBuf &buf, int grid,
+                     hipStream_t st);
 // acc[0..3] += sum over blocks of part[4*blk + 0..3] (fixed order).
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st);
+
+// ---- learned flow warper (kd_flow.hip) ----
+struct FlowMapIO {
+  const uint32_t *Csr, *shfr; // Sobol source (when in_l is null): points k0+1 ...
+  uint32_t k0;
+  const double *in_l; // or caller points [n][order]
+  int64_t n_points;
+  double *out_x;   // [n][order]
+  double *out_jac; // [n]
+  double *out_l;   // optional: the l points used [n][order]
+};
+struct FlowTrainIO {
+  const double *x; // training points [n][order]
+  const double *w; // weights |integrand| [n]
+  int64_t n_points, offset;
+  double inv_norm; // 1 / (mean(w) * batch)
+  float *part;     // per-block gradients [grid][n_params]
+  double *loss_part;
+  int n_params;
+};
+// flow_map_kernel<order>: 64-thread blocks, grid-stride over 64-point batches.
+hipError_t launch_flow_map(int order, const FlowParams &fp, const FlowMapIO &io, int grid, hipStream_t st);
+// flow_train_kernel<order>: one 64-point batch per 64-thread block.
+hipError_t launch_flow_train(int order, const FlowParams &fp, const FlowTrainIO &io, int grid, hipStream_t st);
+hipError_t launch_flow_grad_reduce(const float *part, int nblk, int n_params, float *grad, const double *loss_part,
+                                   double *loss, hipStream_t st);
+hipError_t launch_flow_adam(float *w, float *m, float *v, const float *g, int n, float lr, float b1, float b2,
+                            float eps, int step, hipStream_t st);
+hipError_t launch_flow_pack(const float *w, uint16_t *pk, int order, int layers, hipStream_t st);
+hipError_t launch_flow_abs(const double *vals, int64_t n, double *w, double *part, int grid, hipStream_t st);
 
 } // namespace kd

@@ -14,6 +14,8 @@ G0KeldyshContourT = _m.G0KeldyshContourT
 IntegrandGDirect = _m.IntegrandGDirect
 ComputeChargeQDirect = _m.ComputeChargeQDirect
 ComputeCurrentJDirect = _m.ComputeCurrentJDirect
+IntegratorDirect = _m.IntegratorDirect
+run_batch = _m.run_batch
 
 __all__ = ["GfIndexT", "G0ModelOmega", "G0Model", "G0KeldyshContourT", "IntegrandGDirect",
-           "ComputeChargeQDirect", "ComputeCurrentJDirect"]
+           "ComputeChargeQDirect", "ComputeCurrentJDirect", "IntegratorDirect", "run_batch"]

@@ -1,0 +1,90 @@
+"""Batched uneven orders in one hipGraph launch, the launch-shape autotuner and
+the FP32-LU precision modes, through the C-ABI.
+
+- kd_run_batch over orders 1..N with uneven point counts gives, per order, the
+  accumulator a separate kd_run gives on the same points (same kernels, same
+  fixed-order reductions: equal to 1e-15 relative).
+- kd_autotune leaves generator state and accumulator untouched; any launch
+  shape gives the same sum up to FP64 reassociation (1e-12 relative).
+- FP32 LU vs FP64 per order on the same points: relative error of the
+  accumulated estimate <= 1e-4 (orders 1-8); KD_PREC_FP32_LU keeps FP64 from
+  order 10 (identical to the FP64 run).
+"""
+import numpy as np
+import pytest
+
+from keldy_amd import capi
+from oracle import oracle as O
+from tests import gpu_util as U
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(flat_g0, order, seed, prec=capi.KD_PREC_FP64, t=10.0):
+    tr, (descs, maps, tab) = U.chain([("plasma", t), ("map", "inverse", t, 1.0, 1)])
+    ctx = U.charge_ctx(flat_g0, order, t, prec)
+    ctx.set_warper_chain(descs, maps, tab)
+    s = O.Sobol(order, 0, True, True, seed)
+    ctx.set_sobol(*s.tables(), 0)
+    return ctx
+
+
+def _rel(a, b):
+    a, b = complex(a.re, a.im), complex(b.re, b.im)
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def test_run_batch_matches_individual_runs(flat_g0):
+    orders = list(range(1, 9))
+    counts = [(1 << 18) // (n * n) for n in orders]
+    batch = [_ctx(flat_g0, n, 40 + n) for n in orders]
+    accs, ms = capi.run_batch(batch, counts)
+    assert ms > 0
+    for n, m, a, c in zip(orders, counts, accs, batch):
+        ref = _ctx(flat_g0, n, 40 + n).run(m)
+        assert a.n_points == ref.n_points == m and a.n_in_domain == ref.n_in_domain
+        assert _rel(a, ref) <= 1e-15, f"order {n}"
+        assert c.sobol_state() == m
+    # continuation: a second batch call picks up where the first stopped
+    accs2, _ = capi.run_batch(batch[:3], counts[:3])
+    for n, m, a in zip(orders[:3], counts[:3], accs2):
+        ref = _ctx(flat_g0, n, 40 + n)
+        ref.run(m)
+        r2 = ref.run(m)
+        assert _rel(a, r2) <= 1e-15
+
+
+def test_run_batch_errors(flat_g0):
+    c = _ctx(flat_g0, 3, 1)
+    with pytest.raises(capi.KdError, match="twice"):
+        capi.run_batch([c, c], [64, 64])
+
+
+@pytest.mark.parametrize("order", [3, 8])
+def test_autotune(flat_g0, order):
+    ctx = _ctx(flat_g0, order, 7)
+    lanes, wpc, ms = ctx.autotune(1 << 16)
+    assert lanes in (16, 32, 64) or lanes == 1 << order
+    assert wpc in (8, 16, 32) and ms > 0
+    assert ctx.sobol_state() == 0 and ctx.accum().n_points == 0 and ctx.accum().re == 0.0
+    a = ctx.run(1 << 16)
+    ref = _ctx(flat_g0, order, 7)
+    ref.set_launch_shape(64, 16)
+    r = ref.run(1 << 16)
+    assert _rel(a, r) <= 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2, 4, 6, 8, 10])
+def test_fp32_error_per_order(flat_g0, order):
+    m = max(1 << 12, (1 << 18) // (order * order))
+    r64 = _ctx(flat_g0, order, 3).run(m)
+    r32 = _ctx(flat_g0, order, 3, capi.KD_PREC_FP32_LU).run(m)
+    rel = _rel(r32, r64)
+    print(f"order {order}: FP32-LU relative error of the estimate {rel:.3e}")
+    if order >= 10:
+        assert rel <= 1e-15  # FP64 LU kept from order 10
+        rall = _ctx(flat_g0, order, 3, capi.KD_PREC_FP32_LU_ALL).run(m)
+        print(f"order {order}: FP32 at every order: {_rel(rall, r64):.3e}")
+        assert _rel(rall, r64) <= 1e-2
+    else:
+        assert rel <= 1e-4

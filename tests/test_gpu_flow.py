@@ -1,0 +1,154 @@
+"""The learned flow warper on the GPU: the map and the training gradient of the
+hand-written HIP kernels against the FP64 flow oracle (oracle/flow_oracle.py,
+bf16 operand rounding emulated), the flow inside the warper chain against the
+oracle chain, and what training gains over the estimator without it.
+
+Tolerances:
+  map x = T(l): 99% of points within 1e-6 (x) and 1e-5 (log J), all within 1e-4
+      and 1e-3: the network runs in FP32 with FP32-accumulated MFMA products,
+      the oracle in FP64 (typical |dx| ~ 3e-9 at n = 8, 6 layers); rarely
+      the two land on opposite sides of a bf16 rounding tie of a hidden unit,
+      which moves that unit by one bf16 ulp
+  loss: relative 1e-5; gradient: normwise relative <= 3e-2 per layer
+      (the backward products take bf16 operands: dz, h, dO, a)
+  chain with flow: normwise relative 1e-5 against oracle chain at the oracle's x
+"""
+import numpy as np
+import pytest
+
+from keldy_amd import capi
+from oracle import flow_oracle as F
+from oracle import oracle as O
+from tests import gpu_util as U
+from tests.flow_util import flow_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,layers", [(1, 2), (3, 4), (8, 6), (12, 3)])
+def test_flow_map_vs_oracle(n, layers):
+    ctx = capi.Context(0)
+    p = flow_params(n, layers, 100 + n)
+    ctx.flow_set_params(n, layers, p)
+    rng = np.random.default_rng(n)
+    l = rng.uniform(0.0, 1.0, (1000, n))
+    l[0] = 0.0      # clamped to 2^-40 on both sides
+    l[1] = 1.0 - 1e-17
+    x, jac = ctx.flow_map(l)
+    # the two boundary points: finite, inside the cube (J may underflow to 0)
+    assert np.all(np.isfinite(x[:2])) and np.all((x[:2] >= 0) & (x[:2] <= 1))
+    assert np.all(np.isfinite(jac[:2])) and np.all(jac[:2] >= 0)
+    xo, ljo = F.forward(p, n, layers, l[2:], emulate=True)
+    dx = np.max(np.abs(x[2:] - xo), axis=1)
+    dj = np.abs(np.log(jac[2:]) - ljo)
+    assert np.quantile(dx, 0.99) <= 1e-6 and np.max(dx) <= 1e-4
+    assert np.quantile(dj, 0.99) <= 1e-5 and np.max(dj) <= 1e-3
+    # the device parameters round-trip unchanged
+    np.testing.assert_array_equal(ctx.flow_get_params(), p.astype(np.float32))
+
+
+def test_flow_identity_init():
+    ctx = capi.Context(0)
+    ctx.flow_init(8, 6, seed=3)
+    l = np.random.default_rng(0).uniform(0.001, 0.999, (256, 8))
+    x, jac = ctx.flow_map(l)
+    np.testing.assert_allclose(x, l, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(jac, 1.0, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("n,layers", [(2, 2), (3, 4), (8, 6)])
+def test_flow_gradient_vs_oracle(n, layers):
+    ctx = capi.Context(0)
+    p = flow_params(n, layers, 7 * n)
+    ctx.flow_set_params(n, layers, p)
+    rng = np.random.default_rng(11)
+    m = 2048 + 37  # a partial last block of 64 points
+    x = rng.uniform(0.02, 0.98, (m, n))
+    w = rng.uniform(0.0, 2.0, m)
+    loss, g = ctx.flow_loss_grad(x, w)
+    lo, go = F.loss_and_grad(p, n, layers, x, w, emulate=True)
+    assert abs(loss - lo) <= 1e-5 * max(abs(lo), 1e-3)
+    off = 0
+    for k in range(layers):
+        sz = F.layer_size(n, k)
+        gk, gok = g[off:off + sz].astype(np.float64), go[off:off + sz]
+        assert np.linalg.norm(gk - gok) <= 3e-2 * np.linalg.norm(gok), f"layer {k}"
+        off += sz
+
+
+def _chain_with_flow(flat_g0, order, t, p, layers):
+    tr, (descs, maps, tab) = U.chain([("plasma", t), ("map", "inverse", t, 1.0, 1)])
+    descs = list(descs) + [capi.WarperDesc(capi.KD_WARP_FLOW, 0, 0.0, 0)]
+    ctx = U.charge_ctx(flat_g0, order, t)
+    ctx.flow_set_params(order, layers, p)
+    ctx.set_warper_chain(descs, maps, tab)
+    return ctx, tr
+
+
+@pytest.mark.parametrize("order", [2, 5])
+def test_chain_with_flow_vs_oracle(flat_g0, order):
+    t, layers = 10.0, 4
+    p = flow_params(order, layers, 31)
+    ctx, tr = _chain_with_flow(flat_g0, order, t, p, layers)
+    l = np.random.default_rng(order).uniform(0, 1, (512, order))
+    got, dom, jac = ctx.eval_warped_batch(l)
+    xo, ljo = F.forward(p, order, layers, l, emulate=True)
+    f = O.charge_integrand(U.oracle_charge(flat_g0), t)
+    ref = np.array([O.eval_warped(f, tr, xi)[0] for xi in xo]) * np.exp(ljo)
+    scale = np.max(np.abs(ref))
+    d = np.abs(got - ref) / scale
+    assert np.quantile(d, 0.99) <= 1e-6 and np.max(d) <= 1e-3
+    # run over Sobol points == sum of the batch values at the same points
+    s = O.Sobol(order, 0, True, True, 5)
+    pts = s.points(4096)
+    ctx.set_sobol(*s.tables(), 0)
+    acc = ctx.run(4096)
+    v, _, _ = ctx.eval_warped_batch(pts)
+    assert abs(complex(acc.re, acc.im) - v.sum()) <= 1e-12 * np.sum(np.abs(v))
+
+
+def test_flow_must_be_last(flat_g0):
+    ctx = U.charge_ctx(flat_g0, 3, 10.0)
+    ctx.flow_init(3, 2)
+    descs = [capi.WarperDesc(capi.KD_WARP_FLOW, 0, 0.0, 0), capi.WarperDesc(capi.KD_WARP_PLASMA, 0, 10.0, 0)]
+    with pytest.raises(capi.KdError, match="last warper"):
+        ctx.set_warper_chain(descs, [], np.zeros(0))
+
+
+def _replicas(c, reps, m, seed0):
+    est = []
+    for r in range(reps):
+        c.reset_rng("sobol", 0, True, True, seed0 + r)
+        n0 = c.reduce_nr_points_run()
+        s0 = c.reduce_result() * n0 if n0 else 0j
+        c.run(m)
+        est.append((c.reduce_result() * c.reduce_nr_points_run() - s0) / m)
+    return np.array(est)
+
+
+def test_training_gain_and_unbiasedness():
+    """Order 3 charge, crude plain chain [plasma(10), uniform(10)]: the trained
+    flow must lower the loss, keep the estimate unbiased (agreement with the
+    plain estimator within the RQMC error) and reduce the RQMC variance on the
+    same replica seeds."""
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t, order = 10.0, 3
+    par = dict(beta=-1.0, bias_V=0.0, eps_d=0.0, Gamma=1.0, time_max=t, nr_time_points_gf=10001,
+               alpha=0.0, bath_type="flatband", ft_method="analytic")
+    model = kio.G0Model(kio.G0ModelOmega(par), False)
+    c = kio.ComputeChargeQDirect(model, t, order, 0.0)
+    c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    c.warper.emplace_back(kw.make_product_1d_simple_uniform(t))
+    plain = _replicas(c, 8, 1 << 14, 50)
+    flow, losses = kw.train_flow_warper(c, n_layers=6, nr_samples=1 << 16, steps=400, batch=1 << 12,
+                                        lr=3e-3, lr_final=1e-4)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < losses[:5].mean() - 0.1
+    c.warper.emplace_back(flow)
+    learned = _replicas(c, 8, 1 << 14, 50)
+    sp, sl = plain.std(ddof=1), learned.std(ddof=1)
+    err = np.hypot(sp, sl) / np.sqrt(8)
+    assert abs(learned.mean() - plain.mean()) <= 5 * err
+    ratio = sp ** 2 / sl ** 2
+    assert ratio > 2.0, f"variance ratio plain/learned = {ratio:.2f}"

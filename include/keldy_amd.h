@@ -156,6 +156,18 @@ int kd_last_timing(const kd_ctx *ctx, double *total_ms, double *lu_ms, double *p
  * (the algorithm's own count, used for roofline figures). */
 int kd_flops_per_point(const kd_ctx *ctx, double *flops);
 
+/* ---- time binning --------------------------------------------------------
+ * Replaces integrand_g_direct_time (impurity_oneband/wick_direct_time.cpp:42-163)
+ * with the integrator's binner_t<1> result (binner.hpp:147-352): in kd_run,
+ * each point's warped value is also added to the bin of its earliest vertex
+ * time (bins of width (xmax - xmin) / nbins on [xmin, xmax], xmax in the last
+ * bin), and every point, in domain or not, counts one value added to that bin
+ * or one dropped value when its earliest time is outside [xmin, xmax].
+ * nbins = 0 switches binning off.  values: nbins x (re, im) sums (divide by
+ * the points run for the estimate); counts: nbins. */
+int kd_set_time_bins(kd_ctx *ctx, double xmin, double xmax, int32_t nbins);
+int kd_get_time_bins(kd_ctx *ctx, double *values, uint64_t *counts, uint64_t *dropped);
+
 /* ---- launch shape and batched runs ---------------------------------------
  * No counterpart in the reference (its integrator runs one order per object).
  * kd_autotune times the Keldysh-sum kernel on sample_points Sobol points of
@@ -190,8 +202,10 @@ int kd_flow_loss_grad(kd_ctx *ctx, const double *x, const double *w, int64_t n, 
                       double *loss);
 /* Training set: the next n_samples Sobol points of the ctx's generator (which
  * advances) pushed through the warper chain WITHOUT the flow; keeps the points
- * and |warped integrand| on the device; returns mean |warped integrand|. */
-int kd_flow_collect(kd_ctx *ctx, int64_t n_samples, double *w_mean);
+ * and the weights w = |warped integrand|^weight_power on the device (1: the
+ * flow's density targets |integrand|; below 1: a tempered target); returns
+ * the mean weight. */
+int kd_flow_collect(kd_ctx *ctx, int64_t n_samples, double weight_power, double *w_mean);
 /* Adam steps on consecutive aligned batches of the training set (batch | n_samples,
  * 64 | batch, batch <= 2^16), learning rate cosine-decayed from lr to lr_final; per-step loss
  * into losses (steps doubles, may be NULL). */

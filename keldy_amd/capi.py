@@ -51,7 +51,8 @@ SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_se
            "kd_run", "kd_get_accum", "kd_reset_accum", "kd_sample_device", "kd_last_timing",
            "kd_flops_per_point", "kd_flow_n_params", "kd_flow_init", "kd_flow_set_params",
            "kd_flow_get_params", "kd_flow_map", "kd_flow_loss_grad", "kd_flow_collect", "kd_flow_train",
-           "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch"]
+           "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch",
+           "kd_set_time_bins", "kd_get_time_bins"]
 
 _lib = None
 
@@ -93,12 +94,14 @@ def lib():
         L.kd_flow_get_params.argtypes = [vp, fp, C.c_int64]
         L.kd_flow_map.argtypes = [vp, dp, C.c_int64, dp, dp]
         L.kd_flow_loss_grad.argtypes = [vp, dp, dp, C.c_int64, C.c_double, fp, dp]
-        L.kd_flow_collect.argtypes = [vp, C.c_int64, dp]
+        L.kd_flow_collect.argtypes = [vp, C.c_int64, C.c_double, dp]
         L.kd_flow_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_float, dp]
         L.kd_autotune.argtypes = [vp, C.c_int64, ip, ip, dp]
         L.kd_set_launch_shape.argtypes = [vp, C.c_int32, C.c_int32]
         L.kd_get_launch_shape.argtypes = [vp, ip, ip]
         L.kd_run_batch.argtypes = [C.POINTER(vp), C.POINTER(C.c_uint64), C.c_int32, C.POINTER(Accum), dp]
+        L.kd_set_time_bins.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
+        L.kd_get_time_bins.argtypes = [vp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -221,6 +224,19 @@ class Context:
         self._check(lib().kd_flops_per_point(self.h, C.byref(f)))
         return f.value
 
+    # -- time binning ----------------------------------------------------
+    def set_time_bins(self, xmin, xmax, nbins):
+        self._check(lib().kd_set_time_bins(self.h, xmin, xmax, nbins))
+        self.nbins = nbins
+
+    def time_bins(self):
+        """-> (complex sums per bin, values added per bin, dropped count)"""
+        v = np.empty((self.nbins, 2))
+        cnt = np.empty(self.nbins, dtype=np.uint64)
+        d = C.c_uint64()
+        self._check(lib().kd_get_time_bins(self.h, _dp(v), cnt.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(d)))
+        return v[:, 0] + 1j * v[:, 1], cnt, d.value
+
     # -- launch shape ----------------------------------------------------
     def autotune(self, sample_points):
         """-> (lanes per point, waves per CU, LU ms per 2^20 points)"""
@@ -269,9 +285,9 @@ class Context:
                                             g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
         return loss.value, g
 
-    def flow_collect(self, n_samples):
+    def flow_collect(self, n_samples, weight_power=1.0):
         wm = C.c_double()
-        self._check(lib().kd_flow_collect(self.h, n_samples, C.byref(wm)))
+        self._check(lib().kd_flow_collect(self.h, n_samples, weight_power, C.byref(wm)))
         return wm.value
 
     def flow_train(self, steps, batch, lr, lr_final):

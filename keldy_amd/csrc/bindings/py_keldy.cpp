@@ -86,6 +86,8 @@ template <typename W> void bind_product_1d(py::module_ &m, const char *name) {
   bind_map_methods<W>(c);
 }
 
+template <typename T, typename C> void bind_integrator_methods(C &c);
+
 template <typename T> void bind_integrator(py::module_ &m, const char *name) {
   py::class_<T, integrator_direct> c(m, name);
   c.def(py::init<g0_model, double, int, double>(), py::arg("model"), py::arg("time"), py::arg("order"),
@@ -94,7 +96,13 @@ template <typename T> void bind_integrator(py::module_ &m, const char *name) {
              return new T(params_from_dict(params), time, order, cutoff);
            }),
            py::arg("params"), py::arg("time"), py::arg("order"), py::arg("cutoff_integrand"))
-      .def_readwrite("warper", &T::warper)
+      .def("reduce_result", &T::reduce_result)
+      .def("get_integrand", &T::get_integrand);
+  bind_integrator_methods<T>(c);
+}
+
+template <typename T, typename C> void bind_integrator_methods(C &c) {
+  c.def_readwrite("warper", &T::warper)
       .def(
           "evaluate_warped_integrand",
           [](T &s, std::vector<double> li, int start, bool keep) { return s.evaluate_warped_integrand(li, start, keep); },
@@ -122,11 +130,9 @@ template <typename T> void bind_integrator(py::module_ &m, const char *name) {
       .def("run", &T::run, py::arg("nr_steps"), py::call_guard<py::gil_scoped_release>())
       .def("reset_rng", &T::reset_rng, py::arg("rng_name"), py::arg("rng_state_seed"), py::arg("do_shift") = false,
            py::arg("do_scramble") = false, py::arg("rng_seed_shift") = 0)
-      .def("reduce_result", &T::reduce_result)
       .def("reduce_nr_points_run", &T::reduce_nr_points_run)
       .def("reduce_nr_points_in_domain", &T::reduce_nr_points_in_domain)
       .def("reduce_nr_fallback", &T::reduce_nr_fallback)
-      .def("get_integrand", &T::get_integrand)
       .def("get_warper", &T::get_warper)
       .def(
           "set_precision",
@@ -148,11 +154,12 @@ template <typename T> void bind_integrator(py::module_ &m, const char *name) {
       .def(
           "train_flow_warper",
           [](T &s, int n_layers, long nr_samples, int steps, int batch, double lr, double lr_final, int seed_shift,
-             uint64_t init_seed) {
+             uint64_t init_seed, double weight_power) {
             std::pair<warper_flow_t, std::vector<double>> r;
             {
               py::gil_scoped_release rel;
-              r = s.train_flow_warper(n_layers, nr_samples, steps, batch, lr, lr_final, seed_shift, init_seed);
+              r = s.train_flow_warper(n_layers, nr_samples, steps, batch, lr, lr_final, seed_shift, init_seed,
+                                      weight_power);
             }
             py::array_t<double> losses(py::ssize_t(r.second.size()));
             std::copy(r.second.begin(), r.second.end(), losses.mutable_data());
@@ -160,7 +167,7 @@ template <typename T> void bind_integrator(py::module_ &m, const char *name) {
           },
           py::arg("n_layers") = 6, py::arg("nr_samples") = 1 << 20, py::arg("steps") = 2000,
           py::arg("batch") = 1 << 14, py::arg("lr") = 1e-3, py::arg("lr_final") = 1e-5,
-          py::arg("rng_seed_shift") = 1000003, py::arg("init_seed") = 0,
+          py::arg("rng_seed_shift") = 1000003, py::arg("init_seed") = 0, py::arg("weight_power") = 1.0,
           "Train a learned coupling-flow warper on |warped integrand| samples of the current chain "
           "(returns (WarperFlowT, per-step loss)); append it last to the warper train.");
 }
@@ -424,6 +431,60 @@ PYBIND11_MODULE(_keldy_host, mod) {
   py::class_<integrator_direct>(im, "IntegratorDirect", "common base of the compute_* integrators");
   bind_integrator<compute_charge_Q_direct>(im, "ComputeChargeQDirect");
   bind_integrator<compute_current_J_direct>(im, "ComputeCurrentJDirect");
+  // time-binned charge (compute_obs.hpp:85-101, wick_direct_time.hpp)
+  py::class_<binner::binner_1d_t>(im, "Binner1", "binner_t<1> (binner.hpp:147-352)")
+      .def("get_data", [](binner::binner_1d_t const &b) {
+        py::array_t<dcomplex> a(py::ssize_t(b.data.size()));
+        std::copy(b.data.begin(), b.data.end(), a.mutable_data());
+        return a;
+      })
+      .def("get_nr_values_added", [](binner::binner_1d_t const &b) {
+        py::array_t<long> a(py::ssize_t(b.nr_values_added.size()));
+        std::copy(b.nr_values_added.begin(), b.nr_values_added.end(), a.mutable_data());
+        return a;
+      })
+      .def("get_nr_values_dropped", [](binner::binner_1d_t const &b) { return b.nr_values_dropped; })
+      .def("get_nr_bins", [](binner::binner_1d_t const &b, int axis) {
+        if (axis != 0) throw keldy_error("Axis " + std::to_string(axis) + " is larger than the number of dimensions.");
+        return b.nr_bins;
+      }, py::arg("axis") = 0)
+      .def("get_bin_coord", [](binner::binner_1d_t const &b, int axis) {
+        if (axis != 0) throw keldy_error("Axis " + std::to_string(axis) + " is larger than the number of dimensions.");
+        return b.get_bin_coord();
+      }, py::arg("axis") = 0)
+      .def("get_bin_size", [](binner::binner_1d_t const &b, int axis) {
+        if (axis != 0) throw keldy_error("Axis " + std::to_string(axis) + " is larger than the number of dimensions.");
+        return b.get_bin_size();
+      }, py::arg("axis") = 0);
+  py::class_<binner::sparse_binner_1d_t>(im, "SparseBinner1", "sparse_binner_t<1> (binner.hpp:59-140)")
+      .def_readonly("data", &binner::sparse_binner_1d_t::data)
+      .def("sum_moduli", &binner::sparse_binner_1d_t::sum_moduli);
+  py::class_<integrand_g_direct_time>(im, "IntegrandGDirectTime")
+      .def(py::init<g0_keldysh_contour_t, gf_index_t, gf_index_t, double>(), py::arg("g0_"), py::arg("external_A_"),
+           py::arg("external_B_"), py::arg("cutoff_") = 0.)
+      .def("__call__", &integrand_g_direct_time::operator(), py::arg("times"), py::arg("keep_u_hypercube") = true);
+  {
+    using T = compute_charge_Q_direct_time;
+    py::class_<T, integrator_direct> c(im, "ComputeChargeQDirectTime");
+    c.def(py::init<g0_model, double, int, int, double>(), py::arg("model"), py::arg("time"), py::arg("order"),
+          py::arg("nr_time_slices"), py::arg("cutoff_integrand"))
+        .def(py::init([](py::dict params, double time, int order, int slices, double cutoff) {
+               return new T(params_from_dict(params), time, order, slices, cutoff);
+             }),
+             py::arg("params"), py::arg("time"), py::arg("order"), py::arg("nr_time_slices"),
+             py::arg("cutoff_integrand"))
+        .def("reduce_result", &T::reduce_result_binned)
+        .def("get_integrand", &T::get_integrand_time)
+        .def("evaluate_warped_integrand", &T::evaluate_warped_integrand_time, py::arg("li_vec"),
+             py::arg("start_domain_nr"), py::arg("keep_u_hypercube") = true)
+        .def(
+            "evaluate_warped_integrand",
+            [](T &s, std::vector<double> li) {
+              return s.evaluate_warped_integrand_time(li, int(s.warper.size()), true);
+            },
+            py::arg("li_vec"));
+    bind_integrator_methods<T>(c);
+  }
   im.def(
       "run_batch",
       [](py::list integrators, std::vector<long> nr_steps) {

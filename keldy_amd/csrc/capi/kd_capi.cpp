@@ -69,7 +69,8 @@ struct kd_ctx {
   size_t stage_cap = 0;
   // compacted points between the two kernel stages
   void *d_prep = nullptr;
-  size_t prep_cap = 0;
+  size_t prep_cap = 0, prep_pts = 0;
+  int prep_order = -1;
   float last_ms = 0.f, last_lu_ms = 0.f, last_prep_ms = 0.f;
   int last_launches = 0, last_chunks = 0;
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
@@ -77,6 +78,12 @@ struct kd_ctx {
   int lu_lanes_max = 64, lu_waves_per_cu = 16;
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
+  // time binning (kd_set_time_bins)
+  int nbins = 0;
+  double bin_min = 0.0, bin_max = 0.0;
+  double2 *d_bins = nullptr, *d_bin_part = nullptr;
+  unsigned long long *d_bin_counts = nullptr;
+  int bin_part_cap = 0;
 };
 
 namespace {
@@ -117,6 +124,15 @@ int grid_for(const kd_ctx *c, int64_t n_points) {
   return (int)(nb < 1 ? 1 : nb);
 }
 
+int ensure_bin_partials(kd_ctx *c, int grid) {
+  if (c->nbins <= 0 || grid <= c->bin_part_cap) return KD_OK;
+  dfree(c->d_bin_part);
+  c->bin_part_cap = 0;
+  KD_HIP(c, hipMalloc(&c->d_bin_part, sizeof(double2) * (size_t)c->nbins * (size_t)grid));
+  c->bin_part_cap = grid;
+  return KD_OK;
+}
+
 int ensure_partials(kd_ctx *c, int grid) {
   if (grid <= c->part_cap) return KD_OK;
   dfree(c->d_part);
@@ -146,6 +162,11 @@ kd::KdParams base_params(const kd_ctx *c) {
   p.Csr = c->d_Csr;
   p.shfr = c->d_shfr;
   p.k0 = c->k;
+  p.nbins = c->nbins;
+  p.bin_min = c->bin_min;
+  p.bin_max = c->bin_max;
+  p.bin_size = c->nbins > 0 ? (c->bin_max - c->bin_min) / c->nbins : 0.0;
+  p.bin_counts = c->d_bin_counts;
   return p;
 }
 
@@ -158,23 +179,30 @@ int need_integrand(kd_ctx *c) {
 // Points per launch pair (prep + integrand); bounds the compacted-point buffer.
 constexpr int64_t kChunk = int64_t(1) << 21;
 
-int ensure_prep(kd_ctx *c, int order, kd::PrepBuf *buf) {
-  const size_t n = (size_t)kChunk;
-  const size_t bytes = n * (sizeof(double) * (size_t)(order > 0 ? order : 1) + sizeof(double) + sizeof(int)) +
-                       (n / 64) * sizeof(int) + 256;
-  if (bytes > c->prep_cap) {
+// Compacted-point buffer for chunks of up to n_points points (at most kChunk):
+// u [n][order], jac [n], idx [n], cnt [n / 64].  Grows on demand only, so many
+// small contexts (one per external time, say) stay small.
+int ensure_prep(kd_ctx *c, int order, int64_t n_points, kd::PrepBuf *buf) {
+  const size_t n = (size_t)((std::min<int64_t>(std::max<int64_t>(n_points, 64), kChunk) + 63) / 64 * 64);
+  const size_t per_pt = sizeof(double) * (size_t)(order > 0 ? order : 1) + sizeof(double) + sizeof(int);
+  if (n > c->prep_pts || c->prep_order != order) {
+    const size_t m = std::max(n, c->prep_order == order ? c->prep_pts : (size_t)0);
+    const size_t bytes = m * per_pt + (m / 64) * sizeof(int) + 256;
     dfree(c->d_prep);
     c->prep_cap = 0;
     KD_HIP(c, hipMalloc(&c->d_prep, bytes));
     c->prep_cap = bytes;
+    c->prep_pts = m;
+    c->prep_order = order;
   }
+  const size_t np = c->prep_pts;
   char *base = (char *)c->d_prep;
   buf->u = (double *)base;
-  base += n * sizeof(double) * (size_t)(order > 0 ? order : 1);
+  base += np * sizeof(double) * (size_t)(order > 0 ? order : 1);
   buf->jac = (double *)base;
-  base += n * sizeof(double);
+  base += np * sizeof(double);
   buf->idx = (int *)base;
-  base += n * sizeof(int);
+  base += np * sizeof(int);
   buf->cnt = (int *)base;
   return KD_OK;
 }
@@ -213,11 +241,13 @@ int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) { return lau
 // Allocate every buffer a launch of p0 needs (before a graph capture).
 int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
   kd::PrepBuf buf;
-  int rc = ensure_prep(c, c->order, &buf);
+  int rc = ensure_prep(c, c->order, p0.n_points, &buf);
   if (rc) return rc;
   if (c->flow_in_chain && p0.mode != kd::MODE_EVAL_U && c->order > 0 && (rc = ensure_flow_stage(c, c->order)))
     return rc;
-  return ensure_partials(c, grid_for(c, std::min<int64_t>(kChunk, p0.n_points)));
+  const int grid = grid_for(c, std::min<int64_t>(kChunk, p0.n_points));
+  if ((rc = ensure_partials(c, grid))) return rc;
+  return p0.mode == kd::MODE_RUN ? ensure_bin_partials(c, grid) : KD_OK;
 }
 
 // Prep + integrand kernels over p0.n_points points in chunks (and, in
@@ -225,7 +255,7 @@ int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
 // each kernel is bracketed by HIP events on the ctx stream (kd_last_timing).
 int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
   kd::PrepBuf buf;
-  int rc = ensure_prep(c, c->order, &buf);
+  int rc = ensure_prep(c, c->order, p0.n_points, &buf);
   if (rc) return rc;
   const int N = c->order;
   // The flow warper, when last in the chain, is applied first: its own stage
@@ -256,6 +286,10 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
     if (p.mode == kd::MODE_RUN) {
       if ((rc = ensure_partials(c, grid))) return rc;
       p.partials = c->d_part;
+      if ((rc = ensure_bin_partials(c, grid))) return rc;
+      p.bin_part = c->d_bin_part;
+    } else {
+      p.nbins = 0;
     }
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], c->stream));
     if (flow) {
@@ -286,6 +320,10 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
     if (p.mode == kd::MODE_RUN && !c->no_reduce) {
       KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->d_acc, c->stream));
       launches += 1;
+      if (p.nbins > 0) {
+        KD_HIP(c, kd::launch_reduce_bins(c->d_bin_part, grid, p.nbins, c->d_bins, c->stream));
+        launches += 1;
+      }
     }
   }
   if (timed) KD_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -365,6 +403,9 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_tw);
   dfree(c->d_fpart);
   dfree(c->d_floss);
+  dfree(c->d_bins);
+  dfree(c->d_bin_part);
+  dfree(c->d_bin_counts);
   dfree(c->d_Csr);
   dfree(c->d_shfr);
   dfree(c->d_acc);
@@ -591,6 +632,7 @@ int kd_get_sobol_state(const kd_ctx *c, uint32_t *k) {
 static int run_common(kd_ctx *c, uint64_t n_steps, int mode, double *d_l, double *d_value) {
   int rc = need_integrand(c);
   if (rc) return rc;
+  if (c->nbins > 0 && c->order == 0) return fail(c, KD_ERR_ARG, "Order 0 is not implemented");
   if (!c->has_sobol) return fail(c, KD_ERR_STATE, "rng not set (kd_set_sobol)");
   if (c->sobol_dim != c->order) return fail(c, KD_ERR_ARG, "Sobol dimension differs from the integrand order");
   if ((uint64_t)c->k + n_steps > (1ull << 31))
@@ -637,6 +679,10 @@ int kd_reset_accum(kd_ctx *c) {
   if (!c) return KD_ERR_ARG;
   KD_HIP(c, hipSetDevice(c->device));
   KD_HIP(c, hipMemsetAsync(c->d_acc, 0, 4 * sizeof(double), c->stream));
+  if (c->nbins > 0) {
+    KD_HIP(c, hipMemsetAsync(c->d_bins, 0, sizeof(double2) * c->nbins, c->stream));
+    KD_HIP(c, hipMemsetAsync(c->d_bin_counts, 0, sizeof(unsigned long long) * (c->nbins + 1), c->stream));
+  }
   KD_HIP(c, hipStreamSynchronize(c->stream));
   c->n_points = 0;
   return KD_OK;
@@ -855,12 +901,13 @@ int kd_flow_loss_grad(kd_ctx *c, const double *x, const double *w, int64_t n, do
   return KD_OK;
 }
 
-int kd_flow_collect(kd_ctx *c, int64_t n_samples, double *w_mean) {
+int kd_flow_collect(kd_ctx *c, int64_t n_samples, double weight_power, double *w_mean) {
   if (!c) return KD_ERR_ARG;
   int rc = need_integrand(c);
   if (rc) return rc;
   if (n_samples < 64 || n_samples > (int64_t(1) << 24))
     return fail(c, KD_ERR_ARG, "training sample count must be in [64, 2^24]");
+  if (!(weight_power > 0.0) || weight_power > 4.0) return fail(c, KD_ERR_ARG, "weight_power must be in (0, 4]");
   const int N = c->order;
   KD_HIP(c, hipSetDevice(c->device));
   KD_HIP(c, hipStreamSynchronize(c->stream));
@@ -880,7 +927,7 @@ int kd_flow_collect(kd_ctx *c, int64_t n_samples, double *w_mean) {
   double *dpart = nullptr;
   if (!rc && hipMalloc(&dpart, sizeof(double) * grid) != hipSuccess) rc = fail(c, KD_ERR_HIP, "hipMalloc failed");
   std::vector<double> part(grid);
-  if (!rc && kd::launch_flow_abs(dvals, n_samples, c->d_tw, dpart, grid, c->stream) != hipSuccess)
+  if (!rc && kd::launch_flow_abs(dvals, n_samples, weight_power, c->d_tw, dpart, grid, c->stream) != hipSuccess)
     rc = fail(c, KD_ERR_HIP, "flow_abs launch failed");
   if (!rc && hipMemcpyAsync(part.data(), dpart, sizeof(double) * grid, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
     rc = fail(c, KD_ERR_HIP, "copy failed");
@@ -959,6 +1006,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   kd::KdParams p = base_params(c);
   p.mode = kd::MODE_RUN;
   p.n_points = sample_points;
+  p.nbins = 0; // no bin counts from tuning launches
   c->no_reduce = true;
   for (int li = 0; li < 3 && !rc; li++) {
     const int L = lane_opts[li];
@@ -1088,6 +1136,50 @@ int kd_run_batch(kd_ctx *const *ctxs, const uint64_t *n_steps, int32_t count, kd
     if (out && (rc = kd_get_accum(c, &out[i]))) return fail(c0, rc, c->err);
   }
   if (ms) *ms = t_ms;
+  return KD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// time binning (integrand_g_direct_time + binner_t<1>)
+
+int kd_set_time_bins(kd_ctx *c, double xmin, double xmax, int32_t nbins) {
+  if (!c) return KD_ERR_ARG;
+  if (nbins < 0 || nbins > 4096) return fail(c, KD_ERR_ARG, "number of bins must be in [0, 4096]");
+  if (nbins > 0 && !(xmin < xmax)) {
+    return fail(c, KD_ERR_ARG, "Boundaries of binning range are in wrong order (" + std::to_string(xmin) +
+                                   " >= " + std::to_string(xmax) + ")");
+  }
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  dfree(c->d_bins);
+  dfree(c->d_bin_part);
+  dfree(c->d_bin_counts);
+  c->bin_part_cap = 0;
+  c->nbins = 0;
+  if (nbins == 0) return KD_OK;
+  KD_HIP(c, hipMalloc(&c->d_bins, sizeof(double2) * nbins));
+  KD_HIP(c, hipMalloc(&c->d_bin_counts, sizeof(unsigned long long) * (nbins + 1)));
+  KD_HIP(c, hipMemset(c->d_bins, 0, sizeof(double2) * nbins));
+  KD_HIP(c, hipMemset(c->d_bin_counts, 0, sizeof(unsigned long long) * (nbins + 1)));
+  c->nbins = nbins;
+  c->bin_min = xmin;
+  c->bin_max = xmax;
+  return KD_OK;
+}
+
+int kd_get_time_bins(kd_ctx *c, double *values, uint64_t *counts, uint64_t *dropped) {
+  if (!c) return KD_ERR_ARG;
+  if (c->nbins <= 0) return fail(c, KD_ERR_STATE, "time bins not set (kd_set_time_bins)");
+  KD_HIP(c, hipSetDevice(c->device));
+  std::vector<unsigned long long> cnt(c->nbins + 1);
+  if (values)
+    KD_HIP(c, hipMemcpyAsync(values, c->d_bins, sizeof(double2) * c->nbins, hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipMemcpyAsync(cnt.data(), c->d_bin_counts, sizeof(unsigned long long) * (c->nbins + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  if (counts)
+    for (int i = 0; i < c->nbins; i++) counts[i] = cnt[i];
+  if (dropped) *dropped = cnt[c->nbins];
   return KD_OK;
 }
 

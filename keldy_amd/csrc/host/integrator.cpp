@@ -20,6 +20,19 @@ void device_ctx::check(int rc) const {
   if (rc != KD_OK) throw keldy_error(kd_last_error(ctx_.get()));
 }
 
+namespace binner {
+std::vector<double> binner_1d_t::get_bin_coord() const {
+  std::vector<double> c(static_cast<size_t>(nr_bins));
+  for (long i = 0; i < nr_bins; i++) c[size_t(i)] = xmin + (double(i) + 0.5) * get_bin_size();
+  return c;
+}
+double sparse_binner_1d_t::sum_moduli() const {
+  double s = 0.0;
+  for (auto const &d : data) s += std::abs(d.second);
+  return s;
+}
+} // namespace binner
+
 namespace impurity_oneband {
 
 namespace {
@@ -153,13 +166,13 @@ void integrator_direct::sync_warper() {
 
 std::pair<warpers::warper_flow_t, std::vector<double>>
 integrator_direct::train_flow_warper(int n_layers, long nr_samples, int steps, int batch, double lr, double lr_final,
-                                     int rng_seed_shift, uint64_t init_seed) {
+                                     int rng_seed_shift, uint64_t init_seed, double weight_power) {
   sync_warper();
   const sobol gen(dimension, 0, true, true, rng_seed_shift);
   ctx_.check(kd_set_sobol(ctx_.get(), gen.dim(), gen.csr().data(), gen.shfr().data(), gen.state()));
   rng_dirty_ = true; // the next run re-uploads the integrator's own generator
   double w_mean = 0.0;
-  ctx_.check(kd_flow_collect(ctx_.get(), int64_t(nr_samples), &w_mean));
+  ctx_.check(kd_flow_collect(ctx_.get(), int64_t(nr_samples), weight_power, &w_mean));
   ctx_.check(kd_flow_init(ctx_.get(), dimension, n_layers, init_seed));
   std::vector<double> losses(size_t(std::max(steps, 0)));
   ctx_.check(kd_flow_train(ctx_.get(), steps, batch, float(lr), float(lr_final), losses.data()));
@@ -297,6 +310,64 @@ compute_charge_Q_direct::compute_charge_Q_direct(g0_model model, double time, in
    : integrator_direct(integrand_g_direct(g0_keldysh_contour_t(std::move(model)), gf_index_t(time, up, forward),
                                           gf_index_t(time, up, backward), cutoff_integrand),
                        order, "sobol_unshifted", 0) {}
+
+// --------------------------------------------------------------------------
+// time-binned charge
+namespace {
+std::pair<binner::sparse_binner_1d_t, int> binned(dcomplex value, int in_domain, std::vector<double> const &times) {
+  binner::sparse_binner_1d_t r;
+  r.accumulate(value, *std::min_element(times.begin(), times.end()));
+  return {r, in_domain};
+}
+} // namespace
+
+std::pair<binner::sparse_binner_1d_t, int> integrand_g_direct_time::operator()(std::vector<double> const &times,
+                                                                              bool keep_u_hypercube) const {
+  if (times.empty()) throw keldy_error("Order 0 is not implemented");
+  auto [v, dom] = direct(times, keep_u_hypercube);
+  return binned(v, dom, times);
+}
+
+compute_charge_Q_direct_time::compute_charge_Q_direct_time(g0_model model, double time, int order,
+                                                           int nr_time_slices, double cutoff_integrand)
+   : integrator_direct(integrand_g_direct(g0_keldysh_contour_t(model), gf_index_t(time, up, forward),
+                                          gf_index_t(time, up, backward), cutoff_integrand),
+                       order, "sobol_unshifted", 0),
+     integrand_time(g0_keldysh_contour_t(std::move(model)), gf_index_t(time, up, forward),
+                    gf_index_t(time, up, backward), cutoff_integrand),
+     time_(time), nr_time_slices_(nr_time_slices) {
+  if (nr_time_slices < 1) throw keldy_error("nr_time_slices must be >= 1");
+  if (!(0.0 < time))
+    throw keldy_error("Boundaries of binning range are in wrong order (0 >= " + std::to_string(time) + ")");
+  ctx_.check(kd_set_time_bins(ctx_.get(), 0.0, time, nr_time_slices));
+}
+
+binner::binner_1d_t compute_charge_Q_direct_time::reduce_result_binned() const {
+  binner::binner_1d_t b;
+  b.xmin = 0.0;
+  b.xmax = time_;
+  b.nr_bins = nr_time_slices_;
+  std::vector<double> v(2 * size_t(nr_time_slices_));
+  std::vector<uint64_t> cnt(static_cast<size_t>(nr_time_slices_));
+  uint64_t dropped = 0;
+  ctx_.check(kd_get_time_bins(ctx_.get(), v.data(), cnt.data(), &dropped));
+  const double inv = n_points_ ? 1.0 / double(n_points_) : 0.0;
+  for (int i = 0; i < nr_time_slices_; i++) {
+    b.data.emplace_back(v[2 * i] * inv, v[2 * i + 1] * inv);
+    b.nr_values_added.push_back(long(cnt[i]));
+  }
+  b.nr_values_dropped = long(dropped);
+  return b;
+}
+
+std::pair<binner::sparse_binner_1d_t, double>
+compute_charge_Q_direct_time::evaluate_warped_integrand_time(std::vector<double> const &li_vec, int start_domain_nr,
+                                                             bool keep_u_hypercube) {
+  auto [v, dom] = evaluate_warped_integrand(li_vec, start_domain_nr, keep_u_hypercube);
+  auto ui = warper.map_reverse(li_vec, start_domain_nr, 0).first;
+  auto r = binned(v, int(dom), ui);
+  return {r.first, dom};
+}
 
 compute_current_J_direct::compute_current_J_direct(g0_model model, double time, int order, double cutoff_integrand)
    : integrator_direct(integrand_g_direct(g0_keldysh_contour_t(std::move(model)), gf_index_t(time, up, forward, 0, 1),

@@ -28,6 +28,29 @@ class device_ctx {
   std::shared_ptr<kd_ctx> ctx_;
 };
 
+namespace binner {
+
+/// One-axis binner (binner.hpp:147-352, binner_t<1>): sums and counts per bin
+/// of [xmin, xmax], nr_bins equal bins (xmax falls in the last one).
+struct binner_1d_t {
+  double xmin = 0.0, xmax = 1.0;
+  long nr_bins = 1;
+  std::vector<dcomplex> data;
+  std::vector<long> nr_values_added;
+  long nr_values_dropped = 0;
+  double get_bin_size() const { return (xmax - xmin) / double(nr_bins); }
+  std::vector<double> get_bin_coord() const;
+};
+
+/// Sparse binner (binner.hpp:59-140, sparse_binner_t<1>): (coordinate, value) pairs.
+struct sparse_binner_1d_t {
+  std::vector<std::pair<double, dcomplex>> data;
+  void accumulate(dcomplex value, double x) { data.emplace_back(x, value); }
+  double sum_moduli() const;
+};
+
+} // namespace binner
+
 namespace impurity_oneband {
 
 void check_window(const dcomplex *out, long n);
@@ -109,11 +132,12 @@ class integrator_direct {
   /// of the current chain (scrambled Sobol, seed shift rng_seed_shift, the
   /// flow warper excluded), then trains an n_layers coupling flow on the
   /// device for `steps` Adam steps of `batch` points, learning rate cosine-
-  /// decayed from lr to lr_final.  Returns the warper (append it last) and the
-  /// per-step loss.
+  /// decayed from lr to lr_final, toward the density |integrand|^weight_power.
+  /// Returns the warper (append it last) and the per-step loss.
   std::pair<warpers::warper_flow_t, std::vector<double>> train_flow_warper(int n_layers, long nr_samples, int steps,
                                                                           int batch, double lr, double lr_final,
-                                                                          int rng_seed_shift, uint64_t init_seed);
+                                                                          int rng_seed_shift, uint64_t init_seed,
+                                                                          double weight_power = 1.0);
 
  protected:
   int dimension = 1;
@@ -125,6 +149,43 @@ class integrator_direct {
   int precision_ = KD_PREC_FP64;
   uint64_t uploaded_stamp_ = 0;
   uint64_t uploaded_flow_stamp_ = 0;
+};
+
+/// Keldysh-sum integrand binned by the earliest vertex time
+/// (wick_direct_time.hpp:35-60, wick_direct_time.cpp:42-163).
+class integrand_g_direct_time {
+ public:
+  using result_t = binner::sparse_binner_1d_t;
+  integrand_g_direct_time(g0_keldysh_contour_t g0_, gf_index_t external_A_, gf_index_t external_B_,
+                          double cutoff_ = 0.)
+     : direct(std::move(g0_), external_A_, external_B_, cutoff_) {}
+  std::pair<result_t, int> operator()(std::vector<double> const &times, bool keep_u_hypercube = true) const;
+  integrand_g_direct const &base() const { return direct; }
+
+ private:
+  integrand_g_direct direct;
+};
+
+/// Time-resolved charge: the charge integrand binned by the earliest vertex
+/// time into nr_time_slices bins of [0, time] (compute_obs.hpp:85-101).
+class compute_charge_Q_direct_time : public integrator_direct {
+ public:
+  compute_charge_Q_direct_time(g0_model model, double time, int order, int nr_time_slices, double cutoff_integrand);
+  compute_charge_Q_direct_time(model_param_t params, double time, int order, int nr_time_slices,
+                               double cutoff_integrand)
+     : compute_charge_Q_direct_time(g0_model{g0_model_omega{params}, false}, time, order, nr_time_slices,
+                                    cutoff_integrand) {}
+  /// binner_t<1> of the estimate: bin sums divided by the points run
+  binner::binner_1d_t reduce_result_binned() const;
+  integrand_g_direct_time get_integrand_time() const { return integrand_time; }
+  std::pair<binner::sparse_binner_1d_t, double> evaluate_warped_integrand_time(std::vector<double> const &li_vec,
+                                                                              int start_domain_nr,
+                                                                              bool keep_u_hypercube = true);
+
+ private:
+  integrand_g_direct_time integrand_time;
+  double time_;
+  int nr_time_slices_;
 };
 
 /// Several integrators (e.g. orders 1..N with uneven point counts) run as one

@@ -71,7 +71,23 @@ struct KdParams {
   double *partials;
   // optional per-point initial Jacobian (output of the flow stage), or null
   const double *in_jac;
+  // time binning by the earliest vertex time (binner_t<1>, binner.hpp:147-352):
+  // nbins > 0 in MODE_RUN; bin_part [grid][nbins] per-block sums,
+  // bin_counts [nbins + 1] (values added per bin, then the dropped ones)
+  int nbins;
+  double bin_min, bin_max, bin_size;
+  double2 *bin_part;
+  unsigned long long *bin_counts;
 };
+
+// Bin of coordinate x (binner.hpp coord2index / in_bounds): -1 when outside
+// [xmin, xmax]; x == xmax falls in the last bin.
+__device__ __forceinline__ int bin_index(const KdParams &p, double x) {
+  if (!(x >= p.bin_min && x <= p.bin_max)) return -1;
+  if (x == p.bin_max) return p.nbins - 1;
+  const int i = (int)((x - p.bin_min) / p.bin_size);
+  return i < p.nbins ? i : p.nbins - 1;
+}
 
 // Compacted in-domain points between the two kernel stages (batch b of 64
 // points owns slots [64 b, 64 b + cnt[b])).

@@ -537,13 +537,14 @@ __global__ __launch_bounds__(256) void flow_pack_kernel(const float *__restrict_
   pk[idx] = f2bf(val);
 }
 
-// w[i] = |value_i| and per-block sums of w.
-__global__ __launch_bounds__(256) void flow_abs_kernel(const double *__restrict__ vals, int64_t n,
+// w[i] = |value_i|^power and per-block sums of w.
+__global__ __launch_bounds__(256) void flow_abs_kernel(const double *__restrict__ vals, int64_t n, double power,
                                                        double *__restrict__ w, double *__restrict__ part) {
   __shared__ double sh[256];
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const double a = hypot(vals[2 * i], vals[2 * i + 1]);
+    const double m = hypot(vals[2 * i], vals[2 * i + 1]);
+    const double a = power == 1.0 ? m : pow(m, power);
     w[i] = a;
     s += a;
   }
@@ -618,8 +619,9 @@ hipError_t launch_flow_pack(const float *w, uint16_t *pk, int order, int layers,
   return hipGetLastError();
 }
 
-hipError_t launch_flow_abs(const double *vals, int64_t n, double *w, double *part, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(flow_abs_kernel, dim3(grid), dim3(256), 0, st, vals, n, w, part);
+hipError_t launch_flow_abs(const double *vals, int64_t n, double power, double *w, double *part, int grid,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(flow_abs_kernel, dim3(grid), dim3(256), 0, st, vals, n, power, w, part);
   return hipGetLastError();
 }
 

@@ -210,6 +210,14 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
         for (int i = 0; i < N; i++) dom = dom && !(x[i] < 0.0);
       }
     }
+    if (p.nbins > 0 && valid) {
+      // binner accumulate(value or 0, min time): one count per point
+      double tmin = x[0];
+#pragma unroll
+      for (int i = 1; i < N; i++) tmin = fmin(tmin, x[i]);
+      const int bi = bin_index(p, tmin);
+      atomicAdd(p.bin_counts + (bi >= 0 ? bi : p.nbins), 1ull);
+    }
     const uint64_t mask = __ballot(dom);
     const int64_t b = pi >> 6;
     if (dom) {
@@ -242,8 +250,13 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
   constexpr int NU = N > 0 ? N : 1;
   __shared__ double2 W[PPW][D * D];
   __shared__ double Us[PPW][NU];
+  extern __shared__ double2 sbins[]; // p.nbins entries when time-binning (dynamic LDS)
 
   const int lane = threadIdx.x;
+  if (p.nbins > 0) {
+    for (int i = threadIdx.x; i < p.nbins; i += 64) sbins[i] = make_double2(0.0, 0.0);
+    __syncthreads();
+  }
   const int64_t nbatch = (p.n_points + 63) / 64;
   double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
 
@@ -342,6 +355,16 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
           acc_re += sre;
           acc_im += sim;
           acc_dom += 1.0;
+          if (p.nbins > 0) {
+            double tmin = Us[q][0];
+#pragma unroll
+            for (int i = 1; i < N; i++) tmin = fmin(tmin, Us[q][i]);
+            const int bi = bin_index(p, tmin);
+            if (bi >= 0) {
+              atomicAdd(&sbins[bi].x, sre);
+              atomicAdd(&sbins[bi].y, sim);
+            }
+          }
         } else {
           const int64_t idx = b * 64 + buf.idx[slot];
           p.out_vals[2 * idx] = sre;
@@ -369,7 +392,26 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
       o[2] = acc_dom;
       o[3] = acc_fb;
     }
+    if (p.nbins > 0) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < p.nbins; i += 64) p.bin_part[(int64_t)blockIdx.x * p.nbins + i] = sbins[i];
+    }
   }
+}
+
+// acc[b] += sum over blocks of part[blk][b] (fixed order), one thread per bin.
+__global__ __launch_bounds__(256) void reduce_bins_kernel(const double2 *__restrict__ part, int nblk, int nbins,
+                                                          double2 *__restrict__ acc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < nblk; k++) {
+    const double2 v = part[(int64_t)k * nbins + b];
+    re += v.x;
+    im += v.y;
+  }
+  acc[b].x += re;
+  acc[b].y += im;
 }
 
 // Deterministic sum of per-block partials into acc[0..3] (+=).
@@ -403,12 +445,13 @@ template <int N> static hipError_t launch_prep_n(const KdParams &p, const PrepBu
 template <int N, typename T>
 static hipError_t launch_lu_n(int lanes, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
   // lane-count variants only where they differ (2^N > lanes)
+  const size_t shm = p.nbins > 0 ? sizeof(double2) * (size_t)p.nbins : 0;
   if (lanes == 16 && N >= 5)
-    hipLaunchKernelGGL((integrand_kernel<N, T, 16>), dim3(grid), dim3(64), 0, st, p, buf);
+    hipLaunchKernelGGL((integrand_kernel<N, T, 16>), dim3(grid), dim3(64), shm, st, p, buf);
   else if (lanes == 32 && N >= 6)
-    hipLaunchKernelGGL((integrand_kernel<N, T, 32>), dim3(grid), dim3(64), 0, st, p, buf);
+    hipLaunchKernelGGL((integrand_kernel<N, T, 32>), dim3(grid), dim3(64), shm, st, p, buf);
   else
-    hipLaunchKernelGGL((integrand_kernel<N, T, 64>), dim3(grid), dim3(64), 0, st, p, buf);
+    hipLaunchKernelGGL((integrand_kernel<N, T, 64>), dim3(grid), dim3(64), shm, st, p, buf);
   return hipGetLastError();
 }
 
@@ -450,6 +493,11 @@ hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, con
 
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st) {
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, part, nblk, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_bins(const double2 *part, int nblk, int nbins, double2 *acc, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_bins_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st, part, nblk, nbins, acc);
   return hipGetLastError();
 }
 

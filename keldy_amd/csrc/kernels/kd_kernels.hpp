@@ -27,6 +27,8 @@ hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, con
                      hipStream_t st);
 // acc[0..3] += sum over blocks of part[4*blk + 0..3] (fixed order).
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st);
+// acc[b] += sum over blocks of part[blk][b], b < nbins (fixed order).
+hipError_t launch_reduce_bins(const double2 *part, int nblk, int nbins, double2 *acc, hipStream_t st);
 
 // ---- learned flow warper (kd_flow.hip) ----
 struct FlowMapIO {
@@ -56,6 +58,7 @@ hipError_t launch_flow_grad_reduce(const float *part, int nblk, int n_params, fl
 hipError_t launch_flow_adam(float *w, float *m, float *v, const float *g, int n, float lr, float b1, float b2,
                             float eps, int step, hipStream_t st);
 hipError_t launch_flow_pack(const float *w, uint16_t *pk, int order, int layers, hipStream_t st);
-hipError_t launch_flow_abs(const double *vals, int64_t n, double *w, double *part, int grid, hipStream_t st);
+hipError_t launch_flow_abs(const double *vals, int64_t n, double power, double *w, double *part, int grid,
+                           hipStream_t st);
 
 } // namespace kd

@@ -15,7 +15,12 @@ IntegrandGDirect = _m.IntegrandGDirect
 ComputeChargeQDirect = _m.ComputeChargeQDirect
 ComputeCurrentJDirect = _m.ComputeCurrentJDirect
 IntegratorDirect = _m.IntegratorDirect
+ComputeChargeQDirectTime = _m.ComputeChargeQDirectTime
+IntegrandGDirectTime = _m.IntegrandGDirectTime
+Binner1 = _m.Binner1
+SparseBinner1 = _m.SparseBinner1
 run_batch = _m.run_batch
 
 __all__ = ["GfIndexT", "G0ModelOmega", "G0Model", "G0KeldyshContourT", "IntegrandGDirect",
-           "ComputeChargeQDirect", "ComputeCurrentJDirect", "IntegratorDirect", "run_batch"]
+           "ComputeChargeQDirect", "ComputeCurrentJDirect", "IntegratorDirect", "ComputeChargeQDirectTime",
+           "IntegrandGDirectTime", "Binner1", "SparseBinner1", "run_batch"]

@@ -53,13 +53,15 @@ __all__ = [n for n in dir() if n[0].isupper() or n.startswith(("make_", "vi_", "
 
 
 def train_flow_warper(computer, n_layers=6, nr_samples=1 << 20, steps=2000, batch=1 << 14, lr=1e-3,
-                      lr_final=1e-5, rng_seed_shift=1000003, init_seed=0):
+                      lr_final=1e-5, rng_seed_shift=1000003, init_seed=0, weight_power=1.0):
     """Learned coupling-flow warper (no counterpart in keldy).
 
     Collects nr_samples scrambled-Sobol points (seed shift rng_seed_shift) of
     `computer`'s warped integrand with its current chain, then trains the flow
     on the device (hand-written forward/backward/Adam kernels) to make its
-    density proportional to |warped integrand|.  Returns (WarperFlowT,
-    per-step loss); append the warper last: computer.warper.emplace_back(w).
+    density proportional to |warped integrand|^weight_power.  Returns
+    (WarperFlowT, per-step loss); append the warper last:
+    computer.warper.emplace_back(w).
     """
-    return computer.train_flow_warper(n_layers, nr_samples, steps, batch, lr, lr_final, rng_seed_shift, init_seed)
+    return computer.train_flow_warper(n_layers, nr_samples, steps, batch, lr, lr_final, rng_seed_shift, init_seed,
+                                      weight_power)

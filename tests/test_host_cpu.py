@@ -277,3 +277,23 @@ def test_pilot_fit_from_samples():
     dens = 0.5 * (f1[:-1] + f1[1:])
     assert dens[1] > dens[10] > dens[18] > 0
     assert abs(np.log(dens[2] / dens[17]) - 3 * (t[17] - t[2])) < 0.3
+
+
+def test_dot_lead_g0_semicircle_kats():
+    """model_semicircle_test.cpp Semicirc_contour: dot-dot, dot-lead and
+    lead-dot g^<(t=1) at 1e-3 (values from ctint_keldysh), and the t <-> -t
+    symmetries g^{<,>}_{ab}(t) = -conj(g^{<,>}_{ba}(-t)) (1e-8)."""
+    kio = pytest.importorskip("keldy_amd.impurity_oneband")
+    p = dict(beta=10.0, bias_V=2.0, eps_d=-1.0, Gamma=1.5, time_max=100.0, nr_time_points_gf=1000, alpha=0.0,
+             half_bandwidth=2.0, bath_type="semicircle", ft_method="contour")
+    m = kio.G0Model(kio.G0ModelOmega(p), True)
+    tm = m.time_mesh
+    k, km = int(np.argmin(abs(tm - 1.0))), int(np.argmin(abs(tm + 1.0)))
+    assert tm[k] == pytest.approx(1.0, abs=1e-12)
+    L, G = m.g0_lesser, m.g0_greater
+    assert abs(L[0, 0, k] - (-0.23107349410223865 + 0.13213346415985422j)) < 1e-3
+    assert abs(L[0, 1, k] - (0.17154101374667907 + 0.13375649387573293j)) < 1e-3
+    assert abs(L[1, 0, k] - (-0.0066645142937036958 + 0.12215571972575601j)) < 1e-3
+    for T in (L, G):
+        assert abs(T[0, 0, k] + np.conj(T[0, 0, km])) < 1e-8
+        assert abs(T[0, 1, k] + np.conj(T[1, 0, km])) < 1e-8

@@ -182,7 +182,21 @@ def main():
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-alphas", type=float, nargs="+", default=[1.0, 0.5, 0.25],
                     help="weight powers of the flow candidates (density ~ |f|^alpha)")
+    ap.add_argument("--algorithm", default=None, choices=["config", "tree"],
+                    help="Keldysh-sum algorithm (default: the library's)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters), then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_launch": n}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(points would shard by contiguous Sobol index ranges with one 32-byte all-reduce).",
@@ -195,6 +209,8 @@ def main():
     t0 = time.perf_counter()
     with _stdout_to_stderr():
         model, comp, proj = plain_computer(args.order, t_max, args.pilot_points, args.precision)
+    if args.algorithm:
+        comp.set_algorithm(args.algorithm)
     setup_s = time.perf_counter() - t0
 
     def measure(label):

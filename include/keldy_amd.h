@@ -44,6 +44,16 @@ extern "C" {
 #define KD_PREC_FP32_LU 1
 #define KD_PREC_FP32_LU_ALL 2
 
+/* Keldysh-sum algorithm.  KD_ALGO_CONFIG (the default): one bordered LU per
+ * configuration sigma (2^n of them, sharing the staged Wick matrix).
+ * KD_ALGO_TREE (orders >= 6; lower orders use KD_ALGO_CONFIG): the
+ * configurations share the Schur complements of their common leading vertex
+ * choices -- one table per node of the binary tree of choices -- so order 8
+ * needs ~5x fewer flops, at lower issue efficiency; kd_autotune picks between
+ * them.  Same arithmetic up to reassociation (FP64 results agree to ~1e-13). */
+#define KD_ALGO_CONFIG 0
+#define KD_ALGO_TREE 1
+
 typedef struct kd_ctx kd_ctx;
 
 /* gf_index_t (model.hpp:61-76) */
@@ -172,14 +182,18 @@ int kd_get_time_bins(kd_ctx *ctx, double *values, uint64_t *counts, uint64_t *dr
  * No counterpart in the reference (its integrator runs one order per object).
  * kd_autotune times the Keldysh-sum kernel on sample_points Sobol points of
  * the ctx's chain (from the current generator state, which does not advance;
- * the accumulator is untouched) for every lanes-per-point x waves-per-CU
- * candidate and keeps the fastest for this ctx.  kd_run_batch runs several
+ * the accumulator is untouched) for every candidate -- the tree algorithm and
+ * the per-configuration kernel at 64, 32 or 16 lanes per point, each at 8, 16
+ * or 32 waves per CU -- and keeps the fastest for this ctx (lanes = 0 when the
+ * tree wins).  kd_run_batch runs several
  * ctxs (e.g. orders 1..10, uneven point counts) as ONE hipGraph launch: one
  * graph branch per ctx (its persistent kernels on its own stream), forked from
  * and joined to ctxs[0]'s stream; each ctx advances and accumulates as by
  * kd_run.  ms = wall time of the graph launch. */
 int kd_autotune(kd_ctx *ctx, int64_t sample_points, int32_t *lanes, int32_t *waves_per_cu, double *ms_per_mpoint);
 int kd_set_launch_shape(kd_ctx *ctx, int32_t lanes, int32_t waves_per_cu);
+int kd_set_algorithm(kd_ctx *ctx, int32_t algo);
+int kd_get_algorithm(const kd_ctx *ctx, int32_t *algo);
 int kd_get_launch_shape(const kd_ctx *ctx, int32_t *lanes, int32_t *waves_per_cu);
 int kd_run_batch(kd_ctx *const *ctxs, const uint64_t *n_steps, int32_t count, kd_accum *out, double *ms);
 

@@ -24,6 +24,7 @@ HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17",
 
 HIP_SOURCES = [
     "kernels/kd_integrand.hip",
+    "kernels/kd_tree.hip",
     "kernels/kd_flow.hip",
     "capi/kd_capi.cpp",
 ]

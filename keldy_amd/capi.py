@@ -20,6 +20,7 @@ KD_PREC_FP64, KD_PREC_FP32_LU, KD_PREC_FP32_LU_ALL = 0, 1, 2
 KD_MAP_UNIFORM, KD_MAP_EXPONENTIAL, KD_MAP_INVERSE, KD_MAP_INVERSE_SQ = 0, 1, 2, 3
 KD_MAP_INVERSE_POW, KD_MAP_NEAREST, KD_MAP_HYBRID = 4, 5, 6
 KD_WARP_IDENTITY, KD_WARP_PLASMA, KD_WARP_PRODUCT, KD_WARP_FLOW = 0, 1, 2, 3
+KD_ALGO_CONFIG, KD_ALGO_TREE = 0, 1
 
 
 class GfIndex(C.Structure):
@@ -52,7 +53,7 @@ SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_se
            "kd_flops_per_point", "kd_flow_n_params", "kd_flow_init", "kd_flow_set_params",
            "kd_flow_get_params", "kd_flow_map", "kd_flow_loss_grad", "kd_flow_collect", "kd_flow_train",
            "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch",
-           "kd_set_time_bins", "kd_get_time_bins"]
+           "kd_set_time_bins", "kd_get_time_bins", "kd_set_algorithm", "kd_get_algorithm"]
 
 _lib = None
 
@@ -101,6 +102,8 @@ def lib():
         L.kd_get_launch_shape.argtypes = [vp, ip, ip]
         L.kd_run_batch.argtypes = [C.POINTER(vp), C.POINTER(C.c_uint64), C.c_int32, C.POINTER(Accum), dp]
         L.kd_set_time_bins.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
+        L.kd_set_algorithm.argtypes = [vp, C.c_int32]
+        L.kd_get_algorithm.argtypes = [vp, ip]
         L.kd_get_time_bins.argtypes = [vp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
@@ -246,6 +249,14 @@ class Context:
 
     def set_launch_shape(self, lanes, waves_per_cu):
         self._check(lib().kd_set_launch_shape(self.h, lanes, waves_per_cu))
+
+    def set_algorithm(self, algo):
+        self._check(lib().kd_set_algorithm(self.h, algo))
+
+    def algorithm(self):
+        a = C.c_int32()
+        self._check(lib().kd_get_algorithm(self.h, C.byref(a)))
+        return a.value
 
     def launch_shape(self):
         l, w = C.c_int32(), C.c_int32()

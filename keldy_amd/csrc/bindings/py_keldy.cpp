@@ -147,6 +147,19 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               throw keldy_error("precision must be 'fp64', 'fp32' (FP64 LU kept from order 10) or 'fp32_all'");
           },
           py::arg("precision"))
+      .def(
+          "set_algorithm",
+          [](T &s, std::string const &a) {
+            int v;
+            if (a == "tree")
+              v = KD_ALGO_TREE;
+            else if (a == "config")
+              v = KD_ALGO_CONFIG;
+            else
+              throw keldy_error("algorithm must be 'tree' or 'config'");
+            s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
+          },
+          py::arg("algorithm"), "Keldysh-sum algorithm: 'config' (default) or 'tree' (orders >= 6)")
       .def("last_timing", &T::last_timing)
       .def("flops_per_point", &T::flops_per_point)
       .def("autotune", &T::autotune, py::arg("sample_points") = 1 << 18, py::call_guard<py::gil_scoped_release>(),

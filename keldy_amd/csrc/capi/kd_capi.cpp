@@ -76,6 +76,7 @@ struct kd_ctx {
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
   // point and waves per CU of the persistent grid
   int lu_lanes_max = 64, lu_waves_per_cu = 16;
+  int algo = KD_ALGO_CONFIG; // Keldysh-sum algorithm (the tree applies from order 6 on)
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
   // time binning (kd_set_time_bins)
@@ -113,11 +114,13 @@ int ensure_stage(kd_ctx *c, size_t bytes) {
   return KD_OK;
 }
 
+bool use_tree(const kd_ctx *c) { return c->algo == KD_ALGO_TREE && c->order >= 6; }
+
 // LU-kernel grid: one workgroup per group of points (64 / lanes points per
 // group), capped at lu_waves_per_cu waves per CU -- the kernel is persistent
 // over the rest.
 int grid_for(const kd_ctx *c, int64_t n_points) {
-  const int64_t per_batch = kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
+  const int64_t per_batch = use_tree(c) ? 64 : kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
   int64_t nb = (n_points + 63) / 64 * per_batch;
   int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * c->lu_waves_per_cu;
   if (nb > cap) nb = cap;
@@ -314,7 +317,10 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
     KD_HIP(c, kd::launch_prep(N, p, buf, grid_prep, c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
-    KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, p, buf, grid, c->stream));
+    if (use_tree(c))
+      KD_HIP(c, kd::launch_tree(N, c->precision, p, buf, grid, c->stream));
+    else
+      KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, p, buf, grid, c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
     launches += 2;
     if (p.mode == kd::MODE_RUN && !c->no_reduce) {
@@ -706,8 +712,28 @@ int kd_last_timing(const kd_ctx *c, double *total_ms, double *lu_ms, double *pre
   return KD_OK;
 }
 
+// Executed flops per point of the tree algorithm (kd_tree.hip): a child table
+// of size Dc costs 8 Dc^2 (entries, one complex multiply-add each) + 13 Dc
+// (scaled pivot row: reciprocal 7 + multiply 6); the bottom 4x4 bordered LU
+// per configuration 14 complex multiply-adds (112) + 3 reciprocals (21) + 6
+// row scalings (36) + 3 determinant products (18), then path x bottom
+// determinant (12), det^2 * s (12) and the signed FP64 sum (2).
+static double tree_flops(int N) {
+  auto node = [](double dc) { return 8.0 * dc * dc + 13.0 * dc; };
+  const int A = N - 6;
+  double f = 0.0;
+  for (int k = 1; k <= A; k++) f += std::ldexp(node(2.0 * (N - k) + 1), k);
+  f += std::ldexp(2 * node(11) + 4 * node(9) + 8 * node(7), A);
+  f += std::ldexp(112.0 + 21 + 36 + 18 + 12 + 12 + 2, N);
+  return f;
+}
+
 int kd_flops_per_point(const kd_ctx *c, double *flops) {
   if (!c || !flops) return KD_ERR_ARG;
+  if (use_tree(c)) {
+    *flops = tree_flops(c->order);
+    return KD_OK;
+  }
   const double N = c->order < 0 ? 0 : c->order;
   // per configuration: row-wise bordered LU (complex MAC = 8, complex mul = 6,
   // complex reciprocal = 7), det^2 * s (12) and the signed FP64 sum (2)
@@ -1000,18 +1026,22 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
     return fail(c, KD_ERR_RANGE, "More sample points requested then generator can hold.");
   KD_HIP(c, hipSetDevice(c->device));
   const int lane_opts[3] = {64, 32, 16}, wave_opts[3] = {8, 16, 32};
-  const int old_l = c->lu_lanes_max, old_w = c->lu_waves_per_cu;
-  int best_l = old_l, best_w = old_w;
+  const int old_l = c->lu_lanes_max, old_w = c->lu_waves_per_cu, old_a = c->algo;
+  int best_l = old_l, best_w = old_w, best_a = old_a;
   float best = 1e30f;
   kd::KdParams p = base_params(c);
   p.mode = kd::MODE_RUN;
   p.n_points = sample_points;
   p.nbins = 0; // no bin counts from tuning launches
   c->no_reduce = true;
-  for (int li = 0; li < 3 && !rc; li++) {
-    const int L = lane_opts[li];
+  // candidates: the tree (orders >= 6; its lanes are fixed) and the
+  // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16
+  // or 32 waves per CU
+  for (int li = (c->order >= 6 ? -1 : 0); li < 3 && !rc; li++) {
+    const int L = li < 0 ? 64 : lane_opts[li];
     if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
     for (int wi = 0; wi < 3 && !rc; wi++) {
+      c->algo = li < 0 ? KD_ALGO_TREE : KD_ALGO_CONFIG;
       c->lu_lanes_max = L;
       c->lu_waves_per_cu = wave_opts[wi];
       float t = 1e30f;
@@ -1023,6 +1053,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
         best = t;
         best_l = L;
         best_w = wave_opts[wi];
+        best_a = c->algo;
       }
     }
   }
@@ -1030,12 +1061,14 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   if (rc) {
     c->lu_lanes_max = old_l;
     c->lu_waves_per_cu = old_w;
+    c->algo = old_a;
     return rc;
   }
   c->lu_lanes_max = best_l;
   c->lu_waves_per_cu = best_w;
+  c->algo = best_a;
   c->tune_ms_per_mpt = best * 1048576.f / (float)sample_points;
-  if (lanes) *lanes = kd::lu_lanes(c->order, best_l);
+  if (lanes) *lanes = use_tree(c) ? 0 : kd::lu_lanes(c->order, best_l);
   if (waves_per_cu) *waves_per_cu = best_w;
   if (ms_per_mpoint) *ms_per_mpoint = c->tune_ms_per_mpt;
   return KD_OK;
@@ -1047,6 +1080,19 @@ int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
   if (waves_per_cu < 1 || waves_per_cu > 64) return fail(c, KD_ERR_ARG, "waves_per_cu must be in [1, 64]");
   c->lu_lanes_max = lanes;
   c->lu_waves_per_cu = waves_per_cu;
+  return KD_OK;
+}
+
+int kd_set_algorithm(kd_ctx *c, int32_t algo) {
+  if (!c) return KD_ERR_ARG;
+  if (algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE) return fail(c, KD_ERR_ARG, "unknown Keldysh-sum algorithm");
+  c->algo = algo;
+  return KD_OK;
+}
+
+int kd_get_algorithm(const kd_ctx *c, int32_t *algo) {
+  if (!c || !algo) return KD_ERR_ARG;
+  *algo = use_tree(c) ? KD_ALGO_TREE : KD_ALGO_CONFIG;
   return KD_OK;
 }
 

@@ -20,19 +20,10 @@
 
 #include "kd_device.hpp"
 #include "kd_kernels.hpp"
+#include "kd_lu.hpp"
 
 namespace kd {
 
-template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
-
-template <typename T> struct PivotTol;
-template <> struct PivotTol<double> { static constexpr double v = 1.0 / 4096.0; };
-template <> struct PivotTol<float> { static constexpr float v = 1.0f / 256.0f; };
 
 // det(A)^2 * s for one Keldysh configuration, unpivoted row-wise LU in T.
 template <int N, typename T>
@@ -261,12 +252,16 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
   double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
 
   // Work item = one group of PPW compacted points of a 64-point batch, so that
-  // a launch of few points still spreads over every CU.
+  // a launch of few points still spreads over every CU.  Items are numbered
+  // group-major (all batches' first groups, then their second groups, ...):
+  // the compacted points sit at the front of each batch, so the items that
+  // hold work come first and are dealt evenly over the workgroups.
   constexpr int GPB = 64 / PPW; // groups per batch
   const int64_t nitems = nbatch * GPB;
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const int64_t b = it / GPB;
-    const int g0i = (int)(it - b * GPB) * PPW;
+    const int64_t grp = it / nbatch;
+    const int64_t b = it - grp * nbatch;
+    const int g0i = (int)grp * PPW;
     const int cnt = buf.cnt[b];
     if (g0i >= cnt) continue;
     {

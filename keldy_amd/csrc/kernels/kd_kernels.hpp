@@ -25,6 +25,9 @@ inline int lu_lanes(int order, int lanes_max) {
 // threads; lanes = the maximum lanes per point (64, 32 or 16).
 hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, const PrepBuf &buf, int grid,
                      hipStream_t st);
+// integrand_tree_kernel<order, T> (orders 6..12): the Keldysh sum by the
+// Schur-complement tree, one point per 64-thread workgroup item.
+hipError_t launch_tree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // acc[0..3] += sum over blocks of part[4*blk + 0..3] (fixed order).
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st);
 // acc[b] += sum over blocks of part[blk][b], b < nbins (fixed order).

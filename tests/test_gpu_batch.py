@@ -64,12 +64,14 @@ def test_run_batch_errors(flat_g0):
 def test_autotune(flat_g0, order):
     ctx = _ctx(flat_g0, order, 7)
     lanes, wpc, ms = ctx.autotune(1 << 16)
-    assert lanes in (16, 32, 64) or lanes == 1 << order
+    # lanes == 0: the Schur-tree algorithm won (orders >= 6)
+    assert lanes in (16, 32, 64) or lanes == 1 << order or (lanes == 0 and order >= 6)
     assert wpc in (8, 16, 32) and ms > 0
     assert ctx.sobol_state() == 0 and ctx.accum().n_points == 0 and ctx.accum().re == 0.0
     a = ctx.run(1 << 16)
     ref = _ctx(flat_g0, order, 7)
     ref.set_launch_shape(64, 16)
+    ref.set_algorithm(capi.KD_ALGO_CONFIG)
     r = ref.run(1 << 16)
     assert _rel(a, r) <= 1e-12
 
@@ -88,3 +90,28 @@ def test_fp32_error_per_order(flat_g0, order):
         assert _rel(rall, r64) <= 1e-2
     else:
         assert rel <= 1e-4
+
+
+@pytest.mark.parametrize("order", [6, 7, 8, 9, 10])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tree_matches_per_configuration(flat_g0, order, prec):
+    """The Schur-tree algorithm and the per-configuration LU on the same points:
+    FP64 to 1e-12 normwise (reassociation of a cancelling 2^n-term sum), FP32
+    to 1e-4."""
+    t = 10.0
+    rng = np.random.default_rng(order)
+    u = rng.uniform(0.0, t, (256, order))
+    u[:128] = -np.sort(-u[:128], axis=1)
+    a = U.charge_ctx(flat_g0, order, t, prec)
+    b = U.charge_ctx(flat_g0, order, t, prec)
+    a.set_algorithm(capi.KD_ALGO_TREE)
+    b.set_algorithm(capi.KD_ALGO_CONFIG)
+    assert a.algorithm() == capi.KD_ALGO_TREE and b.algorithm() == capi.KD_ALGO_CONFIG
+    va, da = a.eval_batch(u)
+    vb, db = b.eval_batch(u)
+    assert np.array_equal(da, db)
+    tol = 1e-12 if prec == 0 else 1e-4
+    assert np.max(np.abs(va - vb)) <= tol * np.max(np.abs(vb))
+    fa, fb = a.flops_per_point(), b.flops_per_point()
+    assert fa < fb
+    print(f"order {order}: tree {fa:.0f} flops/point vs per-configuration {fb:.0f} ({fb / fa:.1f}x)")

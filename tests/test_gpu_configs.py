@@ -1,0 +1,106 @@
+"""The parity-test workloads at (or near) their full sizes, through the host
+API, against the CPU oracle where the oracle finishes in seconds and by
+size-independent checks beyond.
+
+C2  order 6, t = 10, flat band with eps_d = 0.5, V = 0.2, beta = 50, g0 by
+    contour integration; chain [plasma_uv(10), inverse(10, 1), pilot-fitted
+    product map (2^14 pilot points, 500 bins, sigma 0.1)]; 2^22 scrambled
+    Sobol points.  FP64 against the oracle on the first 2^15 points (1e-10
+    relative); FP32-LU against FP64 on all 2^22 points (error stated, <= 1e-4).
+C3  orders 1..10 with floor(2^24 / n^2) points each, [plasma_uv(10),
+    inverse(10, 1)], FP32 LU (FP64 from order 10), each order's launch shape
+    autotuned, all orders in one hipGraph launch; per order equal to the same
+    integrator run on its own (1e-12 relative: its own tuning may pick another
+    launch shape, which reassociates the sums).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _c2_params(time_max):
+    return dict(beta=50.0, bias_V=0.2, eps_d=0.5, Gamma=1.0, alpha=0.0, half_bandwidth=10.0,
+                bath_type="flatband", time_max=time_max, nr_time_points_gf=int(time_max * 1000),
+                ft_method="contour")
+
+
+def test_c2_order6_contour_fitted():
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    order, t = 6, 10.0
+    model = kio.G0Model(kio.G0ModelOmega(_c2_params(t)), False)
+    wide = kio.G0Model(kio.G0ModelOmega(_c2_params(order * t)), False)
+    pilot = kio.ComputeChargeQDirect(wide, t, order, 0.0)
+    pilot.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    pilot.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+    proj = kw.make_warper_from_proj(pilot, order, 500, 1 << 14, 0.1, keep_u_hypercube=False)
+
+    def comp(prec):
+        c = kio.ComputeChargeQDirect(model, t, order, 0.0)
+        c.set_precision(prec)
+        c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+        c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+        c.warper.emplace_back(proj)
+        c.reset_rng("sobol", 0, True, True, 0)
+        return c
+
+    # FP64 against the oracle on the first 2^15 points
+    c = comp("fp64")
+    m = 1 << 15
+    c.run(m)
+    g0 = O.G0Table(model.g0_lesser[0, 0], model.g0_greater[0, 0], model.time_max)
+    maps = []
+    for a in range(order):
+        tk, f1, F, cut = proj.get_1d_warper(a).knots()
+        maps.append(O.Map1d("nearest", 1.0, t=np.array(tk), f=np.array(f1), F=np.array(F), n_inv=cut + 1))
+    train = O.Train([O.plasma(t), O.product(O.Map1d("inverse", t, w=1.0)), O.product(maps)])
+    s, npo, ndom = O.run(O.charge_integrand(g0, t), train, O.Sobol(order, 0, True, True, 0), m, threads=8)
+    assert c.reduce_nr_points_in_domain() == ndom
+    assert abs(c.reduce_result() * m - s) <= 1e-10 * abs(s)
+    # the full 2^22 points, FP64 and FP32-LU
+    n = 1 << 22
+    c64, c32 = comp("fp64"), comp("fp32")
+    c64.run(n)
+    c32.run(n)
+    r64, r32 = c64.reduce_result(), c32.reduce_result()
+    rel = abs(r32 - r64) / abs(r64)
+    q = (-1j * (1j ** order) * r64).real
+    print(f"C2: Q_6 = {q:.8e}; FP32-LU relative error over 2^22 points {rel:.3e}; "
+          f"fallback configurations {c64.reduce_nr_fallback()} / {c32.reduce_nr_fallback()}")
+    assert rel <= 1e-4
+
+
+def test_c3_orders_1_to_10_one_launch():
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t = 10.0
+    model = kio.G0Model(kio.G0ModelOmega(dict(beta=-1.0, bias_V=0.0, eps_d=0.0, Gamma=1.0, time_max=t,
+                                               nr_time_points_gf=10000, alpha=0.0, bath_type="flatband",
+                                               ft_method="analytic")), False)
+    orders = list(range(1, 11))
+    counts = [(1 << 24) // (n * n) for n in orders]
+
+    def make(n):
+        c = kio.ComputeChargeQDirect(model, t, n, 0.0)
+        c.set_precision("fp32")
+        c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+        c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+        c.reset_rng("sobol", 0, True, True, 11)
+        return c
+
+    batch = [make(n) for n in orders]
+    shapes = [c.autotune(1 << 16) for c in batch]
+    ms = kio.run_batch(batch, counts)
+    for n, c, (lanes, wpc, mpt) in zip(orders, batch, shapes):
+        print(f"C3 order {n}: {counts[n - 1]} points, tuned {'tree' if lanes == 0 else f'{int(lanes)} lanes/point'}"
+              f", {int(wpc)} waves/CU, LU {mpt:.3f} ms per 2^20 points")
+    print(f"C3: all orders in one graph launch: {ms:.1f} ms")
+    for n in (1, 5, 10):
+        ref = make(n)
+        ref.autotune(1 << 16)
+        ref.run(counts[n - 1])
+        a, b = batch[n - 1].reduce_result(), ref.reduce_result()
+        assert abs(a - b) <= 1e-12 * abs(b), f"order {n}"

@@ -44,6 +44,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 TARGET_REL = 1e-3
 PILOT_M = 1 << 16  # points per pilot replica when judging flow candidates
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "integrand_kernel_n8_fp64_pmc.md")
 
 
 class _stdout_to_stderr:
@@ -59,6 +60,31 @@ class _stdout_to_stderr:
         sys.stdout.flush()
         os.dup2(self.saved, 1)
         os.close(self.saved)
+
+
+def pmc_traffic(kernel, points_per_launch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the
+    full-size launches), each scaled by the factor that makes the same
+    counter read a plain device copy of known size correctly.  None when the
+    summary does not cover this kernel at this launch size."""
+    try:
+        text = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        f = line.split()
+        if len(f) < 2 or f[0] != "pmc-traffic" or f[1] != kernel:
+            continue
+        kv = dict(x.split("=", 1) for x in f[2:])
+        if int(kv["points_per_launch"]) != points_per_launch:
+            continue
+        fetch = float(kv["fetch_kib"]) * 1024 * float(kv["fetch_factor"])
+        write = float(kv["write_kib"]) * 1024 * float(kv["write_factor"])
+        return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+                "fetch_factor": float(kv["fetch_factor"]), "write_factor": float(kv["write_factor"]),
+                "source": os.path.relpath(PMC_SUMMARY, ROOT)}
+    return None
 
 
 def model_params():
@@ -290,6 +316,10 @@ def main():
     achieved = flops / lu_s / 1e12
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     launches_per_step = max(1, -(-args.points // (1 << 21)))
+    algo = comp.get_algorithm()
+    kname = (f"integrand_{'tree_' if algo == 'tree' else ''}kernel"
+             f"<{args.order},{'double' if args.precision == 'fp64' else 'float'}>")
+    traffic = pmc_traffic(kname, min(args.points, 1 << 21))
 
     estimator = {"plain": summary(plain), "setup_s": setup_s}
     if selection:
@@ -327,9 +357,10 @@ def main():
                    "pilot_points": args.pilot_points},
         "estimator": estimator,
         "roofline": {"bound": "valu-fp64" if args.precision == "fp64" else "valu-fp32",
-                     "kernel": f"integrand_kernel<{args.order},{'double' if args.precision == 'fp64' else 'float'}>",
+                     "kernel": kname,
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "flops_per_point": fpp,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_detail": traffic, "flops_per_point": fpp,
                      "lu_ms_per_launch": float(head["lu_ms"].mean()) / launches_per_step},
     }
     if not args.no_cpu:

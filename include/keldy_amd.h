@@ -166,6 +166,16 @@ int kd_last_timing(const kd_ctx *ctx, double *total_ms, double *lu_ms, double *p
  * (the algorithm's own count, used for roofline figures). */
 int kd_flops_per_point(const kd_ctx *ctx, double *flops);
 
+/* ---- pilot projection on the device ---------------------------------------
+ * make_warper_from_proj's data gathering (warpers/make_warper_from_proj.hpp:80-114):
+ * the warped integrand with the ctx's chain at the first n_samples points of
+ * the pilot generator (Csr, shfr: normally the unshifted sobol(order, 0)),
+ * then per axis a and bin b of [0, 1] (num_bins bins) the sum of |value| over
+ * the samples whose coordinate a falls in b, and their count:
+ * sums, counts [order][num_bins].  The ctx's own generator is restored. */
+int kd_pilot_histograms(kd_ctx *ctx, const uint32_t *Csr, const uint32_t *shfr, int64_t n_samples, int32_t num_bins,
+                        int32_t keep_u_hypercube, double *sums, uint64_t *counts);
+
 /* ---- time binning --------------------------------------------------------
  * Replaces integrand_g_direct_time (impurity_oneband/wick_direct_time.cpp:42-163)
  * with the integrator's binner_t<1> result (binner.hpp:147-352): in kd_run,

@@ -53,7 +53,8 @@ SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_se
            "kd_flops_per_point", "kd_flow_n_params", "kd_flow_init", "kd_flow_set_params",
            "kd_flow_get_params", "kd_flow_map", "kd_flow_loss_grad", "kd_flow_collect", "kd_flow_train",
            "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch",
-           "kd_set_time_bins", "kd_get_time_bins", "kd_set_algorithm", "kd_get_algorithm"]
+           "kd_set_time_bins", "kd_get_time_bins", "kd_set_algorithm", "kd_get_algorithm",
+           "kd_pilot_histograms"]
 
 _lib = None
 
@@ -103,6 +104,7 @@ def lib():
         L.kd_run_batch.argtypes = [C.POINTER(vp), C.POINTER(C.c_uint64), C.c_int32, C.POINTER(Accum), dp]
         L.kd_set_time_bins.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
         L.kd_set_algorithm.argtypes = [vp, C.c_int32]
+        L.kd_pilot_histograms.argtypes = [vp, up, up, C.c_int64, C.c_int32, C.c_int32, dp, C.POINTER(C.c_uint64)]
         L.kd_get_algorithm.argtypes = [vp, ip]
         L.kd_get_time_bins.argtypes = [vp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         _lib = L

@@ -160,8 +160,19 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
           py::arg("algorithm"), "Keldysh-sum algorithm: 'config' (default) or 'tree' (orders >= 6)")
+      .def("get_algorithm",
+           [](T &s) {
+             int32_t v = KD_ALGO_CONFIG;
+             s.ctx().check(kd_get_algorithm(s.ctx().get(), &v));
+             return std::string(v == KD_ALGO_TREE ? "tree" : "config");
+           })
       .def("last_timing", &T::last_timing)
       .def("flops_per_point", &T::flops_per_point)
+      .def("get_integrand_dimension", &T::get_dimension)
+      .def("make_warper_from_proj_device", &T::make_warper_from_proj_device, py::arg("num_bins"),
+           py::arg("nr_samples"), py::arg("sigma"), py::arg("keep_u_hypercube") = false,
+           py::call_guard<py::gil_scoped_release>(),
+           "make_warper_from_proj with the pilot evaluations and the per-axis histograms on the device")
       .def("autotune", &T::autotune, py::arg("sample_points") = 1 << 18, py::call_guard<py::gil_scoped_release>(),
            "tune the Keldysh-sum kernel's launch shape: [lanes per point, waves per CU, LU ms per 2^20 points]")
       .def(

@@ -1229,4 +1229,74 @@ int kd_get_time_bins(kd_ctx *c, double *values, uint64_t *counts, uint64_t *drop
   return KD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// pilot projection on the device (make_warper_from_proj.hpp:80-114)
+
+int kd_pilot_histograms(kd_ctx *c, const uint32_t *Csr, const uint32_t *shfr, int64_t n_samples, int32_t num_bins,
+                        int32_t keep_u_hypercube, double *sums, uint64_t *counts) {
+  if (!c) return KD_ERR_ARG;
+  int rc = need_integrand(c);
+  if (rc) return rc;
+  if (!Csr || !shfr || !sums || !counts) return fail(c, KD_ERR_ARG, "null buffer");
+  if (c->order < 1) return fail(c, KD_ERR_ARG, "Dimension of the integration space must be > 0.");
+  if (num_bins < 2 || num_bins > 65536) return fail(c, KD_ERR_ARG, "num_bins must be in [2, 65536]");
+  if (n_samples < 1 || n_samples > (int64_t(1) << 24)) return fail(c, KD_ERR_ARG, "pilot sample count must be in [1, 2^24]");
+  const int N = c->order;
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  // the pilot generator replaces the ctx's for this call; its tables are restored after
+  std::vector<uint32_t> sCsr, sShfr;
+  const bool had = c->has_sobol;
+  const int sdim = c->sobol_dim;
+  const uint32_t sk = c->k;
+  if (had) {
+    sCsr.resize(31 * (size_t)sdim);
+    sShfr.resize(sdim);
+    KD_HIP(c, hipMemcpy(sCsr.data(), c->d_Csr, sizeof(uint32_t) * sCsr.size(), hipMemcpyDeviceToHost));
+    KD_HIP(c, hipMemcpy(sShfr.data(), c->d_shfr, sizeof(uint32_t) * sShfr.size(), hipMemcpyDeviceToHost));
+  }
+  if ((rc = kd_set_sobol(c, N, Csr, shfr, 0))) return rc;
+  double *dl = nullptr, *dv = nullptr, *ds = nullptr;
+  unsigned long long *dc = nullptr;
+  const size_t nb = (size_t)N * num_bins;
+  if (hipMalloc(&dl, sizeof(double) * (size_t)n_samples * N) != hipSuccess ||
+      hipMalloc(&dv, sizeof(double) * 2 * (size_t)n_samples) != hipSuccess ||
+      hipMalloc(&ds, sizeof(double) * nb) != hipSuccess || hipMalloc(&dc, sizeof(unsigned long long) * nb) != hipSuccess)
+    rc = fail(c, KD_ERR_HIP, "hipMalloc failed");
+  if (!rc) {
+    // warped integrand at the pilot points (the flow, if last, is part of the chain)
+    kd::KdParams p = base_params(c);
+    p.mode = kd::MODE_SAMPLE;
+    p.keep_u = keep_u_hypercube ? 1 : 0;
+    p.n_points = n_samples;
+    p.out_l = dl;
+    p.out_vals = dv;
+    rc = launch_timed(c, p, 0);
+    if (!rc) rc = finish_timing(c);
+  }
+  if (!rc && kd::launch_pilot_hist(dl, dv, n_samples, N, num_bins, ds, dc, c->stream) != hipSuccess)
+    rc = fail(c, KD_ERR_HIP, "pilot histogram launch failed");
+  if (!rc && (hipMemcpyAsync(sums, ds, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipMemcpyAsync(counts, dc, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = fail(c, KD_ERR_HIP, "copy failed");
+  for (void *q : {(void *)dl, (void *)dv, (void *)ds, (void *)dc})
+    if (q) (void)hipFree(q);
+  // restore the ctx's generator
+  if (had) {
+    const int r2 = kd_set_sobol(c, sdim, sCsr.data(), sShfr.data(), sk);
+    if (!rc) rc = r2;
+  } else {
+    dfree(c->d_Csr);
+    dfree(c->d_shfr);
+    c->has_sobol = false;
+    c->k = 0;
+  }
+  if (!rc) {
+    for (size_t i = 0; i < nb; i++)
+      if (!std::isfinite(sums[i])) return fail(c, KD_ERR_ARG, "g0 evaluated outside its time window [-time_max, time_max]");
+  }
+  return rc;
+}
+
 } // extern "C"

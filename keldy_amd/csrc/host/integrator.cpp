@@ -234,6 +234,26 @@ void integrator_direct::run(long nr_steps) {
   finish_device_run(nr_steps);
 }
 
+warpers::warper_product_1d_interp_nearest_t
+integrator_direct::make_warper_from_proj_device(int num_bins, long nr_samples, double sigma, bool keep_u_hypercube) {
+  sync_warper();
+  const sobol g(dimension, 0);
+  const size_t nb = size_t(dimension) * size_t(std::max(num_bins, 0));
+  std::vector<double> sums(nb);
+  std::vector<uint64_t> cnt(nb);
+  ctx_.check(kd_pilot_histograms(ctx_.get(), g.csr().data(), g.shfr().data(), int64_t(nr_samples), num_bins,
+                                 keep_u_hypercube ? 1 : 0, sums.data(), cnt.data()));
+  rng_dirty_ = true; // the device generator was restored; re-upload ours before the next run anyway
+  std::vector<std::vector<double>> data(static_cast<size_t>(dimension));
+  std::vector<std::vector<long>> count(static_cast<size_t>(dimension));
+  for (int a = 0; a < dimension; a++) {
+    data[a].assign(sums.begin() + size_t(a) * num_bins, sums.begin() + size_t(a + 1) * num_bins);
+    count[a].resize(size_t(num_bins));
+    for (int b = 0; b < num_bins; b++) count[a][b] = long(cnt[size_t(a) * num_bins + b]);
+  }
+  return warpers::make_warper_from_proj_histograms(std::move(data), count, dimension, num_bins, sigma);
+}
+
 std::vector<double> integrator_direct::autotune(long sample_points) {
   prepare_device_run();
   int32_t lanes = 0, wpc = 0;

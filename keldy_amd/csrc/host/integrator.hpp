@@ -125,6 +125,11 @@ class integrator_direct {
   /// current chain (generator and accumulator untouched): (lanes per point,
   /// waves per CU, LU ms per 2^20 points).
   std::vector<double> autotune(long sample_points);
+  /// make_warper_from_proj with the pilot on the device: the warped integrand
+  /// (current chain) at the first nr_samples unshifted Sobol points and the
+  /// per-axis histograms are computed there; the host smooths and fits.
+  warpers::warper_product_1d_interp_nearest_t make_warper_from_proj_device(int num_bins, long nr_samples, double sigma,
+                                                                          bool keep_u_hypercube);
   /// device state before / after a run launched elsewhere (run_batch)
   void prepare_device_run();
   void finish_device_run(long nr_steps);

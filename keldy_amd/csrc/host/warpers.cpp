@@ -562,6 +562,15 @@ warper_product_1d_interp_nearest_t make_warper_from_proj_samples(std::vector<vec
       count[a][b]++;
     }
   }
+  return make_warper_from_proj_histograms(std::move(data), count, order, num_bins, sigma);
+}
+
+warper_product_1d_interp_nearest_t make_warper_from_proj_histograms(std::vector<vec> data,
+                                                                    std::vector<std::vector<long>> const &count,
+                                                                    int order, int num_bins, double sigma) {
+  if (num_bins < 2) throw keldy_error("num_bins must be >= 2");
+  if (int(data.size()) != order || int(count.size()) != order) throw keldy_error("histograms of wrong dimension");
+  const double bin_size = 1.0 / num_bins;
   warper_product_1d_interp_nearest_t warper;
   vec time_pts(num_bins + 1), f1_pts(num_bins + 1);
   const double delta = 1.0 / num_bins;

@@ -237,6 +237,11 @@ warper_product_1d_interp_nearest_t make_warper_from_proj(std::function<dcomplex(
 /// nr_samples unshifted Sobol points) and |f| at them.
 warper_product_1d_interp_nearest_t make_warper_from_proj_samples(std::vector<vec> const &l, vec const &abs_values,
                                                                  int order, int num_bins, double sigma);
+/// The fit from per-axis histograms: sums of |f| and sample counts per bin of
+/// [0, 1] (the device computes them, kd_pilot_histograms).
+warper_product_1d_interp_nearest_t make_warper_from_proj_histograms(std::vector<vec> sums,
+                                                                    std::vector<std::vector<long>> const &counts,
+                                                                    int order, int num_bins, double sigma);
 namespace proj_details {
 vec kernel_smoothing(vec const &y_in, double sigma);
 /// weighted least squares y = c0 + c1 x (weights <= 0 skipped), running means

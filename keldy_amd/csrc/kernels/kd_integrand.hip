@@ -394,6 +394,32 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
   }
 }
 
+// Pilot projection (make_warper_from_proj.hpp:80-114 gather_data): per axis a
+// and bin b of [0, 1], the sum of |value| over the samples whose coordinate a
+// falls in b, and their count.  One thread per (axis, bin) scans the samples in
+// order, so the sums are those of a sequential loop, bit for bit.
+__global__ __launch_bounds__(256) void pilot_hist_kernel(const double *__restrict__ l, const double *__restrict__ vals,
+                                                         int64_t n, int order, int nbins, double *__restrict__ sums,
+                                                         unsigned long long *__restrict__ counts) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= order * nbins) return;
+  const int a = t / nbins, b = t - a * nbins;
+  const double bs = 1.0 / nbins;
+  double s = 0.0;
+  unsigned long long c = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const double x = l[i * order + a];
+    if (!(x >= 0.0 && x <= 1.0)) continue;
+    const int bi = x == 1.0 ? nbins - 1 : (int)(x / bs);
+    if (bi == b) {
+      s += hypot(vals[2 * i], vals[2 * i + 1]);
+      c++;
+    }
+  }
+  sums[t] = s;
+  counts[t] = c;
+}
+
 // acc[b] += sum over blocks of part[blk][b] (fixed order), one thread per bin.
 __global__ __launch_bounds__(256) void reduce_bins_kernel(const double2 *__restrict__ part, int nblk, int nbins,
                                                           double2 *__restrict__ acc) {
@@ -488,6 +514,14 @@ hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, con
 
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st) {
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, part, nblk, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_pilot_hist(const double *l, const double *vals, int64_t n, int order, int nbins, double *sums,
+                             unsigned long long *counts, hipStream_t st) {
+  const int tot = order * nbins;
+  hipLaunchKernelGGL(pilot_hist_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, l, vals, n, order, nbins, sums,
+                     counts);
   return hipGetLastError();
 }
 

@@ -30,6 +30,9 @@ hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, con
 hipError_t launch_tree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // acc[0..3] += sum over blocks of part[4*blk + 0..3] (fixed order).
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st);
+// per-(axis, bin) sums of |vals| and counts over n samples l [n][order] (pilot projection)
+hipError_t launch_pilot_hist(const double *l, const double *vals, int64_t n, int order, int nbins, double *sums,
+                             unsigned long long *counts, hipStream_t st);
 // acc[b] += sum over blocks of part[blk][b], b < nbins (fixed order).
 hipError_t launch_reduce_bins(const double2 *part, int nblk, int nbins, double2 *acc, hipStream_t st);
 

@@ -34,19 +34,18 @@ def make_warper_from_proj(warped_integrand, order, num_bins, nr_samples, sigma, 
 
     warped_integrand is either a callable l -> complex (evaluated once per
     pilot point, as in the reference) or an integrator object
-    (ComputeChargeQDirect, ...) whose warped integrand, with its current warper
-    chain, is then evaluated for all pilot points in one device batch --
-    without the positive-time restriction by default, as the reference's own
-    recipe does (projection_test.py:86), so the g0 window must cover
-    order * t.
+    (ComputeChargeQDirect, ...): then the pilot evaluations of its warped
+    integrand with its current warper chain AND the per-axis histograms run on
+    the device (only the smoothing and the fit run on the host) -- without the
+    positive-time restriction by default, as the reference's own recipe does
+    (projection_test.py:86), so the g0 window must cover order * t.
     """
-    batch = getattr(warped_integrand, "evaluate_warped_integrand_batch", None)
-    if batch is None:
+    device = getattr(warped_integrand, "make_warper_from_proj_device", None)
+    if device is None:
         return _w.make_warper_from_proj(warped_integrand, order, num_bins, nr_samples, sigma)
-    g = Sobol(order, 0)
-    l = np.array([g() for _ in range(nr_samples)], dtype=np.float64).reshape(nr_samples, order)
-    vals, _, _ = batch(l, keep_u_hypercube)
-    return _w.make_warper_from_proj_samples(l, np.abs(vals), order, num_bins, sigma)
+    if order != warped_integrand.get_integrand_dimension():
+        raise RuntimeError("order differs from the integrator's dimension")
+    return device(num_bins, nr_samples, sigma, keep_u_hypercube)
 
 
 __all__ = [n for n in dir() if n[0].isupper() or n.startswith(("make_", "vi_", "ui_", "kernel_"))]

@@ -153,7 +153,8 @@ def test_warped_single_vs_batch_and_partial_chain(model30):
 
 def test_pilot_fit_callable_vs_batch(model30):
     """make_warper_from_proj through the per-point callback (reference form)
-    and through one device batch give the same fitted maps."""
+    and through the device path (pilot evaluations and per-axis histograms on
+    the GPU) give the same fitted maps."""
     from keldy_amd import impurity_oneband as kio
     from keldy_amd import warpers as kw
     t = 10.0
@@ -170,3 +171,31 @@ def test_pilot_fit_callable_vs_batch(model30):
     c.warper.emplace_back(w2)
     c.run(2000)
     assert c.reduce_nr_points_run() == 2000
+
+
+def test_pilot_histograms_on_device_match_host_binning(model30):
+    """The device histograms equal the host binning of the same per-point
+    values (same summation order): the fitted knots agree bit for bit, and
+    with the host-sample fit of make_warper_from_proj_samples."""
+    from keldy_amd import common as kc
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t, order, nb, ns = 10.0, 4, 80, 3000
+    c = kio.ComputeChargeQDirect(model30, t, order, 0.0)
+    c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+    wd = c.make_warper_from_proj_device(nb, ns, 0.1, True)
+    g = kc.Sobol(order, 0)
+    l = np.array([g() for _ in range(ns)])
+    vals, _, _ = c.evaluate_warped_integrand_batch(l, True)
+    wh = kw.make_warper_from_proj_samples(l, np.abs(vals), order, nb, 0.1)
+    for a in range(order):
+        np.testing.assert_array_equal(wd.get_1d_warper(a).knots()[1], wh.get_1d_warper(a).knots()[1])
+    # the integrator's own generator is untouched by the pilot
+    c.reset_rng("sobol_unshifted", 0)
+    c.run(100)
+    ref = kio.ComputeChargeQDirect(model30, t, order, 0.0)
+    ref.warper.emplace_back(kw.WarperPlasmaUvT(t))
+    ref.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+    ref.run(100)
+    assert c.reduce_result() == ref.reduce_result()

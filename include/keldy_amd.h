@@ -17,6 +17,12 @@
  *   kd_set_sobol           integrator::reset_rng / keldy::sobol (integrator.hpp:97-112, qrng.hpp:49-114)
  *   kd_run                 integrator::run (integrator.hpp:81-95)
  *   kd_get_accum/kd_reset_accum  reduce_result / reduce_nr_points_* (integrator.hpp:123-131)
+ *   kd_set_kernel_integrand integrand_g_kernel ctor (impurity_oneband/wick_kernel.hpp:56)
+ *   kd_eval_kernel_batch   integrand_g_kernel::operator() (wick_kernel.cpp:42-138)
+ *   kd_set_kernel_bins / kd_get_kernel_bins  compute_gf_kernel's binner_t<1,1>
+ *                          (compute_obs.hpp:105-117, binner.hpp:147-352)
+ *   kd_set_kernel_omega_integrand  integrand_g_kernel_single_omega ctor and
+ *                          operator() (wick_kernel_single_omega.hpp:45-58, .cpp:41-140)
  */
 #ifndef KELDY_AMD_H
 #define KELDY_AMD_H
@@ -183,7 +189,7 @@ int kd_pilot_histograms(kd_ctx *ctx, const uint32_t *Csr, const uint32_t *shfr, 
  * time (bins of width (xmax - xmin) / nbins on [xmin, xmax], xmax in the last
  * bin), and every point, in domain or not, counts one value added to that bin
  * or one dropped value when its earliest time is outside [xmin, xmax].
- * nbins = 0 switches binning off.  values: nbins x (re, im) sums (divide by
+ * nbins in [0, 3072]; 0 switches binning off.  values: nbins x (re, im) sums (divide by
  * the points run for the estimate); counts: nbins. */
 int kd_set_time_bins(kd_ctx *ctx, double xmin, double xmax, int32_t nbins);
 int kd_get_time_bins(kd_ctx *ctx, double *values, uint64_t *counts, uint64_t *dropped);
@@ -234,6 +240,45 @@ int kd_flow_collect(kd_ctx *ctx, int64_t n_samples, double weight_power, double 
  * 64 | batch, batch <= 2^16), learning rate cosine-decayed from lr to lr_final; per-step loss
  * into losses (steps doubles, may be NULL). */
 int kd_flow_train(kd_ctx *ctx, int32_t steps, int32_t batch, float lr, float lr_final, double *losses);
+
+/* ---- kernel method ---------------------------------------------------------
+ * Observables of the integrator besides the Keldysh sum of kd_set_integrand.
+ * KD_OBS_KERNEL (integrand_g_kernel, wick_kernel.cpp:42-138): per point and
+ * Keldysh configuration c of the `order` vertices, one complex entry per vertex
+ * i -- the cofactor of the external row of the same-spin Wick matrix (external
+ * point X first, the last vertex's diagonal entry set to zero as in the
+ * reference), times c's parity and the opposite-spin determinant.  Entry (c, i)
+ * belongs to vertex time u_i and Keldysh index bit i of c.  kd_run adds every
+ * entry times the warper Jacobian to the bin (time u_i, index) of
+ * kd_set_kernel_bins (binner_t<1,1>: time bins of width (xmax - xmin) / nbins,
+ * xmax in the last one, and 2 index bins); the accumulator's re, im hold the
+ * sum of all entries.  KD_OBS_KERNEL_OMEGA (integrand_g_kernel_single_omega):
+ * the scalar sum over entries of value * exp(i omega u_i); every scalar entry
+ * point (kd_eval_batch, kd_eval_warped_batch, kd_run, kd_sample_device) works
+ * with it as with the direct integrand.  Both are FP64 (kd_set_precision does
+ * not apply) and use the pivot check / pivoted fallback of the Keldysh sum.
+ * kd_set_integrand switches back to the direct integrand. */
+#define KD_OBS_DIRECT 0
+#define KD_OBS_KERNEL 1
+#define KD_OBS_KERNEL_OMEGA 2
+
+/* external point X (its split is set to `order`), order in [1, KD_MAX_ORDER];
+ * clears the bins */
+int kd_set_kernel_integrand(kd_ctx *ctx, int32_t order, const kd_gf_index *X);
+int kd_set_kernel_omega_integrand(kd_ctx *ctx, int32_t order, const kd_gf_index *X, double omega);
+/* nbins in [0, 1536] (0: no binning) */
+int kd_set_kernel_bins(kd_ctx *ctx, double xmin, double xmax, int32_t nbins);
+/* values [nbins][2] (re, im) sums since the last kd_reset_accum (divide by the
+ * points run for the estimate); counts [nbins][2] entries added; dropped:
+ * entries whose vertex time was outside [xmin, xmax] */
+int kd_get_kernel_bins(kd_ctx *ctx, double *values, uint64_t *counts, uint64_t *dropped);
+/* Entries at n_points points (host memory): pts are vertex times u (warped =
+ * 0) or hypercube points l pushed through the warper chain (warped = 1, the
+ * entries then carry the Jacobian).  out: [n_points][2^order][order] (re, im),
+ * zero for points out of the domain; in_domain, jacobian [n_points] (may be
+ * NULL).  At most 2^26 entries per call. */
+int kd_eval_kernel_batch(kd_ctx *ctx, const double *pts, int64_t n_points, int32_t warped, int32_t keep_u_hypercube,
+                         double *out, int32_t *in_domain, double *jacobian);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ HIP_SOURCES = [
     "kernels/kd_integrand.hip",
     "kernels/kd_tree.hip",
     "kernels/kd_flow.hip",
+    "kernels/kd_cofactor.hip",
     "capi/kd_capi.cpp",
 ]
 

@@ -21,6 +21,7 @@ KD_MAP_UNIFORM, KD_MAP_EXPONENTIAL, KD_MAP_INVERSE, KD_MAP_INVERSE_SQ = 0, 1, 2,
 KD_MAP_INVERSE_POW, KD_MAP_NEAREST, KD_MAP_HYBRID = 4, 5, 6
 KD_WARP_IDENTITY, KD_WARP_PLASMA, KD_WARP_PRODUCT, KD_WARP_FLOW = 0, 1, 2, 3
 KD_ALGO_CONFIG, KD_ALGO_TREE = 0, 1
+KD_OBS_DIRECT, KD_OBS_KERNEL, KD_OBS_KERNEL_OMEGA = 0, 1, 2
 
 
 class GfIndex(C.Structure):
@@ -54,7 +55,8 @@ SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_se
            "kd_flow_get_params", "kd_flow_map", "kd_flow_loss_grad", "kd_flow_collect", "kd_flow_train",
            "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch",
            "kd_set_time_bins", "kd_get_time_bins", "kd_set_algorithm", "kd_get_algorithm",
-           "kd_pilot_histograms"]
+           "kd_pilot_histograms", "kd_set_kernel_integrand", "kd_set_kernel_omega_integrand",
+           "kd_set_kernel_bins", "kd_get_kernel_bins", "kd_eval_kernel_batch"]
 
 _lib = None
 
@@ -107,6 +109,12 @@ def lib():
         L.kd_pilot_histograms.argtypes = [vp, up, up, C.c_int64, C.c_int32, C.c_int32, dp, C.POINTER(C.c_uint64)]
         L.kd_get_algorithm.argtypes = [vp, ip]
         L.kd_get_time_bins.argtypes = [vp, dp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        u64p = C.POINTER(C.c_uint64)
+        L.kd_set_kernel_integrand.argtypes = [vp, C.c_int32, C.POINTER(GfIndex)]
+        L.kd_set_kernel_omega_integrand.argtypes = [vp, C.c_int32, C.POINTER(GfIndex), C.c_double]
+        L.kd_set_kernel_bins.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
+        L.kd_get_kernel_bins.argtypes = [vp, dp, u64p, u64p]
+        L.kd_eval_kernel_batch.argtypes = [vp, dp, C.c_int64, C.c_int32, C.c_int32, dp, ip, dp]
         _lib = L
     return _lib
 
@@ -241,6 +249,38 @@ class Context:
         d = C.c_uint64()
         self._check(lib().kd_get_time_bins(self.h, _dp(v), cnt.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(d)))
         return v[:, 0] + 1j * v[:, 1], cnt, d.value
+
+    # -- kernel method ---------------------------------------------------
+    def set_kernel_integrand(self, order, X, omega=None):
+        """integrand_g_kernel (omega None) or integrand_g_kernel_single_omega at X"""
+        if omega is None:
+            self._check(lib().kd_set_kernel_integrand(self.h, order, C.byref(X)))
+        else:
+            self._check(lib().kd_set_kernel_omega_integrand(self.h, order, C.byref(X), float(omega)))
+        self.order = order
+
+    def set_kernel_bins(self, xmin, xmax, nbins):
+        self._check(lib().kd_set_kernel_bins(self.h, xmin, xmax, nbins))
+        self.nbins = nbins
+
+    def kernel_bins(self):
+        """-> (complex sums [nbins][2], entries added [nbins][2], dropped)"""
+        v = np.empty((self.nbins, 2, 2))
+        cnt = np.empty((self.nbins, 2), dtype=np.uint64)
+        d = C.c_uint64()
+        self._check(lib().kd_get_kernel_bins(self.h, _dp(v), cnt.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(d)))
+        return v[..., 0] + 1j * v[..., 1], cnt, d.value
+
+    def eval_kernel_batch(self, pts, warped=False, keep_u_hypercube=True):
+        """-> (entries [n][2^order][order] complex, in_domain [n], jacobian [n])"""
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        n, order = pts.shape
+        out = np.empty((n, 1 << order, order, 2))
+        dom = np.empty(n, dtype=np.int32)
+        jac = np.empty(n)
+        self._check(lib().kd_eval_kernel_batch(self.h, _dp(pts), n, int(warped), int(keep_u_hypercube), _dp(out),
+                                               dom.ctypes.data_as(C.POINTER(C.c_int32)), _dp(jac)))
+        return out[..., 0] + 1j * out[..., 1], dom, jac
 
     # -- launch shape ----------------------------------------------------
     def autotune(self, sample_points):

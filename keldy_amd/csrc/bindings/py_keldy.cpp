@@ -509,6 +509,130 @@ PYBIND11_MODULE(_keldy_host, mod) {
             py::arg("li_vec"));
     bind_integrator_methods<T>(c);
   }
+  // kernel method (compute_obs.hpp:102-133, wick_kernel*.hpp)
+  py::class_<binner::binner_1d_1_t>(im, "Binner_1_1", "binner_t<1,1> (binner.hpp:147-352): time x Keldysh index")
+      .def("get_data", [](binner::binner_1d_1_t const &b) {
+        py::array_t<dcomplex> a({py::ssize_t(b.nr_bins), py::ssize_t(b.nr_disc)});
+        std::copy(b.data.begin(), b.data.end(), a.mutable_data());
+        return a;
+      })
+      .def("get_nr_values_added", [](binner::binner_1d_1_t const &b) {
+        py::array_t<long> a({py::ssize_t(b.nr_bins), py::ssize_t(b.nr_disc)});
+        std::copy(b.nr_values_added.begin(), b.nr_values_added.end(), a.mutable_data());
+        return a;
+      })
+      .def("get_nr_values_dropped", [](binner::binner_1d_1_t const &b) { return b.nr_values_dropped; })
+      .def("get_nr_bins", [](binner::binner_1d_1_t const &b, int axis) {
+        if (axis < 0 || axis > 1) throw keldy_error("Axis " + std::to_string(axis) + " is larger than the number of dimensions.");
+        return axis == 0 ? b.nr_bins : b.nr_disc;
+      }, py::arg("axis") = 0)
+      .def("get_bin_coord", [](binner::binner_1d_1_t const &b, int axis) {
+        if (axis == 1) throw keldy_error("Axis 1 is discreet.");
+        if (axis != 0) throw keldy_error("Axis " + std::to_string(axis) + " is larger than the number of dimensions.");
+        return b.get_bin_coord();
+      }, py::arg("axis") = 0)
+      .def("get_bin_size", [](binner::binner_1d_1_t const &b, int axis) {
+        if (axis == 1) throw keldy_error("Axis 1 is discreet.");
+        if (axis != 0) throw keldy_error("Axis " + std::to_string(axis) + " is larger than the number of dimensions.");
+        return b.get_bin_size();
+      }, py::arg("axis") = 0)
+      .def("get_continuous_axes", [](binner::binner_1d_1_t const &b) {
+        return std::vector<std::tuple<double, double, long, double>>{{b.xmin, b.xmax, b.nr_bins, b.get_bin_size()}};
+      })
+      .def("get_discreet_axes", [](binner::binner_1d_1_t const &b) { return std::vector<long>{b.nr_disc}; });
+  py::class_<binner::sparse_binner_1d_1_t>(im, "SparseBinner_1_1", "sparse_binner_t<1,1> (binner.hpp:59-140)")
+      .def_readonly("data", &binner::sparse_binner_1d_1_t::data)
+      .def("sum_moduli", &binner::sparse_binner_1d_1_t::sum_moduli);
+  py::class_<integrand_g_kernel>(im, "IntegrandGKernel")
+      .def(py::init<g0_keldysh_contour_t, gf_index_t>(), py::arg("g0_"), py::arg("g_idx_X_"))
+      .def("__call__", &integrand_g_kernel::operator(), py::arg("times"), py::arg("keep_u_hypercube") = true)
+      .def(
+          "eval_entries",
+          [](integrand_g_kernel const &f, py::array_t<double, py::array::c_style | py::array::forcecast> u, bool keep) {
+            if (u.ndim() != 2 || u.shape(1) < 1) throw keldy_error("expected an array of shape (n, order)");
+            const long n = long(u.shape(0));
+            const int order = int(u.shape(1));
+            if (order > KD_MAX_ORDER) throw keldy_error("order exceeds the device maximum");
+            py::array_t<dcomplex> out({py::ssize_t(n), py::ssize_t(1) << order, py::ssize_t(order)});
+            py::array_t<int> dom(n);
+            {
+              py::gil_scoped_release rel;
+              f.eval_entries(u.data(), n, order, keep, out.mutable_data(), dom.mutable_data());
+            }
+            return py::make_tuple(out, dom);
+          },
+          py::arg("u"), py::arg("keep_u_hypercube") = true,
+          "entries [n][2^order][order]: configuration c, vertex i (time u_i, Keldysh index bit i of c)");
+  py::class_<integrand_g_kernel_single_omega>(im, "IntegrandGKernelSingleOmega")
+      .def(py::init<g0_keldysh_contour_t, gf_index_t, double>(), py::arg("g0_"), py::arg("g_idx_X_"), py::arg("omega_"))
+      .def("__call__", &integrand_g_kernel_single_omega::operator(), py::arg("times"),
+           py::arg("keep_u_hypercube") = true)
+      .def(
+          "eval_batch",
+          [](integrand_g_kernel_single_omega const &f, py::array_t<double, py::array::c_style | py::array::forcecast> u,
+             bool keep) {
+            if (u.ndim() != 2) throw keldy_error("expected an array of shape (n, order)");
+            const long n = long(u.shape(0));
+            py::array_t<dcomplex> out(n);
+            py::array_t<int> dom(n);
+            {
+              py::gil_scoped_release rel;
+              f.eval_batch(u.data(), n, int(u.shape(1)), keep, out.mutable_data(), dom.mutable_data());
+            }
+            return py::make_tuple(out, dom);
+          },
+          py::arg("u"), py::arg("keep_u_hypercube") = true);
+  {
+    using T = compute_gf_kernel;
+    py::class_<T, integrator_direct> c(im, "ComputeGfKernel");
+    c.def(py::init<g0_model, double, int, int>(), py::arg("model"), py::arg("time"), py::arg("order"),
+          py::arg("nr_bins") = 100)
+        .def(py::init([](py::dict params, double time, int order, int nr_bins) {
+               return new T(params_from_dict(params), time, order, nr_bins);
+             }),
+             py::arg("params"), py::arg("time"), py::arg("order"), py::arg("nr_bins") = 100)
+        .def("reduce_result", &T::reduce_result_binned)
+        .def("get_integrand", &T::get_integrand_kernel)
+        .def("evaluate_warped_integrand", &T::evaluate_warped_integrand_kernel, py::arg("li_vec"),
+             py::arg("start_domain_nr"), py::arg("keep_u_hypercube") = true)
+        .def(
+            "evaluate_warped_integrand",
+            [](T &s, std::vector<double> li) {
+              return s.evaluate_warped_integrand_kernel(li, int(s.warper.size()), true);
+            },
+            py::arg("li_vec"))
+        .def(
+            "evaluate_warped_entries_batch",
+            [](T &s, py::array_t<double, py::array::c_style | py::array::forcecast> l, bool keep) {
+              const int order = s.get_dimension();
+              if (l.ndim() != 2 || l.shape(1) != order) throw keldy_error("expected an array of shape (n, order)");
+              const long n = long(l.shape(0));
+              py::array_t<dcomplex> out({py::ssize_t(n), py::ssize_t(1) << order, py::ssize_t(order)});
+              py::array_t<int> dom(n);
+              py::array_t<double> jac(n);
+              {
+                py::gil_scoped_release rel;
+                s.evaluate_warped_entries_batch(l.data(), n, out.mutable_data(), dom.mutable_data(),
+                                                jac.mutable_data(), keep);
+              }
+              return py::make_tuple(out, dom, jac);
+            },
+            py::arg("li"), py::arg("keep_u_hypercube") = true);
+    bind_integrator_methods<T>(c);
+  }
+  {
+    using T = compute_gf_kernel_single_omega;
+    py::class_<T, integrator_direct> c(im, "ComputeGfKernelSingleOmega");
+    c.def(py::init<g0_model, double, double, int>(), py::arg("model"), py::arg("time"), py::arg("omega"),
+          py::arg("order"))
+        .def(py::init([](py::dict params, double time, double omega, int order) {
+               return new T(params_from_dict(params), time, omega, order);
+             }),
+             py::arg("params"), py::arg("time"), py::arg("omega"), py::arg("order"))
+        .def("reduce_result", &T::reduce_result)
+        .def("get_integrand", &T::get_integrand_omega);
+    bind_integrator_methods<T>(c);
+  }
   im.def(
       "run_batch",
       [](py::list integrators, std::vector<long> nr_steps) {

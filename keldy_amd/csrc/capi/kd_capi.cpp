@@ -32,6 +32,8 @@ struct kd_ctx {
   kd_gf_index A{}, B{};
   double cutoff = 0.0;
   int precision = KD_PREC_FP64;
+  int obs = KD_OBS_DIRECT; // observable (kd_set_integrand / kd_set_kernel_integrand)
+  double omega = 0.0;
   // warper chain
   kd_warper_desc *d_warpers = nullptr;
   kd_map1d *d_maps = nullptr;
@@ -81,6 +83,7 @@ struct kd_ctx {
   bool no_reduce = false; // autotuning launches leave the accumulator alone
   // time binning (kd_set_time_bins)
   int nbins = 0;
+  int bin_k = 1; // bins per time bin: 1 (time binning), 2 (kernel method: Keldysh index)
   double bin_min = 0.0, bin_max = 0.0;
   double2 *d_bins = nullptr, *d_bin_part = nullptr;
   unsigned long long *d_bin_counts = nullptr;
@@ -114,7 +117,12 @@ int ensure_stage(kd_ctx *c, size_t bytes) {
   return KD_OK;
 }
 
-bool use_tree(const kd_ctx *c) { return c->algo == KD_ALGO_TREE && c->order >= 6; }
+bool use_tree(const kd_ctx *c) { return c->algo == KD_ALGO_TREE && c->order >= 6 && c->obs == KD_OBS_DIRECT; }
+
+// complex values per point of an evaluation output
+int64_t vals_per_point(const kd_ctx *c) {
+  return c->obs == KD_OBS_KERNEL ? (int64_t(1) << c->order) * c->order : 1;
+}
 
 // LU-kernel grid: one workgroup per group of points (64 / lanes points per
 // group), capped at lu_waves_per_cu waves per CU -- the kernel is persistent
@@ -131,7 +139,7 @@ int ensure_bin_partials(kd_ctx *c, int grid) {
   if (c->nbins <= 0 || grid <= c->bin_part_cap) return KD_OK;
   dfree(c->d_bin_part);
   c->bin_part_cap = 0;
-  KD_HIP(c, hipMalloc(&c->d_bin_part, sizeof(double2) * (size_t)c->nbins * (size_t)grid));
+  KD_HIP(c, hipMalloc(&c->d_bin_part, sizeof(double2) * (size_t)c->nbins * c->bin_k * (size_t)grid));
   c->bin_part_cap = grid;
   return KD_OK;
 }
@@ -170,6 +178,8 @@ kd::KdParams base_params(const kd_ctx *c) {
   p.bin_max = c->bin_max;
   p.bin_size = c->nbins > 0 ? (c->bin_max - c->bin_min) / c->nbins : 0.0;
   p.bin_counts = c->d_bin_counts;
+  p.obs = c->obs;
+  p.omega = c->omega;
   return p;
 }
 
@@ -279,7 +289,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
     p.n_points = m;
     p.k0 = p0.k0 + (uint32_t)off;
     if (p.in_pts) p.in_pts = p0.in_pts + off * N;
-    if (p.out_vals) p.out_vals = p0.out_vals + 2 * off;
+    if (p.out_vals) p.out_vals = p0.out_vals + 2 * off * vals_per_point(c);
     if (p.out_dom) p.out_dom = p0.out_dom + off;
     if (p.out_jac) p.out_jac = p0.out_jac + off;
     if (p.out_l) p.out_l = p0.out_l + off * N;
@@ -317,7 +327,9 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
     KD_HIP(c, kd::launch_prep(N, p, buf, grid_prep, c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
-    if (use_tree(c))
+    if (c->obs != KD_OBS_DIRECT)
+      KD_HIP(c, kd::launch_cofactor(N, p, buf, grid, c->stream));
+    else if (use_tree(c))
       KD_HIP(c, kd::launch_tree(N, c->precision, p, buf, grid, c->stream));
     else
       KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, p, buf, grid, c->stream));
@@ -327,7 +339,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
       KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->d_acc, c->stream));
       launches += 1;
       if (p.nbins > 0) {
-        KD_HIP(c, kd::launch_reduce_bins(c->d_bin_part, grid, p.nbins, c->d_bins, c->stream));
+        KD_HIP(c, kd::launch_reduce_bins(c->d_bin_part, grid, p.nbins * c->bin_k, c->d_bins, c->stream));
         launches += 1;
       }
     }
@@ -337,6 +349,35 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
   c->last_chunks = timed ? (int)nchunks : 0;
   return KD_OK;
 }
+
+// Bin storage shared by the time binning (k = 1 value per bin) and the
+// kernel method (k = 2: the Keldysh index): values [nbins][k], counts
+// [nbins + 1] (values added per time bin, then the dropped ones).
+int set_bins(kd_ctx *c, double xmin, double xmax, int nbins, int k) {
+  if (nbins > 0 && !(xmin < xmax)) {
+    return fail(c, KD_ERR_ARG, "Boundaries of binning range are in wrong order (" + std::to_string(xmin) +
+                                   " >= " + std::to_string(xmax) + ")");
+  }
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  dfree(c->d_bins);
+  dfree(c->d_bin_part);
+  dfree(c->d_bin_counts);
+  c->bin_part_cap = 0;
+  c->nbins = 0;
+  c->bin_k = k;
+  if (nbins == 0) return KD_OK;
+  KD_HIP(c, hipMalloc(&c->d_bins, sizeof(double2) * nbins * k));
+  KD_HIP(c, hipMalloc(&c->d_bin_counts, sizeof(unsigned long long) * (nbins + 1)));
+  KD_HIP(c, hipMemset(c->d_bins, 0, sizeof(double2) * nbins * k));
+  KD_HIP(c, hipMemset(c->d_bin_counts, 0, sizeof(unsigned long long) * (nbins + 1)));
+  c->nbins = nbins;
+  c->bin_min = xmin;
+  c->bin_max = xmax;
+  return KD_OK;
+}
+
+int rc_bins_clear(kd_ctx *c) { return set_bins(c, 0.0, 1.0, 0, 1); }
 
 int finish_timing(kd_ctx *c) {
   KD_HIP(c, hipEventSynchronize(c->ev1));
@@ -467,10 +508,12 @@ int kd_set_integrand(kd_ctx *c, int32_t order, const kd_gf_index *A, const kd_gf
     return fail(c, KD_ERR_ARG, "Need g0_model with make_dot_lead == true to calculate off_diagonal gf.");
   if (A->orbital < 0 || A->orbital > 1 || B->orbital < 0 || B->orbital > 1)
     return fail(c, KD_ERR_ARG, "orbital must be 0 or 1");
+  if (c->obs != KD_OBS_DIRECT && c->nbins > 0 && (rc_bins_clear(c) != KD_OK)) return KD_ERR_HIP;
   c->order = order;
   c->A = *A;
   c->B = *B;
   c->cutoff = cutoff;
+  c->obs = KD_OBS_DIRECT;
   return KD_OK;
 }
 
@@ -487,6 +530,7 @@ int kd_eval_batch_device(kd_ctx *c, const double *d_u, int64_t n_points, int32_t
   if (!c) return KD_ERR_ARG;
   int rc = need_integrand(c);
   if (rc) return rc;
+  if (c->obs == KD_OBS_KERNEL) return fail(c, KD_ERR_STATE, "kernel-method entries: use kd_eval_kernel_batch");
   if (n_points < 0) return fail(c, KD_ERR_ARG, "negative point count");
   if (n_points == 0) return KD_OK;
   KD_HIP(c, hipSetDevice(c->device));
@@ -505,6 +549,7 @@ int kd_eval_batch(kd_ctx *c, const double *u, int64_t n_points, int32_t keep_u_h
   if (!c) return KD_ERR_ARG;
   int rc = need_integrand(c);
   if (rc) return rc;
+  if (c->obs == KD_OBS_KERNEL) return fail(c, KD_ERR_STATE, "kernel-method entries: use kd_eval_kernel_batch");
   if (n_points <= 0) return n_points == 0 ? KD_OK : fail(c, KD_ERR_ARG, "negative point count");
   if ((!u && c->order > 0) || !out) return fail(c, KD_ERR_ARG, "null buffer");
   const size_t nu = (size_t)n_points * (size_t)c->order;
@@ -581,6 +626,7 @@ int kd_eval_warped_batch(kd_ctx *c, const double *l, int64_t n_points, int32_t k
   if (!c) return KD_ERR_ARG;
   int rc = need_integrand(c);
   if (rc) return rc;
+  if (c->obs == KD_OBS_KERNEL) return fail(c, KD_ERR_STATE, "kernel-method entries: use kd_eval_kernel_batch");
   if (n_points <= 0) return n_points == 0 ? KD_OK : fail(c, KD_ERR_ARG, "negative point count");
   if ((!l && c->order > 0) || !out) return fail(c, KD_ERR_ARG, "null buffer");
   const size_t nl = (size_t)n_points * (size_t)c->order;
@@ -638,6 +684,8 @@ int kd_get_sobol_state(const kd_ctx *c, uint32_t *k) {
 static int run_common(kd_ctx *c, uint64_t n_steps, int mode, double *d_l, double *d_value) {
   int rc = need_integrand(c);
   if (rc) return rc;
+  if (mode == kd::MODE_SAMPLE && c->obs == KD_OBS_KERNEL)
+    return fail(c, KD_ERR_STATE, "per-point samples need a scalar integrand (not the binned kernel method)");
   if (c->nbins > 0 && c->order == 0) return fail(c, KD_ERR_ARG, "Order 0 is not implemented");
   if (!c->has_sobol) return fail(c, KD_ERR_STATE, "rng not set (kd_set_sobol)");
   if (c->sobol_dim != c->order) return fail(c, KD_ERR_ARG, "Sobol dimension differs from the integrand order");
@@ -686,7 +734,7 @@ int kd_reset_accum(kd_ctx *c) {
   KD_HIP(c, hipSetDevice(c->device));
   KD_HIP(c, hipMemsetAsync(c->d_acc, 0, 4 * sizeof(double), c->stream));
   if (c->nbins > 0) {
-    KD_HIP(c, hipMemsetAsync(c->d_bins, 0, sizeof(double2) * c->nbins, c->stream));
+    KD_HIP(c, hipMemsetAsync(c->d_bins, 0, sizeof(double2) * c->nbins * c->bin_k, c->stream));
     KD_HIP(c, hipMemsetAsync(c->d_bin_counts, 0, sizeof(unsigned long long) * (c->nbins + 1), c->stream));
   }
   KD_HIP(c, hipStreamSynchronize(c->stream));
@@ -718,6 +766,25 @@ int kd_last_timing(const kd_ctx *c, double *total_ms, double *lu_ms, double *pre
 // per configuration 14 complex multiply-adds (112) + 3 reciprocals (21) + 6
 // row scalings (36) + 3 determinant products (18), then path x bottom
 // determinant (12), det^2 * s (12) and the signed FP64 sum (2).
+// Executed flops per point of the kernel method (kd_cofactor.hip), per
+// configuration: the row-wise LU of the n vertex rows carried over the extra
+// column (complex multiply-adds sum_{i<n} sum_{k<i} (n - k), 8 flops each; n-1
+// reciprocals, 7; sum_{i<n-1} (n - i) row scalings and n-1 pivot products, 6
+// each), p' (2), the scaled back substitution (n-1 products, sum_{i<n-1}
+// (n-1-i) multiply-adds), the coefficient (3 products) and n entries (6 each),
+// plus the accumulation (2 per entry).
+static double cofactor_flops(int N) {
+  if (N < 1) return 0.0;
+  double macs = 0.0, muls = 0.0;
+  for (int i = 0; i < N; i++)
+    for (int k = 0; k < i; k++) macs += N - k;
+  for (int i = 0; i < N - 1; i++) muls += (N - i) + 1;
+  for (int i = 0; i < N - 1; i++) macs += N - 1 - i;
+  muls += (N - 1) + 3 + N;
+  const double per_cfg = 8 * macs + 6 * muls + 7 * (N - 1) + 2 + 4.0 * N;
+  return std::ldexp(per_cfg, N);
+}
+
 static double tree_flops(int N) {
   auto node = [](double dc) { return 8.0 * dc * dc + 13.0 * dc; };
   const int A = N - 6;
@@ -730,6 +797,10 @@ static double tree_flops(int N) {
 
 int kd_flops_per_point(const kd_ctx *c, double *flops) {
   if (!c || !flops) return KD_ERR_ARG;
+  if (c->obs != KD_OBS_DIRECT) {
+    *flops = cofactor_flops(c->order);
+    return KD_OK;
+  }
   if (use_tree(c)) {
     *flops = tree_flops(c->order);
     return KD_OK;
@@ -1037,7 +1108,8 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   // candidates: the tree (orders >= 6; its lanes are fixed) and the
   // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16
   // or 32 waves per CU
-  for (int li = (c->order >= 6 ? -1 : 0); li < 3 && !rc; li++) {
+  const bool direct = c->obs == KD_OBS_DIRECT;
+  for (int li = (c->order >= 6 && direct ? -1 : 0); li < (direct ? 3 : 1) && !rc; li++) {
     const int L = li < 0 ? 64 : lane_opts[li];
     if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
     for (int wi = 0; wi < 3 && !rc; wi++) {
@@ -1190,27 +1262,10 @@ int kd_run_batch(kd_ctx *const *ctxs, const uint64_t *n_steps, int32_t count, kd
 
 int kd_set_time_bins(kd_ctx *c, double xmin, double xmax, int32_t nbins) {
   if (!c) return KD_ERR_ARG;
-  if (nbins < 0 || nbins > 4096) return fail(c, KD_ERR_ARG, "number of bins must be in [0, 4096]");
-  if (nbins > 0 && !(xmin < xmax)) {
-    return fail(c, KD_ERR_ARG, "Boundaries of binning range are in wrong order (" + std::to_string(xmin) +
-                                   " >= " + std::to_string(xmax) + ")");
-  }
-  KD_HIP(c, hipSetDevice(c->device));
-  KD_HIP(c, hipStreamSynchronize(c->stream));
-  dfree(c->d_bins);
-  dfree(c->d_bin_part);
-  dfree(c->d_bin_counts);
-  c->bin_part_cap = 0;
-  c->nbins = 0;
-  if (nbins == 0) return KD_OK;
-  KD_HIP(c, hipMalloc(&c->d_bins, sizeof(double2) * nbins));
-  KD_HIP(c, hipMalloc(&c->d_bin_counts, sizeof(unsigned long long) * (nbins + 1)));
-  KD_HIP(c, hipMemset(c->d_bins, 0, sizeof(double2) * nbins));
-  KD_HIP(c, hipMemset(c->d_bin_counts, 0, sizeof(unsigned long long) * (nbins + 1)));
-  c->nbins = nbins;
-  c->bin_min = xmin;
-  c->bin_max = xmax;
-  return KD_OK;
+  if (c->obs == KD_OBS_KERNEL) return fail(c, KD_ERR_STATE, "the kernel method bins with kd_set_kernel_bins");
+  // the per-workgroup histogram lives in LDS next to the Wick matrices
+  if (nbins < 0 || nbins > 3072) return fail(c, KD_ERR_ARG, "number of bins must be in [0, 3072]");
+  return set_bins(c, xmin, xmax, nbins, 1);
 }
 
 int kd_get_time_bins(kd_ctx *c, double *values, uint64_t *counts, uint64_t *dropped) {
@@ -1240,6 +1295,8 @@ int kd_pilot_histograms(kd_ctx *c, const uint32_t *Csr, const uint32_t *shfr, in
   if (!Csr || !shfr || !sums || !counts) return fail(c, KD_ERR_ARG, "null buffer");
   if (c->order < 1) return fail(c, KD_ERR_ARG, "Dimension of the integration space must be > 0.");
   if (num_bins < 2 || num_bins > 65536) return fail(c, KD_ERR_ARG, "num_bins must be in [2, 65536]");
+  if (c->obs == KD_OBS_KERNEL)
+    return fail(c, KD_ERR_STATE, "per-point samples need a scalar integrand (not the binned kernel method)");
   if (n_samples < 1 || n_samples > (int64_t(1) << 24)) return fail(c, KD_ERR_ARG, "pilot sample count must be in [1, 2^24]");
   const int N = c->order;
   KD_HIP(c, hipSetDevice(c->device));
@@ -1297,6 +1354,104 @@ int kd_pilot_histograms(kd_ctx *c, const uint32_t *Csr, const uint32_t *shfr, in
       if (!std::isfinite(sums[i])) return fail(c, KD_ERR_ARG, "g0 evaluated outside its time window [-time_max, time_max]");
   }
   return rc;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// kernel method (wick_kernel.cpp:42-138, wick_kernel_single_omega.cpp:41-140)
+
+extern "C" {
+
+static int set_kernel(kd_ctx *c, int32_t order, const kd_gf_index *X, int obs, double omega) {
+  if (!c) return KD_ERR_ARG;
+  if (!X) return fail(c, KD_ERR_ARG, "null external point");
+  if (order < 1 || order > KD_MAX_ORDER)
+    return fail(c, KD_ERR_ARG, "order must be in [1, " + std::to_string(KD_MAX_ORDER) + "]");
+  if (!c->has_g0) return fail(c, KD_ERR_STATE, "g0 table not set (kd_set_g0_table)");
+  if (X->orbital < 0 || X->orbital > 1 || (X->orbital != 0 && c->n_orb < 2))
+    return fail(c, KD_ERR_ARG, "orbital must be 0, or 1 with a 2-orbital table");
+  if (X->spin < 0 || X->spin > 1 || X->k_idx < 0 || X->k_idx > 1) return fail(c, KD_ERR_ARG, "bad external point");
+  if (!std::isfinite(omega)) return fail(c, KD_ERR_ARG, "omega must be finite");
+  int rc;
+  if (c->nbins > 0 && (rc = rc_bins_clear(c))) return rc;
+  c->order = order;
+  c->A = *X;
+  c->B = *X;
+  c->cutoff = 0.0;
+  c->obs = obs;
+  c->omega = omega;
+  return KD_OK;
+}
+
+int kd_set_kernel_integrand(kd_ctx *c, int32_t order, const kd_gf_index *X) {
+  return set_kernel(c, order, X, KD_OBS_KERNEL, 0.0);
+}
+
+int kd_set_kernel_omega_integrand(kd_ctx *c, int32_t order, const kd_gf_index *X, double omega) {
+  return set_kernel(c, order, X, KD_OBS_KERNEL_OMEGA, omega);
+}
+
+int kd_set_kernel_bins(kd_ctx *c, double xmin, double xmax, int32_t nbins) {
+  if (!c) return KD_ERR_ARG;
+  if (c->obs != KD_OBS_KERNEL) return fail(c, KD_ERR_STATE, "kernel integrand not set (kd_set_kernel_integrand)");
+  if (nbins < 0 || nbins > 1536) return fail(c, KD_ERR_ARG, "number of bins must be in [0, 1536]");
+  return set_bins(c, xmin, xmax, nbins, 2);
+}
+
+int kd_get_kernel_bins(kd_ctx *c, double *values, uint64_t *counts, uint64_t *dropped) {
+  if (!c) return KD_ERR_ARG;
+  if (c->obs != KD_OBS_KERNEL || c->nbins <= 0) return fail(c, KD_ERR_STATE, "kernel bins not set (kd_set_kernel_bins)");
+  KD_HIP(c, hipSetDevice(c->device));
+  std::vector<unsigned long long> cnt(c->nbins + 1);
+  if (values)
+    KD_HIP(c, hipMemcpyAsync(values, c->d_bins, sizeof(double2) * 2 * c->nbins, hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipMemcpyAsync(cnt.data(), c->d_bin_counts, sizeof(unsigned long long) * (c->nbins + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  if (counts)
+    for (int i = 0; i < c->nbins; i++) counts[2 * i] = counts[2 * i + 1] = cnt[i];
+  if (dropped) *dropped = cnt[c->nbins];
+  return KD_OK;
+}
+
+int kd_eval_kernel_batch(kd_ctx *c, const double *pts, int64_t n_points, int32_t warped, int32_t keep_u_hypercube,
+                         double *out, int32_t *in_domain, double *jacobian) {
+  if (!c) return KD_ERR_ARG;
+  int rc = need_integrand(c);
+  if (rc) return rc;
+  if (c->obs != KD_OBS_KERNEL) return fail(c, KD_ERR_STATE, "kernel integrand not set (kd_set_kernel_integrand)");
+  if (n_points <= 0) return n_points == 0 ? KD_OK : fail(c, KD_ERR_ARG, "negative point count");
+  if (!pts || !out) return fail(c, KD_ERR_ARG, "null buffer");
+  const int64_t vpp = vals_per_point(c);
+  if (n_points > (int64_t(1) << 26) / vpp) return fail(c, KD_ERR_ARG, "too many entries for one call");
+  const size_t np = (size_t)n_points * (size_t)c->order;
+  const size_t nv = 2 * (size_t)n_points * (size_t)vpp;
+  const size_t bytes = sizeof(double) * (np + nv + (size_t)n_points) + sizeof(int32_t) * (size_t)n_points + 64;
+  KD_HIP(c, hipSetDevice(c->device));
+  if ((rc = ensure_stage(c, bytes))) return rc;
+  double *dp = (double *)c->d_stage;
+  double *dout = dp + np;
+  double *djac = dout + nv;
+  int32_t *ddom = (int32_t *)(djac + n_points);
+  KD_HIP(c, hipMemcpyAsync(dp, pts, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+  KD_HIP(c, hipMemsetAsync(dout, 0, sizeof(double) * nv, c->stream));
+  kd::KdParams p = base_params(c);
+  p.mode = warped ? kd::MODE_EVAL_L : kd::MODE_EVAL_U;
+  p.keep_u = keep_u_hypercube ? 1 : 0;
+  p.in_pts = dp;
+  p.n_points = n_points;
+  p.out_vals = dout;
+  p.out_dom = ddom;
+  p.out_jac = djac;
+  if ((rc = launch_timed(c, p, 0))) return rc;
+  KD_HIP(c, hipMemcpyAsync(out, dout, sizeof(double) * nv, hipMemcpyDeviceToHost, c->stream));
+  if (in_domain)
+    KD_HIP(c, hipMemcpyAsync(in_domain, ddom, sizeof(int32_t) * n_points, hipMemcpyDeviceToHost, c->stream));
+  if (jacobian)
+    KD_HIP(c, hipMemcpyAsync(jacobian, djac, sizeof(double) * n_points, hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  return finish_timing(c);
 }
 
 } // extern "C"

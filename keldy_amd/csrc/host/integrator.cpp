@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 namespace keldy {
 
@@ -21,6 +22,28 @@ void device_ctx::check(int rc) const {
 }
 
 namespace binner {
+std::vector<double> binner_1d_1_t::get_bin_coord() const {
+  std::vector<double> c(static_cast<size_t>(nr_bins));
+  for (long i = 0; i < nr_bins; i++) c[size_t(i)] = xmin + (double(i) + 0.5) * get_bin_size();
+  return c;
+}
+// entries at equal coordinates merged first, in order of appearance (binner.hpp:95-114)
+double sparse_binner_1d_1_t::sum_moduli() const {
+  std::map<std::pair<double, long>, size_t> where;
+  std::vector<dcomplex> merged;
+  for (auto const &d : data) {
+    auto it = where.find(d.first);
+    if (it == where.end()) {
+      where.emplace(d.first, merged.size());
+      merged.push_back(d.second);
+    } else {
+      merged[it->second] += d.second;
+    }
+  }
+  double s = 0.0;
+  for (auto const &v : merged) s += std::abs(v);
+  return s;
+}
 std::vector<double> binner_1d_t::get_bin_coord() const {
   std::vector<double> c(static_cast<size_t>(nr_bins));
   for (long i = 0; i < nr_bins; i++) c[size_t(i)] = xmin + (double(i) + 0.5) * get_bin_size();
@@ -57,13 +80,46 @@ integrand_g_direct::integrand_g_direct(g0_keldysh_contour_t g0_in, gf_index_t A,
     throw keldy_error("Need g0_model with make_dot_lead == true to calculate off_diagonal gf.");
 }
 
+integrand_g_direct integrand_g_direct::kernel_method(g0_keldysh_contour_t g0_in, gf_index_t X, int obs,
+                                                     double omega) {
+  integrand_g_direct f(std::move(g0_in), X, X, 0.0);
+  f.obs_ = obs;
+  f.omega_ = omega;
+  return f;
+}
+
 void integrand_g_direct::bind(device_ctx const &ctx, int order) const {
   const g0_model &m = g0_.model;
   ctx.check(kd_set_g0_table(ctx.get(), reinterpret_cast<const double *>(m.lesser.data()),
                             reinterpret_cast<const double *>(m.greater.data()), m.n_orb, m.n_pts, m.time_max,
                             m.model_omega.get_param_alpha()));
   const kd_gf_index a = to_kd(external_A), b = to_kd(external_B);
-  ctx.check(kd_set_integrand(ctx.get(), order, &a, &b, cutoff));
+  if (obs_ == KD_OBS_KERNEL)
+    ctx.check(kd_set_kernel_integrand(ctx.get(), order, &a));
+  else if (obs_ == KD_OBS_KERNEL_OMEGA)
+    ctx.check(kd_set_kernel_omega_integrand(ctx.get(), order, &a, omega_));
+  else
+    ctx.check(kd_set_integrand(ctx.get(), order, &a, &b, cutoff));
+}
+
+void integrand_g_direct::ensure_bound(int order) const {
+  if (order > KD_MAX_ORDER)
+    throw keldy_error("order " + std::to_string(order) + " exceeds the device maximum " + std::to_string(KD_MAX_ORDER));
+  if (obs_ != KD_OBS_DIRECT && order < 1) throw keldy_error("Order 0 is not implemented");
+  if (!ctx_) ctx_ = std::make_shared<device_ctx>();
+  if (bound_order_ != order) {
+    bind(*ctx_, order);
+    bound_order_ = order;
+  }
+}
+
+void integrand_g_direct::eval_entries(const double *u, long n, int order, bool keep_u_hypercube, dcomplex *out,
+                                      int *in_domain) const {
+  if (obs_ != KD_OBS_KERNEL) throw keldy_error("entries exist for the kernel method only");
+  ensure_bound(order);
+  ctx_->check(kd_eval_kernel_batch(ctx_->get(), u, n, 0, keep_u_hypercube ? 1 : 0, reinterpret_cast<double *>(out),
+                                   in_domain, nullptr));
+  check_window(out, n * (long(1) << order) * order);
 }
 
 // The device yields NaN for g0 lookups outside the tabulated window; the
@@ -76,13 +132,8 @@ void check_window(const dcomplex *out, long n) {
 
 void integrand_g_direct::eval_batch(const double *u, long n, int order, bool keep_u_hypercube, dcomplex *out,
                                     int *in_domain) const {
-  if (order > KD_MAX_ORDER)
-    throw keldy_error("order " + std::to_string(order) + " exceeds the device maximum " + std::to_string(KD_MAX_ORDER));
-  if (!ctx_) ctx_ = std::make_shared<device_ctx>();
-  if (bound_order_ != order) {
-    bind(*ctx_, order);
-    bound_order_ = order;
-  }
+  if (obs_ == KD_OBS_KERNEL) throw keldy_error("the binned kernel method has entries, not one value per point");
+  ensure_bound(order);
   ctx_->check(kd_eval_batch(ctx_->get(), u, n, keep_u_hypercube ? 1 : 0, reinterpret_cast<double *>(out),
                             in_domain));
   check_window(out, n);
@@ -396,3 +447,93 @@ compute_current_J_direct::compute_current_J_direct(g0_model model, double time, 
 
 } // namespace impurity_oneband
 } // namespace keldy
+
+namespace keldy::impurity_oneband {
+
+// --------------------------------------------------------------------------
+// kernel method
+namespace {
+binner::sparse_binner_1d_1_t entries_to_sparse(const dcomplex *e, int order, std::vector<double> const &times) {
+  binner::sparse_binner_1d_1_t r;
+  const long nc = long(1) << order;
+  r.data.reserve(size_t(nc * order));
+  for (long c = 0; c < nc; c++)
+    for (int i = 0; i < order; i++) r.accumulate(e[c * order + i], times[size_t(i)], (c >> i) & 1);
+  return r;
+}
+} // namespace
+
+std::pair<binner::sparse_binner_1d_1_t, int> integrand_g_kernel::operator()(std::vector<double> const &times,
+                                                                           bool keep_u_hypercube) const {
+  const int n = int(times.size());
+  if (n == 0) throw keldy_error("Order 0 is not implemented");
+  std::vector<dcomplex> e(size_t(n) << n);
+  int dom = 0;
+  eval_entries(times.data(), 1, n, keep_u_hypercube, e.data(), &dom);
+  if (!dom) return {binner::sparse_binner_1d_1_t{}, 0};
+  return {entries_to_sparse(e.data(), n, times), 1};
+}
+
+compute_gf_kernel::compute_gf_kernel(g0_model model, double time, int order, int nr_bins)
+   : integrator_direct(integrand_g_direct::kernel_method(g0_keldysh_contour_t(model), gf_index_t(time, up, forward),
+                                                         KD_OBS_KERNEL),
+                       order, "sobol_unshifted", 0),
+     kernel(g0_keldysh_contour_t(std::move(model)), gf_index_t(time, up, forward)), time_(time), nr_bins_(nr_bins) {
+  if (nr_bins < 1) throw keldy_error("nr_bins must be >= 1");
+  if (!(0.0 < time))
+    throw keldy_error("Boundaries of binning range are in wrong order (0 >= " + std::to_string(time) + ")");
+  ctx_.check(kd_set_kernel_bins(ctx_.get(), 0.0, time, nr_bins));
+}
+
+binner::binner_1d_1_t compute_gf_kernel::reduce_result_binned() const {
+  binner::binner_1d_1_t b;
+  b.xmin = 0.0;
+  b.xmax = time_;
+  b.nr_bins = nr_bins_;
+  std::vector<double> v(4 * size_t(nr_bins_));
+  std::vector<uint64_t> cnt(2 * size_t(nr_bins_));
+  uint64_t dropped = 0;
+  ctx_.check(kd_get_kernel_bins(ctx_.get(), v.data(), cnt.data(), &dropped));
+  const double inv = n_points_ ? 1.0 / double(n_points_) : 0.0;
+  for (size_t i = 0; i < 2 * size_t(nr_bins_); i++) {
+    b.data.emplace_back(v[2 * i] * inv, v[2 * i + 1] * inv);
+    b.nr_values_added.push_back(long(cnt[i]));
+  }
+  b.nr_values_dropped = long(dropped);
+  return b;
+}
+
+void compute_gf_kernel::evaluate_warped_entries_batch(const double *l, long n, dcomplex *out, int *in_domain,
+                                                      double *jac, bool keep_u_hypercube) {
+  sync_warper();
+  ctx_.check(kd_eval_kernel_batch(ctx_.get(), l, n, 1, keep_u_hypercube ? 1 : 0, reinterpret_cast<double *>(out),
+                                  in_domain, jac));
+  check_window(out, n * (long(1) << dimension) * dimension);
+}
+
+std::pair<binner::sparse_binner_1d_1_t, double>
+compute_gf_kernel::evaluate_warped_integrand_kernel(std::vector<double> const &li_vec, int start_domain_nr,
+                                                    bool keep_u_hypercube) {
+  const int n = dimension;
+  if (int(li_vec.size()) != n) throw keldy_error("point dimension differs from the integrand order");
+  std::vector<dcomplex> e(size_t(n) << n);
+  int dom = 0;
+  auto [ui, jac] = warper.map_reverse(li_vec, start_domain_nr, 0);
+  if (start_domain_nr == int(warper.size())) {
+    double j = 0;
+    evaluate_warped_entries_batch(li_vec.data(), 1, e.data(), &dom, &j, keep_u_hypercube);
+  } else {
+    kernel.eval_entries(ui.data(), 1, n, keep_u_hypercube, e.data(), &dom);
+    for (auto &x : e) x *= jac;
+  }
+  if (!dom) return {binner::sparse_binner_1d_1_t{}, 0.0};
+  return {entries_to_sparse(e.data(), n, ui), 1.0};
+}
+
+compute_gf_kernel_single_omega::compute_gf_kernel_single_omega(g0_model model, double time, double omega, int order)
+   : integrator_direct(integrand_g_direct::kernel_method(g0_keldysh_contour_t(model), gf_index_t(time, up, forward),
+                                                         KD_OBS_KERNEL_OMEGA, omega),
+                       order, "sobol_unshifted", 0),
+     kernel(g0_keldysh_contour_t(std::move(model)), gf_index_t(time, up, forward), omega) {}
+
+} // namespace keldy::impurity_oneband

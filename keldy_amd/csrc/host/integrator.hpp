@@ -49,6 +49,27 @@ struct sparse_binner_1d_t {
   double sum_moduli() const;
 };
 
+/// binner_t<1,1> (binner.hpp:147-352) with one continuous axis [xmin, xmax]
+/// (nr_bins bins, xmax in the last one) and one discrete axis of 2 bins (the
+/// Keldysh index): data and counts [nr_bins][2], row-major.
+struct binner_1d_1_t {
+  double xmin = 0.0, xmax = 1.0;
+  long nr_bins = 1;
+  long nr_disc = 2;
+  std::vector<dcomplex> data;
+  std::vector<long> nr_values_added;
+  long nr_values_dropped = 0;
+  double get_bin_size() const { return (xmax - xmin) / double(nr_bins); }
+  std::vector<double> get_bin_coord() const;
+};
+
+/// Sparse binner (binner.hpp:59-140, sparse_binner_t<1,1>): ((time, index), value).
+struct sparse_binner_1d_1_t {
+  std::vector<std::pair<std::pair<double, long>, dcomplex>> data;
+  void accumulate(dcomplex value, double x, long k) { data.push_back({{x, k}, value}); }
+  double sum_moduli() const;
+};
+
 } // namespace binner
 
 namespace impurity_oneband {
@@ -70,6 +91,16 @@ class integrand_g_direct {
   /// closed forms of orders 0-2 by explicit Wick contractions (wick_direct.cpp:161-228)
   std::pair<result_t, int> eval_no_det(std::vector<double> const &times, bool keep_u_hypercube = true) const;
 
+  /// The kernel method on the same device path (wick_kernel.hpp:43-60,
+  /// wick_kernel_single_omega.hpp:45-58): obs = KD_OBS_KERNEL (entries per
+  /// configuration and vertex) or KD_OBS_KERNEL_OMEGA (their phase-weighted
+  /// sum at omega), with external point X.
+  static integrand_g_direct kernel_method(g0_keldysh_contour_t g0_, gf_index_t X, int obs, double omega = 0.0);
+  /// KD_OBS_KERNEL: entries [n][2^order][order] at vertex times u [n][order]
+  void eval_entries(const double *u, long n, int order, bool keep_u_hypercube, dcomplex *out, int *in_domain) const;
+  int observable() const { return obs_; }
+  double omega() const { return omega_; }
+
   g0_keldysh_contour_t const &g0() const { return g0_; }
   gf_index_t const &A() const { return external_A; }
   gf_index_t const &B() const { return external_B; }
@@ -83,6 +114,44 @@ class integrand_g_direct {
   double cutoff;
   mutable std::shared_ptr<device_ctx> ctx_; // lazily created for per-point calls
   mutable int bound_order_ = -1;
+  int obs_ = KD_OBS_DIRECT;
+  double omega_ = 0.0;
+  void ensure_bound(int order) const;
+};
+
+/// Kernel method integrand (wick_kernel.hpp:43-60): per configuration and
+/// vertex, the cofactor entry at (vertex time, Keldysh index).  On the GPU.
+class integrand_g_kernel {
+ public:
+  using result_t = binner::sparse_binner_1d_1_t;
+  integrand_g_kernel(g0_keldysh_contour_t g0_, gf_index_t g_idx_X_)
+     : base(integrand_g_direct::kernel_method(std::move(g0_), g_idx_X_, KD_OBS_KERNEL)) {}
+  /// the reference's sparse binner: 2^n n entries in its accumulate order
+  std::pair<result_t, int> operator()(std::vector<double> const &times, bool keep_u_hypercube = true) const;
+  void eval_entries(const double *u, long n, int order, bool keep_u_hypercube, dcomplex *out, int *in_domain) const {
+    base.eval_entries(u, n, order, keep_u_hypercube, out, in_domain);
+  }
+  integrand_g_direct const &device() const { return base; }
+
+ private:
+  integrand_g_direct base;
+};
+
+/// Kernel method at one frequency (wick_kernel_single_omega.hpp:45-58).
+class integrand_g_kernel_single_omega {
+ public:
+  using result_t = dcomplex;
+  integrand_g_kernel_single_omega(g0_keldysh_contour_t g0_, gf_index_t g_idx_X_, double omega_)
+     : base(integrand_g_direct::kernel_method(std::move(g0_), g_idx_X_, KD_OBS_KERNEL_OMEGA, omega_)) {}
+  std::pair<result_t, int> operator()(std::vector<double> const &times, bool keep_u_hypercube = true) const {
+    return base(times, keep_u_hypercube);
+  }
+  void eval_batch(const double *u, long n, int order, bool keep_u_hypercube, dcomplex *out, int *in_domain) const {
+    base.eval_batch(u, n, order, keep_u_hypercube, out, in_domain);
+  }
+
+ private:
+  integrand_g_direct base;
 };
 
 /// QMC driver: Sobol points -> warper chain -> integrand -> accumulation,
@@ -191,6 +260,41 @@ class compute_charge_Q_direct_time : public integrator_direct {
   integrand_g_direct_time integrand_time;
   double time_;
   int nr_time_slices_;
+};
+
+/// Kernel method: G(X) kernel binned by vertex time and Keldysh index
+/// (compute_obs.hpp:105-117): binner_t<1,1> over [0, time] x {forward, backward}.
+class compute_gf_kernel : public integrator_direct {
+ public:
+  compute_gf_kernel(g0_model model, double time, int order, int nr_bins = 100);
+  compute_gf_kernel(model_param_t params, double time, int order, int nr_bins = 100)
+     : compute_gf_kernel(g0_model{g0_model_omega{params}, false}, time, order, nr_bins) {}
+  /// binner_t<1,1> of the estimate: bin sums divided by the points run
+  binner::binner_1d_1_t reduce_result_binned() const;
+  integrand_g_kernel get_integrand_kernel() const { return kernel; }
+  std::pair<binner::sparse_binner_1d_1_t, double> evaluate_warped_integrand_kernel(std::vector<double> const &li_vec,
+                                                                                  int start_domain_nr,
+                                                                                  bool keep_u_hypercube = true);
+  /// entries [n][2^order][order], in-domain flags and Jacobians of the full chain at l [n][order]
+  void evaluate_warped_entries_batch(const double *l, long n, dcomplex *out, int *in_domain, double *jac,
+                                     bool keep_u_hypercube = true);
+
+ private:
+  integrand_g_kernel kernel;
+  double time_;
+  int nr_bins_;
+};
+
+/// Kernel method at one frequency (compute_obs.hpp:120-133).
+class compute_gf_kernel_single_omega : public integrator_direct {
+ public:
+  compute_gf_kernel_single_omega(g0_model model, double time, double omega, int order);
+  compute_gf_kernel_single_omega(model_param_t params, double time, double omega, int order)
+     : compute_gf_kernel_single_omega(g0_model{g0_model_omega{params}, false}, time, omega, order) {}
+  integrand_g_kernel_single_omega get_integrand_omega() const { return kernel; }
+
+ private:
+  integrand_g_kernel_single_omega kernel;
 };
 
 /// Several integrators (e.g. orders 1..N with uneven point counts) run as one

@@ -78,7 +78,15 @@ struct KdParams {
   double bin_min, bin_max, bin_size;
   double2 *bin_part;
   unsigned long long *bin_counts;
+  // observable: OBS_DIRECT (the Keldysh sum of integrand_g_direct), or the
+  // kernel method with external point B (OBS_KERNEL: per-vertex cofactor sums
+  // binned by vertex time and Keldysh index, bins [nbins][2]; OBS_KERNEL_OMEGA:
+  // the same entries summed with the phase exp(i omega t))
+  int obs;
+  double omega;
 };
+
+enum { OBS_DIRECT = 0, OBS_KERNEL = 1, OBS_KERNEL_OMEGA = 2 };
 
 // Bin of coordinate x (binner.hpp coord2index / in_bounds): -1 when outside
 // [xmin, xmax]; x == xmax falls in the last bin.

@@ -27,7 +27,7 @@ namespace kd {
 
 // det(A)^2 * s for one Keldysh configuration, unpivoted row-wise LU in T.
 template <int N, typename T>
-__device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, int cfg, bool &bad) {
+__device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, int cfg, T sc, bool &bad) {
   constexpr int D = 2 * N + 1;
   int pr[N + 1], pc[N + 1];
   static_for<0, N>([&](auto I) {
@@ -55,7 +55,7 @@ __device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, 
       }
     });
     T diag0 = T(0);
-    if constexpr (i < N) diag0 = cabs1<T>(row[i]);
+    if constexpr (i < N) diag0 = fmax(cabs1<T>(row[i]), sc);
     constexpr int KE = i < N ? i : N;
     static_for<0, KE>([&](auto K) {
       constexpr int k = K;
@@ -201,13 +201,22 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
         for (int i = 0; i < N; i++) dom = dom && !(x[i] < 0.0);
       }
     }
-    if (p.nbins > 0 && valid) {
+    if (p.nbins > 0 && valid && p.obs == OBS_DIRECT) {
       // binner accumulate(value or 0, min time): one count per point
       double tmin = x[0];
 #pragma unroll
       for (int i = 1; i < N; i++) tmin = fmin(tmin, x[i]);
       const int bi = bin_index(p, tmin);
       atomicAdd(p.bin_counts + (bi >= 0 ? bi : p.nbins), 1ull);
+    }
+    if (p.nbins > 0 && dom && p.obs == OBS_KERNEL) {
+      // an in-domain point adds 2^N entries per vertex (2^(N-1) per Keldysh
+      // index) at that vertex's time; counted per time bin, or dropped
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const int bi = bin_index(p, x[i]);
+        atomicAdd(p.bin_counts + (bi >= 0 ? bi : p.nbins), bi >= 0 ? (1ull << (N > 0 ? N - 1 : 0)) : (1ull << N));
+      }
     }
     const uint64_t mask = __ballot(dom);
     const int64_t b = pi >> 6;
@@ -221,8 +230,11 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
     }
     if (lane == 0) buf.cnt[b] = __popcll(mask);
     if (valid && !dom && p.mode != MODE_RUN) {
-      p.out_vals[2 * pi] = 0.0;
-      p.out_vals[2 * pi + 1] = 0.0;
+      // (the kernel method's per-configuration entries are cleared by the host)
+      if (p.obs != OBS_KERNEL) {
+        p.out_vals[2 * pi] = 0.0;
+        p.out_vals[2 * pi + 1] = 0.0;
+      }
       if (p.out_dom) p.out_dom[pi] = 0;
       if (p.out_jac) p.out_jac[pi] = jac;
     }
@@ -241,6 +253,7 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
   constexpr int NU = N > 0 ? N : 1;
   __shared__ double2 W[PPW][D * D];
   __shared__ double Us[PPW][NU];
+  __shared__ unsigned long long Ssc[PPW]; // per point: max |Re|+|Im| of the vertex block (bits of a double >= 0)
   extern __shared__ double2 sbins[]; // p.nbins entries when time-binning (dynamic LDS)
 
   const int lane = threadIdx.x;
@@ -272,6 +285,7 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
       __asm__ volatile("" : "+v"(lane));
       const int64_t slot0 = b * 64 + g0i;
       if (lane < ng * N) Us[lane / NU][lane % NU] = buf.u[slot0 * N + lane];
+      if (lane < PPW) Ssc[lane] = 0ull;
       __syncthreads();
       // Stage the Wick matrices of the group's points in LDS (wick_direct.cpp:92-101).
       for (int e = lane; e < ng * D * D; e += 64) {
@@ -294,6 +308,7 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
         double2 gv = g0_eval(p, a, bb, !corner);
         if (corner && N > 0) gv = make_double2(0.0, 0.0);
         W[q][rc] = gv;
+        if (r < 2 * N && c < 2 * N) atomicMax(&Ssc[q], (unsigned long long)__double_as_longlong(fabs(gv.x) + fabs(gv.y)));
       }
       __syncthreads();
 
@@ -311,7 +326,7 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
           for (int m = 0; m < CPL; m++) {
             const int cfg = (lane % G) + G * m;
             bool bad;
-            const double2 v = config_value<N, T>(PPW == 1 ? W[0] : W[q], cfg, bad);
+            const double2 v = config_value<N, T>(PPW == 1 ? W[0] : W[q], cfg, (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), bad);
             // branch-free: a divergent branch here doubles the register footprint
             const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
             sre += bad ? 0.0 : sg * v.x;

@@ -28,6 +28,9 @@ hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, con
 // integrand_tree_kernel<order, T> (orders 6..12): the Keldysh sum by the
 // Schur-complement tree, one point per 64-thread workgroup item.
 hipError_t launch_tree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
+// cofactor_kernel<order> (kernel method, orders 1..12, FP64): grid blocks of
+// 64 threads, same work items as the Keldysh-sum kernel.
+hipError_t launch_cofactor(int order, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // acc[0..3] += sum over blocks of part[4*blk + 0..3] (fixed order).
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st);
 // per-(axis, bin) sums of |vals| and counts over n samples l [n][order] (pilot projection)

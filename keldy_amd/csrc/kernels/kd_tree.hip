@@ -52,9 +52,10 @@ template <int N> struct TreeShape {
 template <typename T> __device__ __forceinline__ cx<T> ldc(const double2 &v) { return cmk<T>((T)v.x, (T)v.y); }
 template <typename T> __device__ __forceinline__ cx<T> ldc(const float2 &v) { return cmk<T>((T)v.x, (T)v.y); }
 
-template <typename T> __device__ __forceinline__ bool bad_pivot(cx<T> piv, double2 wdiag) {
+// tabs: PivotTol times the largest |Re|+|Im| of the point's vertex block
+template <typename T> __device__ __forceinline__ bool bad_pivot(cx<T> piv, double2 wdiag, T tabs) {
   const T pm = cabs1<T>(piv);
-  return !(pm >= PivotTol<T>::v * (T)(fabs(wdiag.x) + fabs(wdiag.y))) || pm == T(0);
+  return !(pm >= PivotTol<T>::v * (T)(fabs(wdiag.x) + fabs(wdiag.y))) || !(pm >= tabs) || pm == T(0);
 }
 
 // NCH children from NPAR parent tables (Dp x Dp each, contiguous), one child
@@ -67,7 +68,7 @@ template <typename T, typename PT, int Dp, int NPAR, int NCH, int D0>
 __device__ __forceinline__ void tree_rows(const PT *__restrict__ par, cx<T> *__restrict__ ch,
                                           const double2 *__restrict__ dpar, const int *__restrict__ fpar,
                                           double2 *__restrict__ dch, int *__restrict__ fch,
-                                          const double2 *__restrict__ W, int wbase, int s_dfs, int lane) {
+                                          const double2 *__restrict__ W, int wbase, int s_dfs, int lane, T tabs) {
   constexpr int Dc = Dp - 2;
   for (int r = lane; r < NCH * Dc; r += 64) {
     const int m = r / Dc, a = r - m * Dc;
@@ -85,7 +86,7 @@ __device__ __forceinline__ void tree_rows(const PT *__restrict__ par, cx<T> *__r
       const double2 d = dpar[pm];
       dch[m] = make_double2(d.x * (double)piv.x - d.y * (double)piv.y, d.x * (double)piv.y + d.y * (double)piv.x);
       const int wi = wbase + s;
-      fch[m] = fpar[pm] | (int)bad_pivot<T>(piv, W[wi * D0 + wi]);
+      fch[m] = fpar[pm] | (int)bad_pivot<T>(piv, W[wi * D0 + wi], tabs);
     }
   }
   __syncthreads();
@@ -144,6 +145,7 @@ __global__ __launch_bounds__(64) void integrand_tree_kernel(KdParams p, PrepBuf 
   __shared__ double2 dd[A + 1], dn[4][8]; // path determinants: depth-first levels, slab levels
   __shared__ int ff[A + 1], fl[4][8];     // pivot flags, same layout
   __shared__ double Us[N];
+  __shared__ unsigned long long Ssc; // max |Re|+|Im| of the vertex block (bits of a double >= 0)
   extern __shared__ double2 sbins[];
 
   if (p.nbins > 0) {
@@ -163,6 +165,7 @@ __global__ __launch_bounds__(64) void integrand_tree_kernel(KdParams p, PrepBuf 
     __asm__ volatile("" : "+v"(lane));
     const int64_t slot = b * 64 + g0i;
     if (lane < N) Us[lane] = buf.u[slot * N + lane];
+    if (lane == 0) Ssc = 0ull;
     __syncthreads();
     // Wick matrix in level-0 order (wick_direct.cpp:92-101 picks, interleaved)
     for (int e = lane; e < D0 * D0; e += 64) {
@@ -182,8 +185,10 @@ __global__ __launch_bounds__(64) void integrand_tree_kernel(KdParams p, PrepBuf 
       double2 gv = g0_eval(p, a, bb, !corner);
       if (corner) gv = make_double2(0.0, 0.0);
       W[e] = gv;
+      if (r < 2 * N && c < 2 * N) atomicMax(&Ssc, (unsigned long long)__double_as_longlong(fabs(gv.x) + fabs(gv.y)));
     }
     __syncthreads();
+    const T tabs = (T)(PivotTol<T>::v * __longlong_as_double((long long)Ssc));
 
     double sre = 0.0, sim = 0.0;
     int nfb = 0;
@@ -200,19 +205,19 @@ __global__ __launch_bounds__(64) void integrand_tree_kernel(KdParams p, PrepBuf 
         if (k >= k0) {
           const int s = (r >> (A - 1 - k)) & 1;
           if constexpr (k == 0)
-            tree_rows<T, double2, S::Dk(0), 1, 1, D0>(W, stk + S::off(1), dd, ff, dd + 1, ff + 1, W, 0, s, lane);
+            tree_rows<T, double2, S::Dk(0), 1, 1, D0>(W, stk + S::off(1), dd, ff, dd + 1, ff + 1, W, 0, s, lane, tabs);
           else
             tree_rows<T, cx<T>, S::Dk(k), 1, 1, D0>(stk + S::off(k), stk + S::off(k + 1), dd + k, ff + k, dd + k + 1,
-                                                   ff + k + 1, W, 2 * k, s, lane);
+                                                   ff + k + 1, W, 2 * k, s, lane, tabs);
         }
       });
       // slab: levels A+1 (2 nodes), A+2 (4), A+3 = K (8), breadth first
       if constexpr (A == 0)
-        tree_rows<T, double2, DA, 1, 2, D0>(W, X, dd, ff, dn[1], fl[1], W, 0, 0, lane);
+        tree_rows<T, double2, DA, 1, 2, D0>(W, X, dd, ff, dn[1], fl[1], W, 0, 0, lane, tabs);
       else
-        tree_rows<T, cx<T>, DA, 1, 2, D0>(stk + S::off(A), X, dd + A, ff + A, dn[1], fl[1], W, 2 * A, 0, lane);
-      tree_rows<T, cx<T>, 11, 2, 4, D0>(X, Y, dn[1], fl[1], dn[2], fl[2], W, 2 * (A + 1), 0, lane);
-      tree_rows<T, cx<T>, 9, 4, 8, D0>(Y, X, dn[2], fl[2], dn[3], fl[3], W, 2 * (A + 2), 0, lane);
+        tree_rows<T, cx<T>, DA, 1, 2, D0>(stk + S::off(A), X, dd + A, ff + A, dn[1], fl[1], W, 2 * A, 0, lane, tabs);
+      tree_rows<T, cx<T>, 11, 2, 4, D0>(X, Y, dn[1], fl[1], dn[2], fl[2], W, 2 * (A + 1), 0, lane, tabs);
+      tree_rows<T, cx<T>, 9, 4, 8, D0>(Y, X, dn[2], fl[2], dn[3], fl[3], W, 2 * (A + 2), 0, lane, tabs);
       const double2 dK = dn[3][m];
       const int fK = fl[3][m];
 
@@ -246,7 +251,7 @@ __global__ __launch_bounds__(64) void integrand_tree_kernel(KdParams p, PrepBuf 
           if constexpr (i < 3) {
             const cx<T> piv = row[i];
             const int wi = 2 * (K + i) + ((bb3 >> i) & 1);
-            flag = flag || bad_pivot<T>(piv, W[wi * D0 + wi]);
+            flag = flag || bad_pivot<T>(piv, W[wi * D0 + wi], tabs);
             det = cmul<T>(det, piv);
             const cx<T> inv = cinv<T>(piv);
             static_for<i + 1, 4>([&](auto J) {

@@ -19,8 +19,15 @@ ComputeChargeQDirectTime = _m.ComputeChargeQDirectTime
 IntegrandGDirectTime = _m.IntegrandGDirectTime
 Binner1 = _m.Binner1
 SparseBinner1 = _m.SparseBinner1
+IntegrandGKernel = _m.IntegrandGKernel
+IntegrandGKernelSingleOmega = _m.IntegrandGKernelSingleOmega
+ComputeGfKernel = _m.ComputeGfKernel
+ComputeGfKernelSingleOmega = _m.ComputeGfKernelSingleOmega
+Binner_1_1 = _m.Binner_1_1
+SparseBinner_1_1 = _m.SparseBinner_1_1
 run_batch = _m.run_batch
 
 __all__ = ["GfIndexT", "G0ModelOmega", "G0Model", "G0KeldyshContourT", "IntegrandGDirect",
            "ComputeChargeQDirect", "ComputeCurrentJDirect", "IntegratorDirect", "ComputeChargeQDirectTime",
-           "IntegrandGDirectTime", "Binner1", "SparseBinner1", "run_batch"]
+           "IntegrandGDirectTime", "Binner1", "SparseBinner1", "IntegrandGKernel", "IntegrandGKernelSingleOmega",
+           "ComputeGfKernel", "ComputeGfKernelSingleOmega", "Binner_1_1", "SparseBinner_1_1", "run_batch"]

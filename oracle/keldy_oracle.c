@@ -645,3 +645,199 @@ int ko_run_parallel(const ko_integrand *f, const ko_warper *w, int nw, const ko_
   free(part);
   return err ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* kernel method                                                             */
+/* ------------------------------------------------------------------------ */
+/* first_row_expansion (det_expansion.hpp:44-74): LU with partial pivoting
+ * (LAPACK zgetf2's rule: the row of largest |Re|+|Im|, first on ties), the
+ * determinant from the pivots and interchanges, then the solve A x = det e_1
+ * (zgetrs: interchanges on the right-hand side, unit-lower then upper
+ * triangular substitution).  x = first row of the comatrix of A.  Returns
+ * -1 where zgetrf reports an exactly singular U (the reference throws). */
+int ko_first_row_expansion(double *ad, int m, double *xd, double det_out[2]) {
+  cplx *a = (cplx *)ad, *x = (cplx *)xd;
+  int piv[KO_MAX_DIM + 1];
+  cplx det = 1.0;
+  int sign = 1;
+  for (int k = 0; k < m; k++) {
+    int p = k;
+    double best = fabs(creal(a[k * m + k])) + fabs(cimag(a[k * m + k]));
+    for (int r = k + 1; r < m; r++) {
+      double v = fabs(creal(a[r * m + k])) + fabs(cimag(a[r * m + k]));
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    piv[k] = p;
+    if (a[p * m + k] == 0.0) return -1;
+    if (p != k) {
+      for (int j = 0; j < m; j++) {
+        cplx t = a[k * m + j];
+        a[k * m + j] = a[p * m + j];
+        a[p * m + j] = t;
+      }
+      sign = -sign;
+    }
+    cplx inv = 1.0 / a[k * m + k];
+    for (int r = k + 1; r < m; r++) {
+      a[r * m + k] *= inv; /* L multiplier, kept for the solve */
+      for (int j = k + 1; j < m; j++) a[r * m + j] -= a[r * m + k] * a[k * m + j];
+    }
+    det *= a[k * m + k];
+  }
+  det *= sign;
+  for (int i = 0; i < m; i++) x[i] = 0.0;
+  x[0] = det;
+  for (int k = 0; k < m; k++)
+    if (piv[k] != k) {
+      cplx t = x[k];
+      x[k] = x[piv[k]];
+      x[piv[k]] = t;
+    }
+  for (int i = 1; i < m; i++)
+    for (int j = 0; j < i; j++) x[i] -= a[i * m + j] * x[j];
+  for (int i = m - 1; i >= 0; i--) {
+    for (int j = i + 1; j < m; j++) x[i] -= a[i * m + j] * x[j];
+    x[i] /= a[i * m + i];
+  }
+  det_out[0] = creal(det);
+  det_out[1] = cimag(det);
+  return 0;
+}
+
+/* integrand_g_kernel::operator() (wick_kernel.cpp:42-138).  out: the 2^n x n
+ * entries of the reference's sparse binner in its accumulate order
+ * (configuration-major, vertex-minor), entry (c, i) at vertex time times[i]
+ * and Keldysh index bit i of c; values are complex (re, im).  Returns the
+ * in-domain flag, or -1 where the reference's LAPACK call would throw. */
+int ko_kernel_eval(const ko_g0 *g, const ko_gf_index *X, int n, const double *times, int keep_u_hypercube,
+                   double *out) {
+  uint64_t nconf = (uint64_t)1 << n;
+  for (uint64_t e = 0; e < 2 * nconf * (uint64_t)n; e++) out[e] = 0.0;
+  if (keep_u_hypercube)
+    for (int i = 0; i < n; i++)
+      if (times[i] < 0.0) return 0;
+  if (n == 0) return 1;
+  ko_gf_index a = *X, b = *X;
+  a.timesplit = n;
+  b.timesplit = n;
+  int n2 = 2 * n, ext = 2 * n, d1 = 2 * n + 1;
+  ko_gf_index c1[2 * KO_MAX_DIM], c2[2 * KO_MAX_DIM];
+  for (int i = 0; i < n; i++) {
+    ko_gf_index v = {times[i], 0, i, a.spin, 0};
+    c1[i] = v;
+    v.k_idx = 1;
+    c1[i + n] = v;
+    v.spin = 1 - a.spin;
+    v.k_idx = 0;
+    c2[i] = v;
+    v.k_idx = 1;
+    c2[i + n] = v;
+  }
+  cplx *W1 = (cplx *)malloc(sizeof(cplx) * d1 * d1);
+  cplx *W2 = (cplx *)malloc(sizeof(cplx) * n2 * n2);
+  W1[ext * d1 + ext] = g0c(g, &a, &b, 0);
+  for (int i = 0; i < n2; i++) {
+    W1[ext * d1 + i] = g0c(g, &a, &c1[i], 1);
+    W1[i * d1 + ext] = g0c(g, &c1[i], &b, 1);
+    for (int j = 0; j < n2; j++) {
+      W1[i * d1 + j] = g0c(g, &c1[i], &c1[j], 1);
+      W2[i * n2 + j] = g0c(g, &c2[i], &c2[j], 1);
+    }
+  }
+  cplx *M1 = (cplx *)malloc(sizeof(cplx) * (n + 1) * (n + 1));
+  cplx *M2 = (cplx *)malloc(sizeof(cplx) * n * n);
+  cplx x[KO_MAX_DIM + 1];
+  int pick[KO_MAX_DIM + 1], rc = 1;
+  for (uint64_t idx = 0; idx < nconf && rc == 1; idx++) {
+    /* external point first, then the vertices at their Keldysh indices */
+    pick[0] = ext;
+    for (int i = 0; i < n; i++) pick[i + 1] = i + (int)((idx >> i) & 1u) * n;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) M2[i * n + j] = W2[pick[i + 1] * n2 + pick[j + 1]];
+    for (int i = 0; i <= n; i++)
+      for (int j = 0; j <= n; j++) M1[i * (n + 1) + j] = W1[pick[i] * d1 + pick[j]];
+    M1[n * (n + 1) + n] = 0.0; /* as the reference: element (n, n) set to zero */
+    double dm[2], d2v[2];
+    if (ko_first_row_expansion((double *)M1, n + 1, (double *)x, dm) != 0) {
+      rc = -1;
+      break;
+    }
+    ko_det((double *)M2, n, d2v);
+    cplx f = (double)(1 - 2 * __builtin_parityll(idx)) * (d2v[0] + I * d2v[1]);
+    for (int i = 1; i <= n; i++) {
+      cplx v = x[i] * f;
+      out[2 * (idx * (uint64_t)n + (uint64_t)(i - 1))] = creal(v);
+      out[2 * (idx * (uint64_t)n + (uint64_t)(i - 1)) + 1] = cimag(v);
+    }
+  }
+  free(W1);
+  free(W2);
+  free(M1);
+  free(M2);
+  return rc;
+}
+
+/* integrand_g_kernel_single_omega::operator() (wick_kernel_single_omega.cpp:41-140):
+ * the kernel entries summed with the phase exp(i omega t) of their vertex time. */
+int ko_kernel_omega_eval(const ko_g0 *g, const ko_gf_index *X, int n, const double *times, int keep_u_hypercube,
+                         double omega, double out[2]) {
+  out[0] = out[1] = 0.0;
+  uint64_t m = ((uint64_t)1 << n) * (uint64_t)n;
+  double *e = (double *)malloc(sizeof(double) * 2 * (m ? m : 1));
+  int dom = ko_kernel_eval(g, X, n, times, keep_u_hypercube, e);
+  cplx s = 0.0;
+  for (uint64_t k = 0; k < m; k++) {
+    int i = (int)(k % (uint64_t)n);
+    s += (e[2 * k] + I * e[2 * k + 1]) * cexp(I * omega * times[i]);
+  }
+  free(e);
+  out[0] = creal(s);
+  out[1] = cimag(s);
+  return dom;
+}
+
+/* integrator::run (integrator.hpp:81-95) of compute_gf_kernel (compute_obs.hpp:105-117):
+ * each point's entries, times the warper Jacobian, accumulated in order into
+ * binner_t<1,1> (binner.hpp:147-352) with the time axis [0, xmax] in nbins
+ * bins and the Keldysh-index axis of 2 bins.  bins [nbins][2] complex,
+ * counts [nbins][2], dropped: values outside [0, xmax]. */
+int ko_kernel_run(const ko_g0 *g, const ko_gf_index *X, const ko_warper *w, int nw, ko_sobol *s, uint64_t n_steps,
+                  double xmax, int nbins, double *bins, uint64_t *counts, uint64_t *dropped, uint64_t *n_in_domain) {
+  int n = s->dim;
+  double l[KO_MAX_DIM], u[KO_MAX_DIM];
+  uint64_t m = ((uint64_t)1 << n) * (uint64_t)n;
+  double *e = (double *)malloc(sizeof(double) * 2 * m);
+  double bs = xmax / nbins;
+  int rc = 0;
+  for (uint64_t p = 0; p < n_steps && rc == 0; p++) {
+    if (ko_sobol_next(s, l) != 0) {
+      rc = -1;
+      break;
+    }
+    double jac = ko_warp_reverse(w, nw, n, l, u);
+    int dom = ko_kernel_eval(g, X, n, u, 1, e);
+    if (dom < 0) {
+      rc = -1;
+      break;
+    }
+    *n_in_domain += (uint64_t)dom;
+    if (!dom) continue;
+    for (uint64_t k = 0; k < m; k++) {
+      int i = (int)(k % (uint64_t)n), kk = (int)((k / (uint64_t)n >> i) & 1u);
+      double t = u[i];
+      if (!(t >= 0.0 && t <= xmax)) {
+        (*dropped)++;
+        continue;
+      }
+      int bi = t == xmax ? nbins - 1 : (int)(t / bs);
+      bins[4 * bi + 2 * kk] += e[2 * k] * jac;
+      bins[4 * bi + 2 * kk + 1] += e[2 * k + 1] * jac;
+      counts[2 * bi + kk]++;
+    }
+  }
+  free(e);
+  return rc;
+}

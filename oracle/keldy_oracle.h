@@ -125,6 +125,14 @@ int ko_integrand_eval_no_det(const ko_integrand *f, int n, const double *times,
 /* complex LU determinant with partial pivoting (LAPACK zgetf2 pivot rule) */
 void ko_det(double *a /* m*m*2, destroyed */, int m, double out[2]);
 
+/* ---- kernel method (wick_kernel.cpp:42-138, det_expansion.hpp:44-74) ---- */
+int ko_first_row_expansion(double *a /* m*m*2, destroyed */, int m, double *x /* m*2 */, double det[2]);
+/* entries [2^n][n] (re, im): configuration-major, vertex-minor */
+int ko_kernel_eval(const ko_g0 *g, const ko_gf_index *X, int n, const double *times, int keep_u_hypercube,
+                   double *out);
+int ko_kernel_omega_eval(const ko_g0 *g, const ko_gf_index *X, int n, const double *times, int keep_u_hypercube,
+                         double omega, double out[2]);
+
 /* ---- integrator (integrator.hpp:68-131) -------------------------------- */
 typedef struct {
   double re, im;
@@ -139,6 +147,11 @@ int ko_run(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint6
 /* same sum, points split over `threads` contiguous Sobol ranges (OpenMP) */
 int ko_run_parallel(const ko_integrand *f, const ko_warper *w, int nw, const ko_sobol *s0,
                     uint64_t n_steps, int threads, ko_accum *acc);
+
+/* compute_gf_kernel's run (compute_obs.hpp:105-117) into binner_t<1,1>:
+ * bins [nbins][2] (re, im), counts [nbins][2] */
+int ko_kernel_run(const ko_g0 *g, const ko_gf_index *X, const ko_warper *w, int nw, ko_sobol *s, uint64_t n_steps,
+                  double xmax, int nbins, double *bins, uint64_t *counts, uint64_t *dropped, uint64_t *n_in_domain);
 
 #ifdef __cplusplus
 }

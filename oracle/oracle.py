@@ -98,6 +98,13 @@ def lib():
                              C.c_uint64, C.POINTER(_Accum)]
         L.ko_run_parallel.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int,
                                       C.POINTER(_Sobol), C.c_uint64, C.c_int, C.POINTER(_Accum)]
+        L.ko_first_row_expansion.argtypes = [dp, C.c_int, dp, dp]
+        L.ko_kernel_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int, dp]
+        L.ko_kernel_omega_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int,
+                                           C.c_double, dp]
+        u64p = C.POINTER(C.c_uint64)
+        L.ko_kernel_run.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.POINTER(_Warper), C.c_int,
+                                    C.POINTER(_Sobol), C.c_uint64, C.c_double, C.c_int, dp, u64p, u64p, u64p]
         _lib = L
     return _lib
 
@@ -175,6 +182,13 @@ class G0Table:
 
 def gf_index(time, spin=0, k_idx=0, timesplit=0, orbital=0):
     return _GfIndex(time, k_idx, timesplit, spin, orbital)
+
+
+def g0_eval(g0, a, b, internal=True):
+    """g0_keldysh_contour_t::operator() (model.hpp:244-265) on a G0Table."""
+    o = np.zeros(2)
+    lib().ko_g0_eval(C.byref(g0.c), C.byref(a), C.byref(b), int(internal), _dp(o))
+    return complex(o[0], o[1])
 
 
 class Map1d:
@@ -362,3 +376,64 @@ def run(integrand, train, sobol, n_steps, threads=1):
     if rc != 0:
         raise RuntimeError("More sample points requested then generator can hold.")
     return complex(acc.re, acc.im), acc.n_points, acc.n_in_domain
+
+
+# --------------------------------------------------------------------------
+# kernel method (wick_kernel.cpp:42-138, wick_kernel_single_omega.cpp:41-140)
+def first_row_expansion(m):
+    """det_expansion.hpp:44-74: (first row of the comatrix, determinant)."""
+    a = np.ascontiguousarray(m, dtype=np.complex128).copy()
+    x = np.zeros(a.shape[0], dtype=np.complex128)
+    d = np.zeros(2)
+    if lib().ko_first_row_expansion(_dp(a.view(np.float64)), a.shape[0], _dp(x.view(np.float64)), _dp(d)) != 0:
+        raise RuntimeError("lapack::getrf failed")
+    return x, complex(d[0], d[1])
+
+
+class KernelIntegrand:
+    """integrand_g_kernel (wick_kernel.hpp:43-60) with fixed external point X."""
+
+    def __init__(self, g0, X):
+        self.g0 = g0
+        self.X = X
+
+    def entries(self, times, keep_u_hypercube=True):
+        """(values [2^n][n] complex, in_domain): entry (c, i) belongs to vertex
+        time times[i] and Keldysh index bit i of c."""
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        n = len(t)
+        out = np.zeros((1 << n) * n * 2)
+        dom = lib().ko_kernel_eval(C.byref(self.g0.c), C.byref(self.X), n, _dp(t), int(keep_u_hypercube), _dp(out))
+        if dom < 0:
+            raise RuntimeError("lapack::getrf failed")
+        return out.view(np.complex128).reshape(1 << n, n), dom
+
+    def __call__(self, times, keep_u_hypercube=True):
+        """The reference's sparse binner: [((time, k_idx), value)] in accumulate order."""
+        vals, dom = self.entries(times, keep_u_hypercube)
+        if not dom:
+            return [], 0
+        n = len(times)
+        return [((float(times[i]), (c >> i) & 1), complex(vals[c, i])) for c in range(1 << n) for i in range(n)], dom
+
+    def omega(self, times, omega, keep_u_hypercube=True):
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        out = np.zeros(2)
+        dom = lib().ko_kernel_omega_eval(C.byref(self.g0.c), C.byref(self.X), len(t), _dp(t), int(keep_u_hypercube),
+                                         float(omega), _dp(out))
+        return complex(out[0], out[1]), dom
+
+
+def kernel_run(kintegrand, train, sobol, n_steps, xmax, nbins):
+    """compute_gf_kernel's run (compute_obs.hpp:105-117) into binner_t<1,1>:
+    (bin sums [nbins][2] complex, counts [nbins][2], dropped, n_in_domain)."""
+    bins = np.zeros(nbins * 4)
+    counts = np.zeros(nbins * 2, dtype=np.uint64)
+    dropped, ndom = C.c_uint64(0), C.c_uint64(0)
+    u64p = C.POINTER(C.c_uint64)
+    rc = lib().ko_kernel_run(C.byref(kintegrand.g0.c), C.byref(kintegrand.X), train.arr, len(train.warpers),
+                             C.byref(sobol.s), n_steps, float(xmax), int(nbins), _dp(bins),
+                             counts.ctypes.data_as(u64p), C.byref(dropped), C.byref(ndom))
+    if rc != 0:
+        raise RuntimeError("kernel run failed (generator exhausted or singular matrix)")
+    return bins.view(np.complex128).reshape(nbins, 2), counts.reshape(nbins, 2), dropped.value, ndom.value

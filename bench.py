@@ -208,7 +208,7 @@ def main():
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-alphas", type=float, nargs="+", default=[1.0, 0.5, 0.25],
                     help="weight powers of the flow candidates (density ~ |f|^alpha)")
-    ap.add_argument("--algorithm", default=None, choices=["config", "tree"],
+    ap.add_argument("--algorithm", default=None, choices=["config", "pair", "tree"],
                     help="Keldysh-sum algorithm (default: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters), then exit")
@@ -318,7 +318,7 @@ def main():
     launches_per_step = max(1, -(-args.points // (1 << 21)))
     algo = comp.get_algorithm()
     kname = (f"integrand_{'tree_' if algo == 'tree' else ''}kernel"
-             f"<{args.order},{'double' if args.precision == 'fp64' else 'float'}>")
+             f"<{args.order},{'double' if args.precision == 'fp64' else 'float'}{',pair' if algo == 'pair' else ''}>")
     traffic = pmc_traffic(kname, min(args.points, 1 << 21))
 
     estimator = {"plain": summary(plain), "setup_s": setup_s}

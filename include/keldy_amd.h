@@ -59,6 +59,11 @@ extern "C" {
  * them.  Same arithmetic up to reassociation (FP64 results agree to ~1e-13). */
 #define KD_ALGO_CONFIG 0
 #define KD_ALGO_TREE 1
+/* KD_ALGO_PAIR (orders >= 7 at 64 lanes per point; otherwise KD_ALGO_CONFIG):
+ * the per-configuration LU with the two Keldysh indices of the last vertex
+ * sharing the elimination of the other vertex rows -- a third fewer
+ * multiply-adds at order 8, same arithmetic per configuration. */
+#define KD_ALGO_PAIR 2
 
 typedef struct kd_ctx kd_ctx;
 

@@ -155,16 +155,18 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               v = KD_ALGO_TREE;
             else if (a == "config")
               v = KD_ALGO_CONFIG;
+            else if (a == "pair")
+              v = KD_ALGO_PAIR;
             else
-              throw keldy_error("algorithm must be 'tree' or 'config'");
+              throw keldy_error("algorithm must be 'config', 'pair' or 'tree'");
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
-          py::arg("algorithm"), "Keldysh-sum algorithm: 'config' (default) or 'tree' (orders >= 6)")
+          py::arg("algorithm"), "Keldysh-sum algorithm: 'config' (default), 'pair' (orders >= 7) or 'tree' (orders >= 6)")
       .def("get_algorithm",
            [](T &s) {
              int32_t v = KD_ALGO_CONFIG;
              s.ctx().check(kd_get_algorithm(s.ctx().get(), &v));
-             return std::string(v == KD_ALGO_TREE ? "tree" : "config");
+             return std::string(v == KD_ALGO_TREE ? "tree" : v == KD_ALGO_PAIR ? "pair" : "config");
            })
       .def("last_timing", &T::last_timing)
       .def("flops_per_point", &T::flops_per_point)

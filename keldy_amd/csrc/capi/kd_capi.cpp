@@ -118,6 +118,10 @@ int ensure_stage(kd_ctx *c, size_t bytes) {
 }
 
 bool use_tree(const kd_ctx *c) { return c->algo == KD_ALGO_TREE && c->order >= 6 && c->obs == KD_OBS_DIRECT; }
+// the paired per-configuration LU (orders >= 7 at 64 lanes per point)
+bool use_pair(const kd_ctx *c) {
+  return c->algo == KD_ALGO_PAIR && c->order >= 7 && c->obs == KD_OBS_DIRECT && c->lu_lanes_max == 64;
+}
 
 // complex values per point of an evaluation output
 int64_t vals_per_point(const kd_ctx *c) {
@@ -332,7 +336,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
     else if (use_tree(c))
       KD_HIP(c, kd::launch_tree(N, c->precision, p, buf, grid, c->stream));
     else
-      KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, p, buf, grid, c->stream));
+      KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, use_pair(c), p, buf, grid, c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
     launches += 2;
     if (p.mode == kd::MODE_RUN && !c->no_reduce) {
@@ -785,6 +789,29 @@ static double cofactor_flops(int N) {
   return std::ldexp(per_cfg, N);
 }
 
+// Executed flops per point of the paired LU (config_value_pair), per pair of
+// configurations: the N-1 shared vertex rows over N+2 columns and the external
+// row (complex multiply-add 8, multiply 6, reciprocal 7), the last vertex row
+// and the corner per Keldysh index, det^2 * s, and the two signed FP64 sums.
+static double pair_flops(int N) {
+  const int M = N + 2;
+  double macs = 0.0, muls = 0.0, invs = 0.0;
+  for (int i = 0; i < N - 1; i++) {
+    for (int k = 0; k < i; k++) macs += M - 1 - k;
+    invs += 1;
+    muls += (M - 1 - i) + 1;
+  }
+  for (int k = 0; k < N - 1; k++) macs += M - 1 - k;
+  for (int t = 0; t < 2; t++) {
+    for (int k = 0; k < N - 1; k++) macs += (N - 2 - k) + 2;
+    macs += 1;
+    invs += 1;
+    muls += 4;
+  }
+  const double per_pair = 8 * macs + 6 * muls + 7 * invs + 4;
+  return std::ldexp(per_pair, N - 1);
+}
+
 static double tree_flops(int N) {
   auto node = [](double dc) { return 8.0 * dc * dc + 13.0 * dc; };
   const int A = N - 6;
@@ -803,6 +830,10 @@ int kd_flops_per_point(const kd_ctx *c, double *flops) {
   }
   if (use_tree(c)) {
     *flops = tree_flops(c->order);
+    return KD_OK;
+  }
+  if (use_pair(c)) {
+    *flops = pair_flops(c->order);
     return KD_OK;
   }
   const double N = c->order < 0 ? 0 : c->order;
@@ -1105,15 +1136,16 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   p.n_points = sample_points;
   p.nbins = 0; // no bin counts from tuning launches
   c->no_reduce = true;
-  // candidates: the tree (orders >= 6; its lanes are fixed) and the
-  // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16
-  // or 32 waves per CU
+  // candidates: the paired LU (orders >= 7), the tree (orders >= 6; both at
+  // 64 lanes) and the per-configuration kernel with 64, 32 or 16 lanes per
+  // point, each at 8, 16 or 32 waves per CU
   const bool direct = c->obs == KD_OBS_DIRECT;
-  for (int li = (c->order >= 6 && direct ? -1 : 0); li < (direct ? 3 : 1) && !rc; li++) {
+  const int li0 = !direct ? 0 : c->order >= 7 ? -2 : c->order >= 6 ? -1 : 0;
+  for (int li = li0; li < (direct ? 3 : 1) && !rc; li++) {
     const int L = li < 0 ? 64 : lane_opts[li];
     if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
     for (int wi = 0; wi < 3 && !rc; wi++) {
-      c->algo = li < 0 ? KD_ALGO_TREE : KD_ALGO_CONFIG;
+      c->algo = li == -2 ? KD_ALGO_PAIR : li == -1 ? KD_ALGO_TREE : KD_ALGO_CONFIG;
       c->lu_lanes_max = L;
       c->lu_waves_per_cu = wave_opts[wi];
       float t = 1e30f;
@@ -1157,14 +1189,15 @@ int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
 
 int kd_set_algorithm(kd_ctx *c, int32_t algo) {
   if (!c) return KD_ERR_ARG;
-  if (algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE) return fail(c, KD_ERR_ARG, "unknown Keldysh-sum algorithm");
+  if (algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE && algo != KD_ALGO_PAIR)
+    return fail(c, KD_ERR_ARG, "unknown Keldysh-sum algorithm");
   c->algo = algo;
   return KD_OK;
 }
 
 int kd_get_algorithm(const kd_ctx *c, int32_t *algo) {
   if (!c || !algo) return KD_ERR_ARG;
-  *algo = use_tree(c) ? KD_ALGO_TREE : KD_ALGO_CONFIG;
+  *algo = use_tree(c) ? KD_ALGO_TREE : use_pair(c) ? KD_ALGO_PAIR : KD_ALGO_CONFIG;
   return KD_OK;
 }
 

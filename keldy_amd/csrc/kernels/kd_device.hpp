@@ -26,9 +26,26 @@ template <typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
 template <typename T> __device__ __forceinline__ cx<T> cfms(cx<T> a, cx<T> l, cx<T> b) {
   return cmk<T>(a.x - l.x * b.x + l.y * b.y, a.y - l.x * b.y - l.y * b.x);
 }
+// 1/d from the hardware reciprocal estimate refined by Newton steps
+// r <- r + r (1 - d r) (two in FP64, one in FP32): within an ulp or two of the
+// correctly rounded quotient, without the scaling and fix-up sequence of IEEE
+// division.  For the LU pivots here: nonzero, finite, far from the
+// denormal range (flagged otherwise).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_nr(float d) {
+  float r = __builtin_amdgcn_rcpf(d);
+  const float e = fmaf(-d, r, 1.0f);
+  return fmaf(r, e, r);
+}
 template <typename T> __device__ __forceinline__ cx<T> cinv(cx<T> a) {
   T d = a.x * a.x + a.y * a.y;
-  T r = T(1) / d;
+  T r = rcp_nr(d);
   return cmk<T>(a.x * r, -a.y * r);
 }
 template <typename T> __device__ __forceinline__ T cabs1(cx<T> a) { return fabs(a.x) + fabs(a.y); }

@@ -25,18 +25,29 @@
 namespace kd {
 
 
+// LDS layout of a point's Wick matrix: row/column 2i+s for vertex i with
+// Keldysh index s, 2N for the external point; row stride wick_stride(N) >= 2N+1,
+// padded so that the four candidates (2i+{0,1}) x (2j+{0,1}) of an entry --
+// what the lanes of a wave read at once, one per configuration -- fall in
+// different LDS banks (row stride mod 8 not in {0, 1, 7}, in 16-byte units).
+constexpr int wick_stride(int N) {
+  int s = 2 * N + 1;
+  while (s % 8 == 0 || s % 8 == 1 || s % 8 == 7) s++;
+  return s;
+}
+
 // det(A)^2 * s for one Keldysh configuration, unpivoted row-wise LU in T.
 template <int N, typename T>
 __device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, int cfg, T sc, bool &bad) {
-  constexpr int D = 2 * N + 1;
+  constexpr int DS = wick_stride(N);
   int pr[N + 1], pc[N + 1];
   static_for<0, N>([&](auto I) {
     constexpr int i = I;
-    int pi = i + ((cfg >> i) & 1) * N;
-    pr[i] = pi * D;
+    int pi = 2 * i + ((cfg >> i) & 1);
+    pr[i] = pi * DS;
     pc[i] = pi;
   });
-  pr[N] = 2 * N * D;
+  pr[N] = 2 * N * DS;
   pc[N] = 2 * N;
   cx<T> Uh[N + 1][N + 1];
   cx<T> det = cmk<T>(T(1), T(0));
@@ -86,17 +97,124 @@ __device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, 
   return make_double2((double)v.x, (double)v.y);
 }
 
+// det(A)^2 * s for the two configurations that differ only in the Keldysh
+// index of the last vertex (cfg0 has it 0; v[t], bad[t] for index t).  Their
+// bordered LUs share the elimination of the first N-1 vertex rows: those rows
+// are eliminated once over the columns of both choices of vertex N-1 (and the
+// border), the external row likewise, and only the last vertex row and the
+// final corner update are done per choice -- 268 instead of 408 complex
+// multiply-adds per pair at order 8, same arithmetic per configuration.
+template <int N, typename T>
+__device__ __forceinline__ void config_value_pair(const double2 *__restrict__ Wq, int cfg0, T sc, double2 (&v)[2],
+                                                  bool (&bad)[2]) {
+  constexpr int DS = wick_stride(N);
+  constexpr int M = N + 2; // columns: vertices 0..N-2, vertex N-1 (index 0), vertex N-1 (index 1), border
+  int pr[N - 1], pc[M];
+  static_for<0, N - 1>([&](auto I) {
+    constexpr int i = I;
+    const int pi = 2 * i + ((cfg0 >> i) & 1);
+    pr[i] = pi * DS;
+    pc[i] = pi;
+  });
+  pc[N - 1] = 2 * N - 2;
+  pc[N] = 2 * N - 1;
+  pc[N + 1] = 2 * N;
+  cx<T> Uh[N - 1][M];
+  cx<T> det = cmk<T>(T(1), T(0));
+  bool flag = false;
+  static_for<0, N - 1>([&](auto I) {
+    constexpr int i = I;
+    cx<T> row[M];
+    static_for<0, M>([&](auto J) {
+      constexpr int j = J;
+      const double2 w = Wq[pr[i] + pc[j]];
+      row[j] = cmk<T>((T)w.x, (T)w.y);
+    });
+    const T diag0 = fmax(cabs1<T>(row[i]), sc);
+    static_for<0, i>([&](auto K) {
+      constexpr int k = K;
+      const cx<T> l = row[k];
+      static_for<k + 1, M>([&](auto J) {
+        constexpr int j = J;
+        row[j] = cfms<T>(row[j], l, Uh[k][j]);
+      });
+    });
+    const cx<T> piv = row[i];
+    const T pm = cabs1<T>(piv);
+    flag = flag || !(pm >= PivotTol<T>::v * diag0) || pm == T(0);
+    det = cmul<T>(det, piv);
+    const cx<T> inv = cinv<T>(piv);
+    static_for<i + 1, M>([&](auto J) {
+      constexpr int j = J;
+      Uh[i][j] = cmul<T>(row[j], inv);
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  // external row over the same columns (its border entry, the corner, is 0)
+  cx<T> rb[M];
+  static_for<0, M>([&](auto J) {
+    constexpr int j = J;
+    if constexpr (j == M - 1) {
+      rb[j] = cmk<T>(T(0), T(0));
+    } else {
+      const double2 w = Wq[2 * N * DS + pc[j]];
+      rb[j] = cmk<T>((T)w.x, (T)w.y);
+    }
+  });
+  static_for<0, N - 1>([&](auto K) {
+    constexpr int k = K;
+    const cx<T> l = rb[k];
+    static_for<k + 1, M>([&](auto J) {
+      constexpr int j = J;
+      rb[j] = cfms<T>(rb[j], l, Uh[k][j]);
+    });
+  });
+  __builtin_amdgcn_sched_barrier(0);
+  // the last vertex row for each Keldysh index t
+  static_for<0, 2>([&](auto TT) {
+    constexpr int t = TT;
+    constexpr int ct = N - 1 + t; // its column
+    cx<T> row[M];
+    static_for<0, M>([&](auto J) {
+      constexpr int j = J;
+      if constexpr (j < N - 1 || j == ct || j == M - 1) {
+        const double2 w = Wq[pc[ct] * DS + pc[j]];
+        row[j] = cmk<T>((T)w.x, (T)w.y);
+      }
+    });
+    const T diag0 = fmax(cabs1<T>(row[ct]), sc);
+    static_for<0, N - 1>([&](auto K) {
+      constexpr int k = K;
+      const cx<T> l = row[k];
+      static_for<k + 1, M>([&](auto J) {
+        constexpr int j = J;
+        if constexpr (j < N - 1 || j == ct || j == M - 1) row[j] = cfms<T>(row[j], l, Uh[k][j]);
+      });
+    });
+    const cx<T> piv = row[ct];
+    const T pm = cabs1<T>(piv);
+    bad[t] = flag || !(pm >= PivotTol<T>::v * diag0) || pm == T(0);
+    const cx<T> dt = cmul<T>(det, piv);
+    const cx<T> u = cmul<T>(row[M - 1], cinv<T>(piv));
+    const cx<T> s = cfms<T>(rb[M - 1], rb[ct], u);
+    const cx<T> d2 = cmul<T>(dt, dt);
+    const cx<T> val = cmul<T>(d2, s);
+    v[t] = make_double2((double)val.x, (double)val.y);
+    __builtin_amdgcn_sched_barrier(0);
+  });
+}
+
 // Fallback: bordered LU with partial pivoting over the rows of A (pivot =
 // max |Re|+|Im|, first on ties: the LAPACK zgetf2 rule), FP64.  The row
 // permutation's sign cancels in det(A)^2.
-__device__ __noinline__ double2 config_value_pivoted(const double2 *__restrict__ Wq, int cfg, int N) {
-  const int D = 2 * N + 1, M = N + 1;
+__device__ __noinline__ double2 config_value_pivoted(const double2 *__restrict__ Wq, int cfg, int N, int DS) {
+  const int M = N + 1;
   double2 a[(KD_MAX_ORDER + 1) * (KD_MAX_ORDER + 1)];
   int pk[KD_MAX_ORDER + 1];
-  for (int i = 0; i < N; i++) pk[i] = i + ((cfg >> i) & 1) * N;
+  for (int i = 0; i < N; i++) pk[i] = 2 * i + ((cfg >> i) & 1);
   pk[N] = 2 * N;
   for (int i = 0; i < M; i++)
-    for (int j = 0; j < M; j++) a[i * M + j] = Wq[pk[i] * D + pk[j]];
+    for (int j = 0; j < M; j++) a[i * M + j] = Wq[pk[i] * DS + pk[j]];
   a[N * M + N] = make_double2(0.0, 0.0);
   double2 det = make_double2(1.0, 0.0);
   for (int k = 0; k < N; k++) {
@@ -243,7 +361,9 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
 
 // Stage 2: the Keldysh sum of the compacted points; 64-lane workgroups,
 // grid-stride over groups of points.
-template <int N, typename T, int GL>
+// PR: the two choices of the last vertex's Keldysh index share their
+// elimination (config_value_pair), for 2^N / lanes >= 2.
+template <int N, typename T, int GL, bool PR = false>
 __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) {
   constexpr int NC = 1 << N;            // Keldysh configurations per point
   constexpr int G = NC < GL ? NC : GL;  // lanes per point (GL: the tuned maximum)
@@ -251,7 +371,8 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
   constexpr int CPL = NC / G;           // configurations per lane
   constexpr int D = 2 * N + 1;
   constexpr int NU = N > 0 ? N : 1;
-  __shared__ double2 W[PPW][D * D];
+  constexpr int DS = wick_stride(N);
+  __shared__ double2 W[PPW][D * DS];
   __shared__ double Us[PPW][NU];
   __shared__ unsigned long long Ssc[PPW]; // per point: max |Re|+|Im| of the vertex block (bits of a double >= 0)
   extern __shared__ double2 sbins[]; // p.nbins entries when time-binning (dynamic LDS)
@@ -293,21 +414,19 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
         const int r = rc / D, c = rc - r * D;
         CPt a, bb;
         if (r < 2 * N) {
-          const int v = r < N ? r : r - N;
-          a = CPt{Us[q][v], r < N ? 0 : 1, v, 0};
+          a = CPt{Us[q][r >> 1], r & 1, r >> 1, 0};
         } else {
           a = CPt{p.A.time, p.A.k_idx, N, p.A.orbital};
         }
         if (c < 2 * N) {
-          const int v = c < N ? c : c - N;
-          bb = CPt{Us[q][v], c < N ? 0 : 1, v, 0};
+          bb = CPt{Us[q][c >> 1], c & 1, c >> 1, 0};
         } else {
           bb = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
         }
         const bool corner = (r == 2 * N) && (c == 2 * N);
         double2 gv = g0_eval(p, a, bb, !corner);
         if (corner && N > 0) gv = make_double2(0.0, 0.0);
-        W[q][rc] = gv;
+        W[q][r * DS + c] = gv;
         if (r < 2 * N && c < 2 * N) atomicMax(&Ssc[q], (unsigned long long)__double_as_longlong(fabs(gv.x) + fabs(gv.y)));
       }
       __syncthreads();
@@ -322,23 +441,41 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
           sim = v.y;
         } else {
           uint64_t badmask = 0;
+          if constexpr (PR && CPL >= 2) {
 #pragma unroll 1
-          for (int m = 0; m < CPL; m++) {
-            const int cfg = (lane % G) + G * m;
-            bool bad;
-            const double2 v = config_value<N, T>(PPW == 1 ? W[0] : W[q], cfg, (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), bad);
-            // branch-free: a divergent branch here doubles the register footprint
-            const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
-            sre += bad ? 0.0 : sg * v.x;
-            sim += bad ? 0.0 : sg * v.y;
-            badmask |= (uint64_t)bad << m;
+            for (int mp = 0; mp < CPL / 2; mp++) {
+              const int cfg0 = (lane % G) + G * mp; // last vertex at index 0; + 2^(N-1) for index 1
+              double2 v[2];
+              bool bad[2];
+              config_value_pair<N, T>(PPW == 1 ? W[0] : W[q], cfg0,
+                                      (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), v, bad);
+              const double sg = (__popc(cfg0) & 1) ? -1.0 : 1.0;
+              sre += bad[0] ? 0.0 : sg * v[0].x;
+              sim += bad[0] ? 0.0 : sg * v[0].y;
+              sre += bad[1] ? 0.0 : -sg * v[1].x;
+              sim += bad[1] ? 0.0 : -sg * v[1].y;
+              badmask |= ((uint64_t)bad[0] << mp) | ((uint64_t)bad[1] << (mp + CPL / 2));
+            }
+          } else {
+#pragma unroll 1
+            for (int m = 0; m < CPL; m++) {
+              const int cfg = (lane % G) + G * m;
+              bool bad;
+              const double2 v = config_value<N, T>(PPW == 1 ? W[0] : W[q], cfg,
+                                                   (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), bad);
+              // branch-free: a divergent branch here doubles the register footprint
+              const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
+              sre += bad ? 0.0 : sg * v.x;
+              sim += bad ? 0.0 : sg * v.y;
+              badmask |= (uint64_t)bad << m;
+            }
           }
           // Flagged configurations: partial-pivoting FP64 LU, outside the hot loop.
           while (badmask) {
             const int m = __builtin_ctzll(badmask);
             badmask &= badmask - 1;
             const int cfg = (lane % G) + G * m;
-            const double2 v = config_value_pivoted(W[q], cfg, N);
+            const double2 v = config_value_pivoted(W[q], cfg, N, DS);
             const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
             sre += sg * v.x;
             sim += sg * v.y;
@@ -479,10 +616,12 @@ template <int N> static hipError_t launch_prep_n(const KdParams &p, const PrepBu
 }
 
 template <int N, typename T>
-static hipError_t launch_lu_n(int lanes, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
+static hipError_t launch_lu_n(int lanes, bool pair, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
   // lane-count variants only where they differ (2^N > lanes)
   const size_t shm = p.nbins > 0 ? sizeof(double2) * (size_t)p.nbins : 0;
-  if (lanes == 16 && N >= 5)
+  if (pair && N >= 7 && lanes == 64)
+    hipLaunchKernelGGL((integrand_kernel<N, T, 64, true>), dim3(grid), dim3(64), shm, st, p, buf);
+  else if (lanes == 16 && N >= 5)
     hipLaunchKernelGGL((integrand_kernel<N, T, 16>), dim3(grid), dim3(64), shm, st, p, buf);
   else if (lanes == 32 && N >= 6)
     hipLaunchKernelGGL((integrand_kernel<N, T, 32>), dim3(grid), dim3(64), shm, st, p, buf);
@@ -515,14 +654,14 @@ hipError_t launch_prep(int order, const KdParams &p, const PrepBuf &buf, int gri
 #undef KD_PREP_CALL
 }
 
-hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, const PrepBuf &buf, int grid,
+hipError_t launch_lu(int order, int precision, int lanes, bool pair, const KdParams &p, const PrepBuf &buf, int grid,
                      hipStream_t st) {
   if (lu_is_fp32(order, precision)) {
-#define KD_LU32_CALL(n) launch_lu_n<n, float>(lanes, p, buf, grid, st)
+#define KD_LU32_CALL(n) launch_lu_n<n, float>(lanes, pair, p, buf, grid, st)
     KD_ORDER_SWITCH(order, KD_LU32_CALL)
 #undef KD_LU32_CALL
   }
-#define KD_LU64_CALL(n) launch_lu_n<n, double>(lanes, p, buf, grid, st)
+#define KD_LU64_CALL(n) launch_lu_n<n, double>(lanes, pair, p, buf, grid, st)
   KD_ORDER_SWITCH(order, KD_LU64_CALL)
 #undef KD_LU64_CALL
 }

@@ -23,7 +23,8 @@ inline int lu_lanes(int order, int lanes_max) {
 }
 // integrand_kernel<order, T, lanes> (the Keldysh sum): grid blocks of 64
 // threads; lanes = the maximum lanes per point (64, 32 or 16).
-hipError_t launch_lu(int order, int precision, int lanes, const KdParams &p, const PrepBuf &buf, int grid,
+// pair: the two choices of the last vertex share their elimination (orders >= 7, lanes 64).
+hipError_t launch_lu(int order, int precision, int lanes, bool pair, const KdParams &p, const PrepBuf &buf, int grid,
                      hipStream_t st);
 // integrand_tree_kernel<order, T> (orders 6..12): the Keldysh sum by the
 // Schur-complement tree, one point per 64-thread workgroup item.

@@ -18,16 +18,16 @@ products) appended to the plain chain (applied first to the Sobol point),
 trained on the device for 2000 Adam steps of 2^14 points on 2^20 samples of
 the plain warped integrand (scramble seed 1000003, disjoint from the
 replicas), toward the density |f|^alpha for alpha in 1, 1/2, 1/4.  The plain
-estimator judges the candidates on 16 pilot replicas of 2^16 points (seeds
+estimator judges the candidates on 32 pilot replicas of 2^16 points (seeds
 20000+) and remains the fallback: the best of plain and the flows is measured.
 
 For each estimator: time to 1e-3 = (points needed for 1e-3 relative error) /
-(points per second).  The points needed come from a doubling search: 16
-scrambled replicas at M points, M = 2^12, 2^13, ... until the relative spread
+(points per second).  The points needed come from a doubling search: 64
+(--target-replicas) scrambled replicas at M points, M = 2^12, 2^13, ... until the relative spread
 of one replica's estimate is <= 1e-3, then log-log interpolation between the
 last two M.  Points per second come from the timed steps (`--points` per step,
 replica seeds 0..steps-1, the same Sobol points for both estimators).  value =
-the learned estimator's time (the plain one's with --no-flow).  One-off set-up
+the selected estimator's time (the plain one's with --no-flow).  One-off set-up
 (g0 tabulation, pilot fit, flow training) is reported separately.
 """
 import argparse
@@ -44,6 +44,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 TARGET_REL = 1e-3
 PILOT_M = 1 << 16  # points per pilot replica when judging flow candidates
+PILOT_REPLICAS = 32
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "integrand_kernel_n8_fp64_pmc.md")
 
 
@@ -160,6 +161,51 @@ def points_for_target(comp, replicas, target, m0=1 << 12, m_max=1 << 22):
     return float(n), table, table[-1][2]
 
 
+def kernel_method_line(args, model, proj, t_max):
+    """Throughput of the kernel method (compute_gf_kernel: per-vertex cofactor
+    entries binned by vertex time and Keldysh index, 10 bins) at --order on the
+    plain chain: points/s and the cofactor kernel's FP64 rate (its own flop
+    count) over --steps replicas of --points points.  One JSON line."""
+    import numpy as np
+
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+
+    comp = kio.ComputeGfKernel(model, t_max, args.order, 10)
+    comp.warper.emplace_back(kw.WarperPlasmaUvT(t_max))
+    comp.warper.emplace_back(kw.make_product_1d_simple_inverse(t_max, 1.0))
+    comp.warper.emplace_back(proj)
+    for r in range(args.warmup):
+        comp.reset_rng("sobol", 0, True, True, 1000 + r)
+        comp.run(args.points)
+    walls, lu, dom = [], [], []
+    for r in range(args.steps):
+        comp.reset_rng("sobol", 0, True, True, r)
+        d0 = comp.reduce_nr_points_in_domain()
+        t0 = time.perf_counter()
+        comp.run(args.points)
+        walls.append(time.perf_counter() - t0)
+        lu.append(comp.last_timing()[1])
+        dom.append(comp.reduce_nr_points_in_domain() - d0)
+    wall, lu_s = float(np.sum(walls)), float(np.sum(lu)) * 1e-3
+    fpp = comp.flops_per_point()
+    achieved = float(np.sum(dom)) * fpp / lu_s / 1e12
+    launches = max(1, -(-args.points // (1 << 21)))
+    b = comp.reduce_result()
+    print(json.dumps({
+        "metric": f"kernel-method points per second at n={args.order}", "value": args.steps * args.points / wall,
+        "unit": "points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "dtype": "f64",
+        "config": {"workload": f"compute_gf_kernel order {args.order}, t={t_max}, 10 bins, plain fitted chain",
+                   "points_per_replica": args.points},
+        "roofline": {"bound": "valu-fp64", "kernel": f"cofactor_kernel<{args.order}>", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "flops_per_point": fpp, "lu_ms_per_launch": float(np.mean(lu)) / launches},
+        "check": {"nr_values_dropped": int(b.get_nr_values_dropped()),
+                  "sum_of_bins": complex(b.get_data().sum()).real},
+    }))
+
+
 def cpu_baseline(model, proj, order, t_max, gpu_points_needed, seconds):
     """The CPU oracle (a serial-per-point C port of the reference path) on the
     same warper chain, timed on the host cores."""
@@ -199,6 +245,8 @@ def main():
     ap.add_argument("--order", type=int, default=8)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--pilot-points", type=int, default=1 << 14)
+    ap.add_argument("--target-replicas", type=int, default=64,
+                    help="RQMC replicas per size in the doubling search for the points needed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flow", action="store_true", help="plain estimator only")
@@ -210,6 +258,8 @@ def main():
                     help="weight powers of the flow candidates (density ~ |f|^alpha)")
     ap.add_argument("--algorithm", default=None, choices=["config", "pair", "tree"],
                     help="Keldysh-sum algorithm (default: the library's)")
+    ap.add_argument("--observable", default="charge", choices=["charge", "kernel"],
+                    help="kernel: only time the kernel method (compute_gf_kernel, 10 bins) on the same chain")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters), then exit")
     args = ap.parse_args()
@@ -238,6 +288,9 @@ def main():
     if args.algorithm:
         comp.set_algorithm(args.algorithm)
     setup_s = time.perf_counter() - t0
+    if args.observable == "kernel":
+        kernel_method_line(args, model, proj, t_max)
+        return
 
     def measure(label):
         if args.warmup:
@@ -246,7 +299,7 @@ def main():
         wall = float(walls.sum())
         n_total = args.steps * args.points
         pps = n_total / wall
-        n_needed, table, wall_at_target = points_for_target(comp, 16, TARGET_REL)
+        n_needed, table, wall_at_target = points_for_target(comp, args.target_replicas, TARGET_REL)
         mean = est.mean()
         return {"warper": label, "est": est, "dom": dom, "lu_ms": lu_ms, "wall": wall, "pps": pps,
                 "n_needed": n_needed, "t_needed": n_needed / pps, "table": table,
@@ -280,7 +333,7 @@ def main():
             comp.warper = tr
 
         def pilot_spread():
-            est, _, _, _ = run_replicas(comp, 20_000, 16, PILOT_M)
+            est, _, _, _ = run_replicas(comp, 20_000, PILOT_REPLICAS, PILOT_M)
             return float(est.std(ddof=1) / abs(est.mean()))
 
         # Candidates: flows trained toward |f|^alpha; the plain estimator judges
@@ -323,7 +376,7 @@ def main():
 
     estimator = {"plain": summary(plain), "setup_s": setup_s}
     if selection:
-        estimator["flow_selection"] = {"pilot_points": PILOT_M, "pilot_replicas": 16, "candidates": selection,
+        estimator["flow_selection"] = {"pilot_points": PILOT_M, "pilot_replicas": PILOT_REPLICAS, "candidates": selection,
                                        "selected": head["warper"], "training_s_total": train_s}
     if learned:
         pe, le = plain["est"], learned["est"]

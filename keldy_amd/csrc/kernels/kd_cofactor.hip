@@ -30,15 +30,18 @@
 
 namespace kd {
 
-template <int N>
-__device__ __forceinline__ void cofactor_config(const double2 *__restrict__ Wq, int cfg, double sc, double2 (&v)[N],
-                                                bool &bad) {
-  constexpr int D = 2 * N + 1;
+// emit(integral_constant<i>, entry) receives the entries as the back
+// substitution finalises them (vertex N-1 first); a flagged configuration
+// emits zeros (its entries come from the pivoted fallback).
+template <int N, typename F>
+__device__ __forceinline__ void cofactor_config(const double2 *__restrict__ Wq, int cfg, double sc, bool &bad,
+                                                F &&emit) {
+  constexpr int DS = wick_stride(N);
   int pr[N], pc[N];
   static_for<0, N>([&](auto I) {
     constexpr int i = I;
-    const int pi = i + ((cfg >> i) & 1) * N;
-    pr[i] = pi * D;
+    const int pi = 2 * i + ((cfg >> i) & 1);
+    pr[i] = pi * DS;
     pc[i] = pi;
   });
   double2 Uh[N][N + 1]; // normalised rows 0..N-2 of U, and L^-1 c in column N
@@ -82,8 +85,13 @@ __device__ __forceinline__ void cofactor_config(const double2 *__restrict__ Wq, 
     __builtin_amdgcn_sched_barrier(0);
   });
   const double2 pp = cmk<double>(pfull.x - delta.x, pfull.y - delta.y);
+  const double sg = flag ? 0.0 : (__popc(cfg) & 1) ? 1.0 : -1.0; // minus the parity (0: flagged)
+  double2 coef = cmul<double>(cmul<double>(dprod, dprod), pfull);
+  coef.x *= sg;
+  coef.y *= sg;
   double2 yh[N];
   yh[N - 1] = zlast;
+  emit(std::integral_constant<int, N - 1>{}, cmul<double>(coef, zlast));
   static_for<0, N - 1>([&](auto I) {
     constexpr int i = N - 2 - (int)I;
     double2 s = cmul<double>(pp, Uh[i][N]);
@@ -92,14 +100,7 @@ __device__ __forceinline__ void cofactor_config(const double2 *__restrict__ Wq, 
       s = cfms<double>(s, Uh[i][j], yh[j]);
     });
     yh[i] = s;
-  });
-  const double sg = (__popc(cfg) & 1) ? 1.0 : -1.0; // minus the parity
-  double2 coef = cmul<double>(cmul<double>(dprod, dprod), pfull);
-  coef.x *= sg;
-  coef.y *= sg;
-  static_for<0, N>([&](auto I) {
-    constexpr int i = I;
-    v[i] = cmul<double>(coef, yh[i]);
+    emit(std::integral_constant<int, i>{}, cmul<double>(coef, s));
   });
   bad = flag;
 }
@@ -139,18 +140,18 @@ __device__ __forceinline__ double2 det_pivoted(double2 *a, int m) {
 }
 
 // Fallback for a flagged configuration: entry i = parity * det V * (-det V'_{i<-c}).
-__device__ __noinline__ void cofactor_config_pivoted(const double2 *__restrict__ Wq, int cfg, int N, double2 *v) {
-  const int D = 2 * N + 1;
+__device__ __noinline__ void cofactor_config_pivoted(const double2 *__restrict__ Wq, int cfg, int N, int DS,
+                                                    double2 *v) {
   int pk[KD_MAX_ORDER];
-  for (int i = 0; i < N; i++) pk[i] = i + ((cfg >> i) & 1) * N;
+  for (int i = 0; i < N; i++) pk[i] = 2 * i + ((cfg >> i) & 1);
   double2 a[KD_MAX_ORDER * KD_MAX_ORDER];
   for (int i = 0; i < N; i++)
-    for (int j = 0; j < N; j++) a[i * N + j] = Wq[pk[i] * D + pk[j]];
+    for (int j = 0; j < N; j++) a[i * N + j] = Wq[pk[i] * DS + pk[j]];
   const double2 d2 = det_pivoted(a, N);
   const double sg = (__popc(cfg) & 1) ? 1.0 : -1.0;
   for (int c = 0; c < N; c++) {
     for (int i = 0; i < N; i++)
-      for (int j = 0; j < N; j++) a[i * N + j] = Wq[pk[i] * D + (j == c ? 2 * N : pk[j])];
+      for (int j = 0; j < N; j++) a[i * N + j] = Wq[pk[i] * DS + (j == c ? 2 * N : pk[j])];
     if (c != N - 1) a[(N - 1) * N + N - 1] = make_double2(0.0, 0.0);
     double2 r = cmul<double>(d2, det_pivoted(a, N));
     r.x *= sg;
@@ -169,7 +170,9 @@ __device__ __forceinline__ double2 shfl_xor2(double2 a, int off) {
 // kernel).  G = min(2^N, 64) lanes per point, 2^N / G configurations per lane
 // (configuration = lane + G m): Keldysh indices of vertices 0..log2(G)-1 are
 // fixed per lane, those of the higher vertices follow m.
-template <int N> __global__ __launch_bounds__(64) void cofactor_kernel(KdParams p, PrepBuf buf) {
+// EV: per-configuration entries to p.out_vals (kernel-method evaluation);
+// otherwise per-vertex sums only (runs, single omega, samples).
+template <int N, bool EV> __global__ __launch_bounds__(64) void cofactor_kernel(KdParams p, PrepBuf buf) {
   constexpr int NC = 1 << N;
   constexpr int G = NC < 64 ? NC : 64;
   constexpr int LG = N < 6 ? N : 6;
@@ -177,7 +180,10 @@ template <int N> __global__ __launch_bounds__(64) void cofactor_kernel(KdParams 
   constexpr int CPL = NC / G;
   constexpr int D = 2 * N + 1;
   constexpr int NH = N - LG;
-  __shared__ double2 W[PPW][2 * N * D];
+  constexpr int NS = LG + 2 * NH; // per-lane accumulator slots
+  constexpr int DS = wick_stride(N);
+  __shared__ double2 W[PPW][2 * N * DS];
+  __shared__ double2 Acc[64][NS];
   __shared__ double Us[PPW][N];
   __shared__ unsigned long long Ssc[PPW]; // per point: max |Re|+|Im| of the vertex block (bits of a double >= 0)
   extern __shared__ double2 sbins[]; // [nbins][2] when binning (dynamic LDS)
@@ -204,22 +210,20 @@ template <int N> __global__ __launch_bounds__(64) void cofactor_kernel(KdParams 
     if (lane < ng * N) Us[lane / N][lane % N] = buf.u[slot0 * N + lane];
     if (lane < PPW) Ssc[lane] = 0ull;
     __syncthreads();
-    // vertex rows of the Wick matrix (wick_kernel.cpp:80-94): columns are the
-    // 2N vertex points, then the external point X (split N)
+    // vertex rows of the Wick matrix (wick_kernel.cpp:80-94), row/column 2v+k
+    // for vertex v at Keldysh index k, then the external point X (split N) in
+    // column 2N; row stride wick_stride(N)
     for (int e = lane; e < ng * 2 * N * D; e += 64) {
       const int q = e / (2 * N * D), rc = e - q * (2 * N * D);
       const int r = rc / D, c = rc - r * D;
-      const int vr = r < N ? r : r - N;
-      const CPt a = CPt{Us[q][vr], r < N ? 0 : 1, vr, 0};
+      const CPt a = CPt{Us[q][r >> 1], r & 1, r >> 1, 0};
       CPt bb;
-      if (c < 2 * N) {
-        const int vc = c < N ? c : c - N;
-        bb = CPt{Us[q][vc], c < N ? 0 : 1, vc, 0};
-      } else {
+      if (c < 2 * N)
+        bb = CPt{Us[q][c >> 1], c & 1, c >> 1, 0};
+      else
         bb = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
-      }
       const double2 gv = g0_eval(p, a, bb, true);
-      W[q][rc] = gv;
+      W[q][r * DS + c] = gv;
       if (c < 2 * N) atomicMax(&Ssc[q], (unsigned long long)__double_as_longlong(fabs(gv.x) + fabs(gv.y)));
     }
     __syncthreads();
@@ -228,32 +232,22 @@ template <int N> __global__ __launch_bounds__(64) void cofactor_kernel(KdParams 
     const int64_t slot = slot0 + (q < ng ? q : 0);
     const double jac = buf.jac[slot];
     const int64_t idx = b * 64 + buf.idx[slot];
-    const bool eval_entries = p.mode != MODE_RUN && p.obs == OBS_KERNEL;
-    double2 alo[LG];
-    double2 ahi[NH > 0 ? NH : 1][2];
-    static_for<0, LG>([&](auto I) { alo[I] = make_double2(0.0, 0.0); });
-    static_for<0, (NH > 0 ? NH : 1)>([&](auto I) {
-      ahi[I][0] = make_double2(0.0, 0.0);
-      ahi[I][1] = make_double2(0.0, 0.0);
-    });
+    // per-lane sums in LDS during the configuration loop (registers are the
+    // LU's): slot i for vertex i < LG, LG + 2 (i - LG) + index for the others
+    double2 *acc = Acc[lane];
+    static_for<0, NS>([&](auto I) { acc[(int)I] = make_double2(0.0, 0.0); });
     int nfb = 0;
     if (q < ng) {
       const double2 *Wq = PPW == 1 ? W[0] : W[q];
+      // entry of vertex i (configuration lq + G m) into the per-lane sums
+      auto add = [&](auto I, double2 x, int m) {
+        constexpr int i = I;
+        const int sl = i < LG ? i : LG + 2 * (i - LG) + ((m >> (i - LG)) & 1);
+        acc[sl] = cadd(acc[sl], x);
+      };
       auto accumulate = [&](const double2 *vals, int m, int cfg) {
-        static_for<0, N>([&](auto I) {
-          constexpr int i = I;
-          const double2 x = vals[i];
-          if constexpr (i < LG) {
-            alo[i] = cadd(alo[i], x);
-          } else {
-            const bool hb = (m >> (i - LG)) & 1;
-            ahi[i - LG][0].x += hb ? 0.0 : x.x;
-            ahi[i - LG][0].y += hb ? 0.0 : x.y;
-            ahi[i - LG][1].x += hb ? x.x : 0.0;
-            ahi[i - LG][1].y += hb ? x.y : 0.0;
-          }
-        });
-        if (eval_entries) {
+        static_for<0, N>([&](auto I) { add(I, vals[(int)I], m); });
+        if constexpr (EV) {
           double *o = p.out_vals + 2 * ((idx * NC + cfg) * N);
           static_for<0, N>([&](auto I) {
             constexpr int i = I;
@@ -262,27 +256,39 @@ template <int N> __global__ __launch_bounds__(64) void cofactor_kernel(KdParams 
           });
         }
       };
+      const double sc = __longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]);
       uint64_t badmask = 0;
 #pragma unroll 1
       for (int m = 0; m < CPL; m++) {
         const int cfg = lq + G * m;
-        double2 v[N];
         bool bad;
-        cofactor_config<N>(Wq, cfg, __longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), v, bad);
+        if constexpr (EV) {
+          double2 v[N];
+          cofactor_config<N>(Wq, cfg, sc, bad, [&](auto I, double2 x) { v[(int)I] = x; });
+          if (!bad) accumulate(v, m, cfg);
+        } else {
+          cofactor_config<N>(Wq, cfg, sc, bad, [&](auto I, double2 x) { add(I, x, m); });
+        }
         badmask |= (uint64_t)bad << m;
-        if (!bad) accumulate(v, m, cfg);
       }
       while (badmask) {
         const int m = __builtin_ctzll(badmask);
         badmask &= badmask - 1;
         const int cfg = lq + G * m;
         double2 vf[N];
-        cofactor_config_pivoted(Wq, cfg, N, vf);
+        cofactor_config_pivoted(Wq, cfg, N, DS, vf);
         accumulate(vf, m, cfg);
         nfb++;
       }
     }
     acc_fb += (double)nfb;
+    double2 alo[LG];
+    double2 ahi[NH > 0 ? NH : 1][2];
+    static_for<0, LG>([&](auto I) { alo[I] = acc[(int)I]; });
+    static_for<0, NH>([&](auto I) {
+      ahi[I][0] = acc[LG + 2 * (int)I];
+      ahi[I][1] = acc[LG + 2 * (int)I + 1];
+    });
 
     if (p.obs == OBS_KERNEL_OMEGA) {
       // sum_i (K_i0 + K_i1) exp(i omega t_i) over the point's lanes
@@ -399,7 +405,10 @@ template <int N> static hipError_t launch_cofactor_n(const KdParams &p, const Pr
   } else {
     const bool binned = p.mode == MODE_RUN && p.obs == OBS_KERNEL && p.nbins > 0;
     const size_t shm = binned ? sizeof(double2) * 2 * (size_t)p.nbins : 0;
-    hipLaunchKernelGGL((cofactor_kernel<N>), dim3(grid), dim3(64), shm, st, p, buf);
+    if (p.mode != MODE_RUN && p.obs == OBS_KERNEL)
+      hipLaunchKernelGGL((cofactor_kernel<N, true>), dim3(grid), dim3(64), shm, st, p, buf);
+    else
+      hipLaunchKernelGGL((cofactor_kernel<N, false>), dim3(grid), dim3(64), shm, st, p, buf);
     return hipGetLastError();
   }
 }

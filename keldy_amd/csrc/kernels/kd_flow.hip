@@ -480,17 +480,23 @@ template <int N> __global__ __launch_bounds__(64) void flow_train_kernel(FlowPar
 }
 
 // grad[i] = sum over blocks of part[blk][i] (fixed order); loss likewise.
+// grad[i] = sum over blocks of part[blk][i]: 64 parameters per workgroup, its
+// 4 waves sum the blocks b = w (mod 4) (coalesced rows), then a fixed-order
+// combine in LDS -- enough workgroups to cover the chip, deterministic.
 __global__ __launch_bounds__(256) void flow_grad_reduce_kernel(const float *__restrict__ part, int nblk,
                                                                int n_params, float *__restrict__ grad,
                                                                const double *__restrict__ loss_part,
                                                                double *__restrict__ loss) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n_params) {
-    float s = 0.f;
-    for (int b = 0; b < nblk; b++) s += part[(size_t)b * n_params + i];
-    grad[i] = s;
-  }
-  if (i == 0) {
+  __shared__ float sh[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  float s = 0.f;
+  if (i < n_params)
+    for (int b = w; b < nblk; b += 4) s += part[(size_t)b * n_params + i];
+  sh[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && i < n_params) grad[i] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     double l = 0.0;
     for (int b = 0; b < nblk; b++) l += loss_part[b];
     *loss = l;
@@ -599,7 +605,7 @@ hipError_t launch_flow_train(int order, const FlowParams &fp, const FlowTrainIO 
 
 hipError_t launch_flow_grad_reduce(const float *part, int nblk, int n_params, float *grad, const double *loss_part,
                                    double *loss, hipStream_t st) {
-  hipLaunchKernelGGL(flow_grad_reduce_kernel, dim3((n_params + 255) / 256), dim3(256), 0, st, part, nblk, n_params,
+  hipLaunchKernelGGL(flow_grad_reduce_kernel, dim3((n_params + 63) / 64), dim3(256), 0, st, part, nblk, n_params,
                      grad, loss_part, loss);
   return hipGetLastError();
 }

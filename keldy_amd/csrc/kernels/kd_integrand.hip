@@ -25,17 +25,6 @@
 namespace kd {
 
 
-// LDS layout of a point's Wick matrix: row/column 2i+s for vertex i with
-// Keldysh index s, 2N for the external point; row stride wick_stride(N) >= 2N+1,
-// padded so that the four candidates (2i+{0,1}) x (2j+{0,1}) of an entry --
-// what the lanes of a wave read at once, one per configuration -- fall in
-// different LDS banks (row stride mod 8 not in {0, 1, 7}, in 16-byte units).
-constexpr int wick_stride(int N) {
-  int s = 2 * N + 1;
-  while (s % 8 == 0 || s % 8 == 1 || s % 8 == 7) s++;
-  return s;
-}
-
 // det(A)^2 * s for one Keldysh configuration, unpivoted row-wise LU in T.
 template <int N, typename T>
 __device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, int cfg, T sc, bool &bad) {
@@ -66,7 +55,7 @@ __device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, 
       }
     });
     T diag0 = T(0);
-    if constexpr (i < N) diag0 = fmax(cabs1<T>(row[i]), sc);
+    if constexpr (i < N) diag0 = fmax(fmax(cabs1<T>(row[i]), sc), PivotTol<T>::floor); // > 0: a zero pivot fails
     constexpr int KE = i < N ? i : N;
     static_for<0, KE>([&](auto K) {
       constexpr int k = K;
@@ -79,8 +68,11 @@ __device__ __forceinline__ double2 config_value(const double2 *__restrict__ Wq, 
     if constexpr (i < N) {
       cx<T> piv = row[i];
       T pm = cabs1<T>(piv);
-      flag = flag || !(pm >= PivotTol<T>::v * diag0) || pm == T(0);
-      det = cmul<T>(det, piv);
+      flag = flag || !(pm >= PivotTol<T>::v * diag0);
+      if constexpr (i == 0)
+        det = piv;
+      else
+        det = cmul<T>(det, piv);
       cx<T> inv = cinv<T>(piv);
       static_for<i + 1, N + 1>([&](auto J) {
         constexpr int j = J;
@@ -279,12 +271,16 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
   constexpr int NU = N > 0 ? N : 1;
   __shared__ uint32_t sC[NU * 31];
   __shared__ uint32_t sS[NU];
+  // bin counts of this workgroup (nbins + 1: the last one counts the dropped
+  // values), flushed to the global counters once at the end
+  extern __shared__ unsigned long long scnt[];
   // Sobol points unless the caller (or the flow stage) supplies the points
   const bool sob = (p.mode == MODE_RUN || p.mode == MODE_SAMPLE) && p.in_pts == nullptr;
   if (sob) {
     for (int i = threadIdx.x; i < N * 31; i += blockDim.x) sC[i] = p.Csr[i];
     if (threadIdx.x < N) sS[threadIdx.x] = p.shfr[threadIdx.x];
   }
+  for (int i = threadIdx.x; i <= p.nbins && p.nbins > 0; i += blockDim.x) scnt[i] = 0ull;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int64_t n = p.n_points;
@@ -325,7 +321,7 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
 #pragma unroll
       for (int i = 1; i < N; i++) tmin = fmin(tmin, x[i]);
       const int bi = bin_index(p, tmin);
-      atomicAdd(p.bin_counts + (bi >= 0 ? bi : p.nbins), 1ull);
+      atomicAdd(scnt + (bi >= 0 ? bi : p.nbins), 1ull);
     }
     if (p.nbins > 0 && dom && p.obs == OBS_KERNEL) {
       // an in-domain point adds 2^N entries per vertex (2^(N-1) per Keldysh
@@ -333,7 +329,7 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
 #pragma unroll
       for (int i = 0; i < N; i++) {
         const int bi = bin_index(p, x[i]);
-        atomicAdd(p.bin_counts + (bi >= 0 ? bi : p.nbins), bi >= 0 ? (1ull << (N > 0 ? N - 1 : 0)) : (1ull << N));
+        atomicAdd(scnt + (bi >= 0 ? bi : p.nbins), bi >= 0 ? (1ull << (N > 0 ? N - 1 : 0)) : (1ull << N));
       }
     }
     const uint64_t mask = __ballot(dom);
@@ -356,6 +352,11 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
       if (p.out_dom) p.out_dom[pi] = 0;
       if (p.out_jac) p.out_jac[pi] = jac;
     }
+  }
+  if (p.nbins > 0) {
+    __syncthreads();
+    for (int i = threadIdx.x; i <= p.nbins; i += blockDim.x)
+      if (scnt[i]) atomicAdd(p.bin_counts + i, scnt[i]);
   }
 }
 
@@ -572,19 +573,31 @@ __global__ __launch_bounds__(256) void pilot_hist_kernel(const double *__restric
   counts[t] = c;
 }
 
-// acc[b] += sum over blocks of part[blk][b] (fixed order), one thread per bin.
+// acc[b] += sum over blocks of part[blk][b]: one 256-thread workgroup per bin,
+// strided partial sums then a tree in LDS (a fixed order: deterministic).
 __global__ __launch_bounds__(256) void reduce_bins_kernel(const double2 *__restrict__ part, int nblk, int nbins,
                                                           double2 *__restrict__ acc) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  double re = 0.0, im = 0.0;
-  for (int k = 0; k < nblk; k++) {
+  __shared__ double2 sh[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  double2 s = make_double2(0.0, 0.0);
+  for (int k = t; k < nblk; k += 256) {
     const double2 v = part[(int64_t)k * nbins + b];
-    re += v.x;
-    im += v.y;
+    s.x += v.x;
+    s.y += v.y;
   }
-  acc[b].x += re;
-  acc[b].y += im;
+  sh[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[t].x += sh[t + w].x;
+      sh[t].y += sh[t + w].y;
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    acc[b].x += sh[0].x;
+    acc[b].y += sh[0].y;
+  }
 }
 
 // Deterministic sum of per-block partials into acc[0..3] (+=).
@@ -611,7 +624,8 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double *__re
 }
 
 template <int N> static hipError_t launch_prep_n(const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((prep_kernel<N>), dim3(grid), dim3(256), 0, st, p, buf);
+  const size_t shm = p.nbins > 0 ? sizeof(unsigned long long) * ((size_t)p.nbins + 1) : 0;
+  hipLaunchKernelGGL((prep_kernel<N>), dim3(grid), dim3(256), shm, st, p, buf);
   return hipGetLastError();
 }
 
@@ -680,7 +694,7 @@ hipError_t launch_pilot_hist(const double *l, const double *vals, int64_t n, int
 }
 
 hipError_t launch_reduce_bins(const double2 *part, int nblk, int nbins, double2 *acc, hipStream_t st) {
-  hipLaunchKernelGGL(reduce_bins_kernel, dim3((nbins + 255) / 256), dim3(256), 0, st, part, nblk, nbins, acc);
+  hipLaunchKernelGGL(reduce_bins_kernel, dim3(nbins), dim3(256), 0, st, part, nblk, nbins, acc);
   return hipGetLastError();
 }
 

@@ -22,7 +22,26 @@ template <int B, int E, typename F> __device__ __forceinline__ void static_for(F
 // The kernels form the threshold as max(|diagonal|, block maximum): the same
 // value, in a form that keeps their register allocation.
 template <typename T> struct PivotTol;
-template <> struct PivotTol<double> { static constexpr double v = 1.0 / 4096.0; };
-template <> struct PivotTol<float> { static constexpr float v = 1.0f / 256.0f; };
+// floor: a positive lower bound of the threshold's scale, so that a zero
+// pivot fails the check without a separate test.
+template <> struct PivotTol<double> {
+  static constexpr double v = 1.0 / 4096.0;
+  static constexpr double floor = 1e-290;
+};
+template <> struct PivotTol<float> {
+  static constexpr float v = 1.0f / 256.0f;
+  static constexpr float floor = 1e-30f;
+};
+
+// LDS layout of a point's Wick matrix: row/column 2i+s for vertex i with
+// Keldysh index s, 2N for the external point; row stride wick_stride(N) >= 2N+1,
+// padded so that the four candidates (2i+{0,1}) x (2j+{0,1}) of an entry --
+// what the lanes of a wave read at once, one per configuration -- fall in
+// different LDS banks (row stride mod 8 not in {0, 1, 7}, in 16-byte units).
+constexpr int wick_stride(int N) {
+  int s = 2 * N + 1;
+  while (s % 8 == 0 || s % 8 == 1 || s % 8 == 7) s++;
+  return s;
+}
 
 } // namespace kd

@@ -18,8 +18,10 @@ products) appended to the plain chain (applied first to the Sobol point),
 trained on the device for 2000 Adam steps of 2^14 points on 2^20 samples of
 the plain warped integrand (scramble seed 1000003, disjoint from the
 replicas), toward the density |f|^alpha for alpha in 1, 1/2, 1/4.  The plain
-estimator judges the candidates on 32 pilot replicas of 2^16 points (seeds
-20000+) and remains the fallback: the best of plain and the flows is measured.
+estimator and the candidates are scored on 32 pilot replicas of 2^16 points
+(seeds 20000+) by spread^2 x seconds per point; the best flow is measured, and
+it is the selected estimator only if its score is below 0.8 x the plain one's
+(plain is the fallback).
 
 For each estimator: time to 1e-3 = (points needed for 1e-3 relative error) /
 (points per second).  The points needed come from a doubling search: 64
@@ -45,6 +47,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 TARGET_REL = 1e-3
 PILOT_M = 1 << 16  # points per pilot replica when judging flow candidates
 PILOT_REPLICAS = 32
+SELECT_MARGIN = 0.8  # a flow is selected when its pilot score is below this fraction of the plain one
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "integrand_kernel_n8_fp64_pmc.md")
 
 
@@ -319,6 +322,7 @@ def main():
 
     plain = measure("plain")
     learned = None
+    learned_selected = False
     selection = []
     if not args.no_flow:
         from keldy_amd import warpers as kw
@@ -332,16 +336,23 @@ def main():
                 tr.emplace_back(flow)
             comp.warper = tr
 
-        def pilot_spread():
-            est, _, _, _ = run_replicas(comp, 20_000, PILOT_REPLICAS, PILOT_M)
-            return float(est.std(ddof=1) / abs(est.mean()))
+        def pilot():
+            """(relative spread, device seconds per point) on the pilot replicas"""
+            est, _, _, walls = run_replicas(comp, 20_000, PILOT_REPLICAS, PILOT_M)
+            return float(est.std(ddof=1) / abs(est.mean())), float(walls.sum()) / (PILOT_REPLICAS * PILOT_M)
 
-        # Candidates: flows trained toward |f|^alpha; the plain estimator judges
-        # them on pilot replicas (scramble seeds disjoint from the measured ones)
-        # and stays the fallback.
+        # Candidates: flows trained toward |f|^alpha.  Each is scored on pilot
+        # replicas (scramble seeds disjoint from the measured ones) by
+        # spread^2 x seconds per point (the time to a target error, up to a
+        # common factor); the best flow is measured, and it is the selected
+        # estimator only if its score beats the plain one's by SELECT_MARGIN
+        # (the pilot spreads carry ~10% noise); otherwise plain stays.
         set_chain(None)
-        selection.append({"candidate": "plain", "pilot_rel_spread": pilot_spread()})
-        best, best_spread, train_s, losses = None, selection[0]["pilot_rel_spread"], 0.0, None
+        sp0, spp0 = pilot()
+        plain_score = sp0 * sp0 * spp0
+        selection.append({"candidate": "plain", "pilot_rel_spread": sp0, "pilot_s_per_point": spp0,
+                          "score": plain_score})
+        best, best_score, train_s, best_alpha = None, float("inf"), 0.0, None
         for alpha in args.flow_alphas:
             set_chain(None)
             t0 = time.perf_counter()
@@ -351,17 +362,20 @@ def main():
             ts = time.perf_counter() - t0
             train_s += ts
             set_chain(flow)
-            sp = pilot_spread()
-            selection.append({"candidate": f"flow alpha={alpha}", "pilot_rel_spread": sp, "train_s": ts,
-                              "loss_first": float(ls[0]), "loss_last": float(np.mean(ls[-20:]))})
-            if sp < best_spread:
-                best, best_spread, losses, best_alpha = flow, sp, ls, alpha
+            sp, spp = pilot()
+            score = sp * sp * spp
+            selection.append({"candidate": f"flow alpha={alpha}", "pilot_rel_spread": sp, "pilot_s_per_point": spp,
+                              "score": score, "train_s": ts, "loss_first": float(ls[0]),
+                              "loss_last": float(np.mean(ls[-20:]))})
+            if score < best_score:
+                best, best_score, best_alpha = flow, score, alpha
         if best is not None:
             set_chain(best)
             learned = measure(f"learned flow (alpha={best_alpha})")
-        else:
-            set_chain(None)
-    head = learned or plain
+            learned_selected = best_score < SELECT_MARGIN * plain_score
+            if not learned_selected:
+                set_chain(None)
+    head = learned if (learned and learned_selected) else plain
 
     fpp = comp.flops_per_point()
     flops = float(head["dom"].sum()) * fpp
@@ -377,7 +391,8 @@ def main():
     estimator = {"plain": summary(plain), "setup_s": setup_s}
     if selection:
         estimator["flow_selection"] = {"pilot_points": PILOT_M, "pilot_replicas": PILOT_REPLICAS, "candidates": selection,
-                                       "selected": head["warper"], "training_s_total": train_s}
+                                       "selected": head["warper"], "select_margin": SELECT_MARGIN,
+                                       "training_s_total": train_s}
     if learned:
         pe, le = plain["est"], learned["est"]
         err = np.hypot(pe.std(ddof=1), le.std(ddof=1)) / np.sqrt(len(pe))
@@ -405,7 +420,7 @@ def main():
         "data": "synthetic (scrambled Sobol points; analytic flat-band g0)",
         "config": {"workload": "charge Q_n, n=8, t=10, flatband Gamma=1 T=0 V=0 eps_d=0, "
                                + ("learned 6-layer coupling flow on the plain fitted product-1D chain"
-                                  if learned else "plain fitted product-1D warper"),
+                                  if head is learned else "plain fitted product-1D warper"),
                    "order": args.order, "points_per_replica": args.points, "replicas": args.steps,
                    "pilot_points": args.pilot_points},
         "estimator": estimator,

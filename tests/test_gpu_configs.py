@@ -104,3 +104,39 @@ def test_c3_orders_1_to_10_one_launch():
         ref.run(counts[n - 1])
         a, b = batch[n - 1].reduce_result(), ref.reduce_result()
         assert abs(a - b) <= 1e-12 * abs(b), f"order {n}"
+
+
+def test_c5_current_order10_128_times_full_size():
+    """C5: current J at order 10 for 128 external times in (0, 20], 2^20 scrambled
+    Sobol points per time, semicircle dot-lead g0 (contour tabulation), chain
+    [plasma_uv(t), inverse(t, 1)]; all 128 integrators in one hipGraph launch.
+    Each equals its own run on the same points (same kernels, fixed-order
+    reductions: 1e-15 relative)."""
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    p = dict(beta=-1.0, bias_V=0.6, eps_d=-0.5, Gamma=1.0, time_max=20.0, nr_time_points_gf=2001, alpha=0.0,
+             half_bandwidth=2.0, bath_type="semicircle", ft_method="contour")
+    model = kio.G0Model(kio.G0ModelOmega(p), True)
+    order, m = 10, 1 << 20
+    times = np.linspace(20.0 / 128, 20.0, 128)
+
+    def make(t):
+        c = kio.ComputeCurrentJDirect(model, float(t), order, 0.0)
+        c.warper.emplace_back(kw.WarperPlasmaUvT(float(t)))
+        c.warper.emplace_back(kw.make_product_1d_simple_inverse(float(t), 1.0))
+        c.reset_rng("sobol", 0, True, True, 5)
+        return c
+
+    batch = [make(t) for t in times]
+    ms = kio.run_batch(batch, [m] * len(batch))
+    dom = sum(c.reduce_nr_points_in_domain() for c in batch)
+    J = np.array([-2 * (1j * c.reduce_result()).real for c in batch])
+    print(f"C5: 128 times x 2^20 points at order 10 in one graph launch: {ms:.0f} ms, "
+          f"{dom} points in domain ({dom / (128 * m):.2e}); J_10(t=20) = {J[-1]:.4e}")
+    assert np.all(np.isfinite(J))
+    for i in (0, 64, 127):
+        ref = make(times[i])
+        ref.run(m)
+        a, b = batch[i].reduce_result(), ref.reduce_result()
+        assert abs(a - b) <= 1e-15 * abs(b)
+        assert batch[i].reduce_nr_points_run() == m

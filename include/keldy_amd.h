@@ -64,6 +64,12 @@ extern "C" {
  * sharing the elimination of the other vertex rows -- a third fewer
  * multiply-adds at order 8, same arithmetic per configuration. */
 #define KD_ALGO_PAIR 2
+/* KD_ALGO_SPLIT (orders 6..10; otherwise KD_ALGO_CONFIG): the tree of Schur
+ * complements over the leading 2 (orders 6, 7) or 3 vertices, shared by the
+ * configurations below each node, then one bordered LU per configuration over
+ * the remaining vertices; one point per workgroup.  Order 8: ~3x fewer
+ * multiply-adds than KD_ALGO_CONFIG, same arithmetic up to reassociation. */
+#define KD_ALGO_SPLIT 3
 
 typedef struct kd_ctx kd_ctx;
 
@@ -203,10 +209,10 @@ int kd_get_time_bins(kd_ctx *ctx, double *values, uint64_t *counts, uint64_t *dr
  * No counterpart in the reference (its integrator runs one order per object).
  * kd_autotune times the Keldysh-sum kernel on sample_points Sobol points of
  * the ctx's chain (from the current generator state, which does not advance;
- * the accumulator is untouched) for every candidate -- the tree algorithm and
- * the per-configuration kernel at 64, 32 or 16 lanes per point, each at 8, 16
- * or 32 waves per CU -- and keeps the fastest for this ctx (lanes = 0 when the
- * tree wins).  kd_run_batch runs several
+ * the accumulator is untouched) for every candidate -- the split, paired and
+ * tree algorithms and the per-configuration kernel at 64, 32 or 16 lanes per
+ * point, each at 8, 16 or 32 waves per CU -- and keeps the fastest for this
+ * ctx (lanes = 0 when the tree or the split algorithm wins).  kd_run_batch runs several
  * ctxs (e.g. orders 1..10, uneven point counts) as ONE hipGraph launch: one
  * graph branch per ctx (its persistent kernels on its own stream), forked from
  * and joined to ctxs[0]'s stream; each ctx advances and accumulates as by

@@ -27,6 +27,7 @@ HIP_SOURCES = [
     "kernels/kd_tree.hip",
     "kernels/kd_flow.hip",
     "kernels/kd_cofactor.hip",
+    "kernels/kd_split.hip",
     "capi/kd_capi.cpp",
 ]
 

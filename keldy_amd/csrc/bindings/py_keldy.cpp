@@ -157,16 +157,22 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               v = KD_ALGO_CONFIG;
             else if (a == "pair")
               v = KD_ALGO_PAIR;
+            else if (a == "split")
+              v = KD_ALGO_SPLIT;
             else
-              throw keldy_error("algorithm must be 'config', 'pair' or 'tree'");
+              throw keldy_error("algorithm must be 'config', 'pair', 'split' or 'tree'");
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
-          py::arg("algorithm"), "Keldysh-sum algorithm: 'config' (default), 'pair' (orders >= 7) or 'tree' (orders >= 6)")
+          py::arg("algorithm"), "Keldysh-sum algorithm: 'config' (default), 'pair' (orders >= 7), 'split' (orders 6..10) or "
+          "'tree' (orders >= 6)")
       .def("get_algorithm",
            [](T &s) {
              int32_t v = KD_ALGO_CONFIG;
              s.ctx().check(kd_get_algorithm(s.ctx().get(), &v));
-             return std::string(v == KD_ALGO_TREE ? "tree" : v == KD_ALGO_PAIR ? "pair" : "config");
+             return std::string(v == KD_ALGO_TREE    ? "tree"
+                                : v == KD_ALGO_PAIR  ? "pair"
+                                : v == KD_ALGO_SPLIT ? "split"
+                                                     : "config");
            })
       .def("last_timing", &T::last_timing)
       .def("flops_per_point", &T::flops_per_point)

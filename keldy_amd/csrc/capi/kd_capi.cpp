@@ -123,6 +123,11 @@ bool use_pair(const kd_ctx *c) {
   return c->algo == KD_ALGO_PAIR && c->order >= 7 && c->obs == KD_OBS_DIRECT && c->lu_lanes_max == 64;
 }
 
+// the split tree + per-configuration LU (orders 6..10)
+bool use_split(const kd_ctx *c) {
+  return c->algo == KD_ALGO_SPLIT && c->obs == KD_OBS_DIRECT && kd::split_threads(c->order) > 0;
+}
+
 // complex values per point of an evaluation output
 int64_t vals_per_point(const kd_ctx *c) {
   return c->obs == KD_OBS_KERNEL ? (int64_t(1) << c->order) * c->order : 1;
@@ -132,9 +137,12 @@ int64_t vals_per_point(const kd_ctx *c) {
 // group), capped at lu_waves_per_cu waves per CU -- the kernel is persistent
 // over the rest.
 int grid_for(const kd_ctx *c, int64_t n_points) {
-  const int64_t per_batch = use_tree(c) ? 64 : kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
+  const bool per_point = use_tree(c) || use_split(c); // one workgroup item per point
+  const int64_t per_batch = per_point ? 64 : kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
   int64_t nb = (n_points + 63) / 64 * per_batch;
-  int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * c->lu_waves_per_cu;
+  // waves per CU -> workgroups per CU (the split kernel's workgroups hold several waves)
+  const int wg_waves = use_split(c) ? std::max(1, kd::split_threads(c->order) / 64) : 1;
+  int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * std::max(1, c->lu_waves_per_cu / wg_waves);
   if (nb > cap) nb = cap;
   return (int)(nb < 1 ? 1 : nb);
 }
@@ -335,6 +343,8 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
       KD_HIP(c, kd::launch_cofactor(N, p, buf, grid, c->stream));
     else if (use_tree(c))
       KD_HIP(c, kd::launch_tree(N, c->precision, p, buf, grid, c->stream));
+    else if (use_split(c))
+      KD_HIP(c, kd::launch_split(N, c->precision, p, buf, grid, c->stream));
     else
       KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, use_pair(c), p, buf, grid, c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
@@ -822,6 +832,24 @@ static double tree_flops(int N) {
   return f;
 }
 
+// split: per child node of the tree levels 1..K one pivot reciprocal (7),
+// Dc multipliers (6 each), Dc^2 multiply-adds (8 each) and the path
+// determinant (6); per configuration the bordered LU over M = n - K vertices,
+// det^2 s, the path determinant (6) and the signed FP64 sum
+static double split_flops(int N) {
+  const int K = N <= 7 ? 2 : 3, M = N - K;
+  double f = 0.0;
+  for (int k = 0; k < K; k++) {
+    const double dc = 2.0 * (N - k) - 1;
+    f += std::ldexp(8 * dc * dc + 6 * dc + 13, k + 1);
+  }
+  const double m = M;
+  const double macs = m * (m + 1) * (2 * m + 1) / 6.0;
+  const double muls = m * (m + 1) / 2.0 + m;
+  f += std::ldexp(8 * macs + 6 * muls + 7 * m + 14 + 6, N);
+  return f;
+}
+
 int kd_flops_per_point(const kd_ctx *c, double *flops) {
   if (!c || !flops) return KD_ERR_ARG;
   if (c->obs != KD_OBS_DIRECT) {
@@ -834,6 +862,10 @@ int kd_flops_per_point(const kd_ctx *c, double *flops) {
   }
   if (use_pair(c)) {
     *flops = pair_flops(c->order);
+    return KD_OK;
+  }
+  if (use_split(c)) {
+    *flops = split_flops(c->order);
     return KD_OK;
   }
   const double N = c->order < 0 ? 0 : c->order;
@@ -1136,16 +1168,18 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   p.n_points = sample_points;
   p.nbins = 0; // no bin counts from tuning launches
   c->no_reduce = true;
-  // candidates: the paired LU (orders >= 7), the tree (orders >= 6; both at
-  // 64 lanes) and the per-configuration kernel with 64, 32 or 16 lanes per
-  // point, each at 8, 16 or 32 waves per CU
+  // candidates: the split tree + per-configuration LU (orders 6..10), the
+  // paired LU (orders >= 7), the tree (orders >= 6; these at 64 lanes) and the
+  // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16
+  // or 32 waves per CU
   const bool direct = c->obs == KD_OBS_DIRECT;
-  const int li0 = !direct ? 0 : c->order >= 7 ? -2 : c->order >= 6 ? -1 : 0;
+  const int li0 = !direct ? 0 : kd::split_threads(c->order) > 0 ? -3 : c->order >= 7 ? -2 : c->order >= 6 ? -1 : 0;
   for (int li = li0; li < (direct ? 3 : 1) && !rc; li++) {
+    if (li == -2 && c->order < 7) continue;
     const int L = li < 0 ? 64 : lane_opts[li];
     if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
     for (int wi = 0; wi < 3 && !rc; wi++) {
-      c->algo = li == -2 ? KD_ALGO_PAIR : li == -1 ? KD_ALGO_TREE : KD_ALGO_CONFIG;
+      c->algo = li == -3 ? KD_ALGO_SPLIT : li == -2 ? KD_ALGO_PAIR : li == -1 ? KD_ALGO_TREE : KD_ALGO_CONFIG;
       c->lu_lanes_max = L;
       c->lu_waves_per_cu = wave_opts[wi];
       float t = 1e30f;
@@ -1172,7 +1206,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   c->lu_waves_per_cu = best_w;
   c->algo = best_a;
   c->tune_ms_per_mpt = best * 1048576.f / (float)sample_points;
-  if (lanes) *lanes = use_tree(c) ? 0 : kd::lu_lanes(c->order, best_l);
+  if (lanes) *lanes = use_tree(c) || use_split(c) ? 0 : kd::lu_lanes(c->order, best_l);
   if (waves_per_cu) *waves_per_cu = best_w;
   if (ms_per_mpoint) *ms_per_mpoint = c->tune_ms_per_mpt;
   return KD_OK;
@@ -1189,7 +1223,7 @@ int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
 
 int kd_set_algorithm(kd_ctx *c, int32_t algo) {
   if (!c) return KD_ERR_ARG;
-  if (algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE && algo != KD_ALGO_PAIR)
+  if (algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE && algo != KD_ALGO_PAIR && algo != KD_ALGO_SPLIT)
     return fail(c, KD_ERR_ARG, "unknown Keldysh-sum algorithm");
   c->algo = algo;
   return KD_OK;
@@ -1197,7 +1231,7 @@ int kd_set_algorithm(kd_ctx *c, int32_t algo) {
 
 int kd_get_algorithm(const kd_ctx *c, int32_t *algo) {
   if (!c || !algo) return KD_ERR_ARG;
-  *algo = use_tree(c) ? KD_ALGO_TREE : use_pair(c) ? KD_ALGO_PAIR : KD_ALGO_CONFIG;
+  *algo = use_tree(c) ? KD_ALGO_TREE : use_pair(c) ? KD_ALGO_PAIR : use_split(c) ? KD_ALGO_SPLIT : KD_ALGO_CONFIG;
   return KD_OK;
 }
 

@@ -29,6 +29,11 @@ hipError_t launch_lu(int order, int precision, int lanes, bool pair, const KdPar
 // integrand_tree_kernel<order, T> (orders 6..12): the Keldysh sum by the
 // Schur-complement tree, one point per 64-thread workgroup item.
 hipError_t launch_tree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
+// integrand_split_kernel<order, T> (orders 6..10): tree over the leading
+// vertices, then one bordered LU per configuration; one point per workgroup
+// of split_threads(order) threads (0: order not supported).
+int split_threads(int order);
+hipError_t launch_split(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // cofactor_kernel<order> (kernel method, orders 1..12, FP64): grid blocks of
 // 64 threads, same work items as the Keldysh-sum kernel.
 hipError_t launch_cofactor(int order, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);

@@ -153,3 +153,48 @@ def test_pair_matches_per_configuration_and_oracle(flat_g0, order, prec):
     ra, rb = a.run(1 << 12), b.run(1 << 12)
     assert ra.n_in_domain == rb.n_in_domain
     assert abs(complex(ra.re, ra.im) - complex(rb.re, rb.im)) <= tol * abs(complex(rb.re, rb.im)) * 10
+
+
+@pytest.mark.parametrize("order,prec", [(6, 0), (7, 0), (8, 0), (9, 0), (10, 0), (7, 1), (8, 1)])
+def test_split_matches_per_configuration_and_oracle(flat_g0, order, prec):
+    """The split algorithm (Schur tree over the leading vertices, then one
+    bordered LU per configuration) against the per-configuration LU on the same
+    points: FP64 to 1e-12 normwise, FP32 to 1e-4; against the oracle at order
+    8; the same domain flags; and the same accumulated sum through the driver,
+    with time bins."""
+    t = 10.0
+    rng = np.random.default_rng(70 + order)
+    n = 256 if order <= 9 else 96
+    u = rng.uniform(0.0, t, (n, order))
+    u[: n // 2] = -np.sort(-u[: n // 2], axis=1)
+    u[3, :] = 0.0
+    u[5, :] = u[5, 0]  # coinciding vertex times
+    a = U.charge_ctx(flat_g0, order, t, prec)
+    b = U.charge_ctx(flat_g0, order, t, prec)
+    a.set_algorithm(capi.KD_ALGO_SPLIT)
+    b.set_algorithm(capi.KD_ALGO_CONFIG)
+    assert a.algorithm() == capi.KD_ALGO_SPLIT and b.algorithm() == capi.KD_ALGO_CONFIG
+    va, da = a.eval_batch(u)
+    vb, db = b.eval_batch(u)
+    assert np.array_equal(da, db)
+    tol = 1e-12 if prec == 0 else 1e-4
+    assert np.max(np.abs(va - vb)) <= tol * np.max(np.abs(vb))
+    fa, fb = a.flops_per_point(), b.flops_per_point()
+    assert fa < fb
+    print(f"order {order}: split {fa:.0f} flops/point vs per-configuration {fb:.0f} ({fb / fa:.2f}x)")
+    if order == 8 and prec == 0:
+        ref, rdom = O.charge_integrand(U.oracle_charge(flat_g0), t).batch(u[:32])
+        assert np.array_equal(da[:32], rdom)
+        assert np.max(np.abs(va[:32] - ref)) <= 1e-10 * np.max(np.abs(ref))
+    s = O.Sobol(order, 0, True, True, 4)
+    Csr, shfr = s.tables()
+    for c in (a, b):
+        c.set_sobol(Csr, shfr, 0)
+        c.set_time_bins(0.0, t, 7)
+    m = 1 << 12
+    ra, rb = a.run(m), b.run(m)
+    assert ra.n_points == rb.n_points == m and ra.n_in_domain == rb.n_in_domain
+    assert abs(complex(ra.re, ra.im) - complex(rb.re, rb.im)) <= tol * abs(complex(rb.re, rb.im)) * 10
+    (ba, ca, xa), (bb, cb, xb) = a.time_bins(), b.time_bins()
+    assert np.array_equal(ca, cb) and xa == xb
+    assert np.max(np.abs(ba - bb)) <= tol * 10 * np.max(np.abs(bb))

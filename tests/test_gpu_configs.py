@@ -95,7 +95,8 @@ def test_c3_orders_1_to_10_one_launch():
     shapes = [c.autotune(1 << 16) for c in batch]
     ms = kio.run_batch(batch, counts)
     for n, c, (lanes, wpc, mpt) in zip(orders, batch, shapes):
-        print(f"C3 order {n}: {counts[n - 1]} points, tuned {'tree' if lanes == 0 else f'{int(lanes)} lanes/point'}"
+        algo = c.get_algorithm()
+        print(f"C3 order {n}: {counts[n - 1]} points, tuned {algo if lanes == 0 else f'{algo}, {int(lanes)} lanes/point'}"
               f", {int(wpc)} waves/CU, LU {mpt:.3f} ms per 2^20 points")
     print(f"C3: all orders in one graph launch: {ms:.1f} ms")
     for n in (1, 5, 10):

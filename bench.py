@@ -48,7 +48,9 @@ TARGET_REL = 1e-3
 PILOT_M = 1 << 16  # points per pilot replica when judging flow candidates
 PILOT_REPLICAS = 32
 SELECT_MARGIN = 0.8  # a flow is selected when its pilot score is below this fraction of the plain one
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "integrand_kernel_n8_fp64_pmc.md")
+# committed rocprofv3 --pmc summaries holding "pmc-traffic <kernel> ..." lines
+PMC_SUMMARIES = [os.path.join(ROOT, "profiles", f) for f in
+                 ("integrand_split_kernel_n8_fp64_pmc.md", "integrand_kernel_n8_fp64_pmc.md")]
 
 
 class _stdout_to_stderr:
@@ -72,22 +74,23 @@ def pmc_traffic(kernel, points_per_launch):
     full-size launches), each scaled by the factor that makes the same
     counter read a plain device copy of known size correctly.  None when the
     summary does not cover this kernel at this launch size."""
-    try:
-        text = open(PMC_SUMMARY).read()
-    except OSError:
-        return None
-    for line in text.splitlines():
-        f = line.split()
-        if len(f) < 2 or f[0] != "pmc-traffic" or f[1] != kernel:
+    for path in PMC_SUMMARIES:
+        try:
+            text = open(path).read()
+        except OSError:
             continue
-        kv = dict(x.split("=", 1) for x in f[2:])
-        if int(kv["points_per_launch"]) != points_per_launch:
-            continue
-        fetch = float(kv["fetch_kib"]) * 1024 * float(kv["fetch_factor"])
-        write = float(kv["write_kib"]) * 1024 * float(kv["write_factor"])
-        return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
-                "fetch_factor": float(kv["fetch_factor"]), "write_factor": float(kv["write_factor"]),
-                "source": os.path.relpath(PMC_SUMMARY, ROOT)}
+        for line in text.splitlines():
+            f = line.split()
+            if len(f) < 2 or f[0] != "pmc-traffic" or f[1] != kernel:
+                continue
+            kv = dict(x.split("=", 1) for x in f[2:])
+            if int(kv["points_per_launch"]) != points_per_launch:
+                continue
+            fetch = float(kv["fetch_kib"]) * 1024 * float(kv["fetch_factor"])
+            write = float(kv["write_kib"]) * 1024 * float(kv["write_factor"])
+            return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+                    "fetch_factor": float(kv["fetch_factor"]), "write_factor": float(kv["write_factor"]),
+                    "source": os.path.relpath(path, ROOT)}
     return None
 
 

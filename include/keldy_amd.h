@@ -50,13 +50,17 @@ extern "C" {
 #define KD_PREC_FP32_LU 1
 #define KD_PREC_FP32_LU_ALL 2
 
-/* Keldysh-sum algorithm.  KD_ALGO_CONFIG (the default): one bordered LU per
+/* Keldysh-sum algorithm.  KD_ALGO_DEFAULT (a new ctx's setting): KD_ALGO_SPLIT
+ * at orders 8..10, KD_ALGO_CONFIG otherwise -- the faster of the two on MI355X
+ * for full-size launches; kd_autotune measures every candidate instead.
+ * KD_ALGO_CONFIG: one bordered LU per
  * configuration sigma (2^n of them, sharing the staged Wick matrix).
  * KD_ALGO_TREE (orders >= 6; lower orders use KD_ALGO_CONFIG): the
  * configurations share the Schur complements of their common leading vertex
  * choices -- one table per node of the binary tree of choices -- so order 8
  * needs ~5x fewer flops, at lower issue efficiency; kd_autotune picks between
  * them.  Same arithmetic up to reassociation (FP64 results agree to ~1e-13). */
+#define KD_ALGO_DEFAULT (-1)
 #define KD_ALGO_CONFIG 0
 #define KD_ALGO_TREE 1
 /* KD_ALGO_PAIR (orders >= 7 at 64 lanes per point; otherwise KD_ALGO_CONFIG):

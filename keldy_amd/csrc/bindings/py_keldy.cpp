@@ -159,12 +159,14 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               v = KD_ALGO_PAIR;
             else if (a == "split")
               v = KD_ALGO_SPLIT;
+            else if (a == "default")
+              v = KD_ALGO_DEFAULT;
             else
-              throw keldy_error("algorithm must be 'config', 'pair', 'split' or 'tree'");
+              throw keldy_error("algorithm must be 'default', 'config', 'pair', 'split' or 'tree'");
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
-          py::arg("algorithm"), "Keldysh-sum algorithm: 'config' (default), 'pair' (orders >= 7), 'split' (orders 6..10) or "
-          "'tree' (orders >= 6)")
+          py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (split at orders 8..10, config otherwise), 'config', 'pair' "
+          "(orders >= 7), 'split' (orders 6..10) or 'tree' (orders >= 6)")
       .def("get_algorithm",
            [](T &s) {
              int32_t v = KD_ALGO_CONFIG;

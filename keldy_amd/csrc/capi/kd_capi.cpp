@@ -78,7 +78,7 @@ struct kd_ctx {
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
   // point and waves per CU of the persistent grid
   int lu_lanes_max = 64, lu_waves_per_cu = 16;
-  int algo = KD_ALGO_CONFIG; // Keldysh-sum algorithm (the tree applies from order 6 on)
+  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: split at orders 8..10)
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
   // time binning (kd_set_time_bins)
@@ -125,7 +125,8 @@ bool use_pair(const kd_ctx *c) {
 
 // the split tree + per-configuration LU (orders 6..10)
 bool use_split(const kd_ctx *c) {
-  return c->algo == KD_ALGO_SPLIT && c->obs == KD_OBS_DIRECT && kd::split_threads(c->order) > 0;
+  const bool chosen = c->algo == KD_ALGO_SPLIT || (c->algo == KD_ALGO_DEFAULT && c->order >= 8);
+  return chosen && c->obs == KD_OBS_DIRECT && kd::split_threads(c->order) > 0;
 }
 
 // complex values per point of an evaluation output
@@ -1223,7 +1224,8 @@ int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
 
 int kd_set_algorithm(kd_ctx *c, int32_t algo) {
   if (!c) return KD_ERR_ARG;
-  if (algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE && algo != KD_ALGO_PAIR && algo != KD_ALGO_SPLIT)
+  if (algo != KD_ALGO_DEFAULT && algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE && algo != KD_ALGO_PAIR &&
+      algo != KD_ALGO_SPLIT)
     return fail(c, KD_ERR_ARG, "unknown Keldysh-sum algorithm");
   c->algo = algo;
   return KD_OK;

@@ -213,18 +213,12 @@ template <int N, bool EV> __global__ __launch_bounds__(64) void cofactor_kernel(
     // vertex rows of the Wick matrix (wick_kernel.cpp:80-94), row/column 2v+k
     // for vertex v at Keldysh index k, then the external point X (split N) in
     // column 2N; row stride wick_stride(N)
-    for (int e = lane; e < ng * 2 * N * D; e += 64) {
-      const int q = e / (2 * N * D), rc = e - q * (2 * N * D);
-      const int r = rc / D, c = rc - r * D;
-      const CPt a = CPt{Us[q][r >> 1], r & 1, r >> 1, 0};
-      CPt bb;
-      if (c < 2 * N)
-        bb = CPt{Us[q][c >> 1], c & 1, c >> 1, 0};
-      else
-        bb = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
-      const double2 gv = g0_eval(p, a, bb, true);
-      W[q][r * DS + c] = gv;
-      if (c < 2 * N) atomicMax(&Ssc[q], (unsigned long long)__double_as_longlong(fabs(gv.x) + fabs(gv.y)));
+    // (wick_item: the vertex pairs, then the external column -- items N^2 + 2N ..)
+    constexpr int NI = N * N + 2 * N;
+    for (int e = lane; e < ng * NI; e += 64) {
+      const int q = e / NI, it = e - q * NI;
+      const double wm = wick_item<N>(p, Us[q], W[q], DS, it < N * N ? it : it + 2 * N);
+      if (wm > 0.0) atomicMax(&Ssc[q], (unsigned long long)__double_as_longlong(wm));
     }
     __syncthreads();
 

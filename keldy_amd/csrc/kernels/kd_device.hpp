@@ -168,6 +168,60 @@ __device__ __forceinline__ double2 g0_eval(const KdParams &p, const CPt &a, cons
   return v;
 }
 
+// One item of a point's Wick matrix build (wick_direct.cpp:92-101 picks), stored
+// interleaved by vertex: row/column 2i+s for vertex i with Keldysh index s, 2N
+// for the external points, row stride S.  Items q < N^2 are the vertex pairs
+// (i, j) = (q / N, q mod N): their four entries share the time difference
+// d = t_i - t_j, and compare_3way makes them g0^>(d) or g0^<(d) only --
+// (0,1): lesser; (1,0): greater; (0,0): greater when (i,0) is later on the
+// contour (d > 0, or d = 0 and i > j), else lesser; (1,1): the reverse; equal
+// points (i = j) give lesser(0) - i alpha -- so a pair needs two table
+// lookups instead of four.  Items N^2 .. N^2+4N are the external row (q - N^2
+// < 2N), the external column, and the corner (0; at order 0 g0(A, B)).  Returns the largest
+// |Re|+|Im| of the vertex-block entries written (0 for the others).
+template <int N>
+__device__ __forceinline__ double wick_item(const KdParams &p, const double *__restrict__ Us, double2 *__restrict__ W,
+                                            int S, int q) {
+  if (q < N * N) {
+    const int i = q / N, j = q - i * N;
+    const double d = Us[i] - Us[j];
+    const double2 gl = table_eval(p, p.lesser, 0, 0, d);
+    const double2 gg = table_eval(p, p.greater, 0, 0, d);
+    const int o = d > 0.0 ? 1 : d < 0.0 ? -1 : (i > j) - (i < j);
+    double2 g00 = o > 0 ? gg : gl;
+    double2 g11 = o < 0 ? gg : gl;
+    if (o == 0) {
+      g00.y -= p.alpha;
+      g11.y -= p.alpha;
+    }
+    double2 *w = W + 2 * i * S + 2 * j;
+    w[0] = g00;
+    w[1] = gl;
+    w[S] = gg;
+    w[S + 1] = g11;
+    return fmax(fmax(fabs(g00.x) + fabs(g00.y), fabs(gl.x) + fabs(gl.y)),
+                fmax(fabs(gg.x) + fabs(gg.y), fabs(g11.x) + fabs(g11.y)));
+  }
+  const int e = q - N * N;
+  if (e < 2 * N) {
+    const CPt a{p.A.time, p.A.k_idx, N, p.A.orbital};
+    const CPt b{Us[e >> 1], e & 1, e >> 1, 0};
+    W[2 * N * S + e] = g0_eval(p, a, b, true);
+  } else if (e < 4 * N) {
+    const int r = e - 2 * N;
+    const CPt a{Us[r >> 1], r & 1, r >> 1, 0};
+    const CPt b{p.B.time, p.B.k_idx, N, p.B.orbital};
+    W[r * S + 2 * N] = g0_eval(p, a, b, true);
+  } else if constexpr (N > 0) {
+    W[2 * N * S + 2 * N] = make_double2(0.0, 0.0);
+  } else { // order 0: the value is g0(A, B) itself
+    W[0] = g0_eval(p, CPt{p.A.time, p.A.k_idx, 0, p.A.orbital}, CPt{p.B.time, p.B.k_idx, 0, p.B.orbital}, false);
+  }
+  return 0.0;
+}
+// items of the build above
+template <int N> constexpr int wick_items() { return N * N + 4 * N + 1; }
+
 // ---------------------------------------------------------------------------
 // Sobol point k (digitalseq_b2g.hpp:415-426 / set_state 455-471): the state
 // after k steps is the XOR of the columns selected by the Gray code of k.

@@ -368,26 +368,13 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
       if (lane < ng * N) Us[lane / NU][lane % NU] = buf.u[slot0 * N + lane];
       if (lane < PPW) Ssc[lane] = 0ull;
       __syncthreads();
-      // Stage the Wick matrices of the group's points in LDS (wick_direct.cpp:92-101).
-      for (int e = lane; e < ng * D * D; e += 64) {
-        const int q = e / (D * D), rc = e - q * (D * D);
-        const int r = rc / D, c = rc - r * D;
-        CPt a, bb;
-        if (r < 2 * N) {
-          a = CPt{Us[q][r >> 1], r & 1, r >> 1, 0};
-        } else {
-          a = CPt{p.A.time, p.A.k_idx, N, p.A.orbital};
-        }
-        if (c < 2 * N) {
-          bb = CPt{Us[q][c >> 1], c & 1, c >> 1, 0};
-        } else {
-          bb = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
-        }
-        const bool corner = (r == 2 * N) && (c == 2 * N);
-        double2 gv = g0_eval(p, a, bb, !corner);
-        if (corner && N > 0) gv = make_double2(0.0, 0.0);
-        W[q][r * DS + c] = gv;
-        if (r < 2 * N && c < 2 * N) atomicMax(&Ssc[q], (unsigned long long)__double_as_longlong(fabs(gv.x) + fabs(gv.y)));
+      // Stage the Wick matrices of the group's points in LDS (wick_direct.cpp:92-101),
+      // one vertex pair or external entry per item (wick_item).
+      constexpr int NI = wick_items<N>();
+      for (int e = lane; e < ng * NI; e += 64) {
+        const int q = e / NI;
+        const double wm = wick_item<N>(p, Us[q], W[q], DS, e - q * NI);
+        if (wm > 0.0) atomicMax(&Ssc[q], (unsigned long long)__double_as_longlong(wm));
       }
       __syncthreads();
 

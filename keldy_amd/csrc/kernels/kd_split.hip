@@ -230,27 +230,10 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
       fpath[1] = 0;
     }
     __syncthreads();
-    // Wick matrix, interleaved by vertex (wick_direct.cpp:92-101 picks)
-    double wmax = 0.0; // this thread's largest |Re|+|Im| of the vertex block
-    for (int e = tq; e < D0 * D0; e += TPB) {
-      const int r = e / D0, c = e - r * D0;
-      CPt a, bb;
-      if (r < 2 * N) {
-        a = CPt{Us[r >> 1], r & 1, r >> 1, 0};
-      } else {
-        a = CPt{p.A.time, p.A.k_idx, N, p.A.orbital};
-      }
-      if (c < 2 * N) {
-        bb = CPt{Us[c >> 1], c & 1, c >> 1, 0};
-      } else {
-        bb = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
-      }
-      const bool corner = (r == 2 * N) && (c == 2 * N);
-      double2 gv = g0_eval(p, a, bb, !corner);
-      if (corner) gv = make_double2(0.0, 0.0);
-      W[r * S0 + c] = gv;
-      if (r < 2 * N && c < 2 * N) wmax = fmax(wmax, fabs(gv.x) + fabs(gv.y));
-    }
+    // Wick matrix, interleaved by vertex (wick_direct.cpp:92-101 picks);
+    // wmax: this thread's largest |Re|+|Im| of the vertex block
+    double wmax = 0.0;
+    for (int q = tq; q < wick_items<N>(); q += TPB) wmax = fmax(wmax, wick_item<N>(p, Us, W, S0, q));
     // block maximum: across the wave by shuffles, then one LDS atomic per wave
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) wmax = fmax(wmax, __shfl_xor(wmax, off, 64));

@@ -167,25 +167,11 @@ __global__ __launch_bounds__(64) void integrand_tree_kernel(KdParams p, PrepBuf 
     if (lane < N) Us[lane] = buf.u[slot * N + lane];
     if (lane == 0) Ssc = 0ull;
     __syncthreads();
-    // Wick matrix in level-0 order (wick_direct.cpp:92-101 picks, interleaved)
-    for (int e = lane; e < D0 * D0; e += 64) {
-      const int r = e / D0, c = e - r * D0;
-      CPt a, bb;
-      if (r < 2 * N) {
-        a = CPt{Us[r >> 1], r & 1, r >> 1, 0};
-      } else {
-        a = CPt{p.A.time, p.A.k_idx, N, p.A.orbital};
-      }
-      if (c < 2 * N) {
-        bb = CPt{Us[c >> 1], c & 1, c >> 1, 0};
-      } else {
-        bb = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
-      }
-      const bool corner = (r == 2 * N) && (c == 2 * N);
-      double2 gv = g0_eval(p, a, bb, !corner);
-      if (corner) gv = make_double2(0.0, 0.0);
-      W[e] = gv;
-      if (r < 2 * N && c < 2 * N) atomicMax(&Ssc, (unsigned long long)__double_as_longlong(fabs(gv.x) + fabs(gv.y)));
+    // Wick matrix in level-0 order (wick_direct.cpp:92-101 picks, interleaved),
+    // one vertex pair or external entry per item (wick_item)
+    for (int q = lane; q < wick_items<N>(); q += 64) {
+      const double wm = wick_item<N>(p, Us, W, D0, q);
+      if (wm > 0.0) atomicMax(&Ssc, (unsigned long long)__double_as_longlong(wm));
     }
     __syncthreads();
     const T tabs = (T)(PivotTol<T>::v * __longlong_as_double((long long)Ssc));

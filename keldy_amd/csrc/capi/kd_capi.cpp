@@ -836,13 +836,32 @@ static double tree_flops(int N) {
 // split: per child node of the tree levels 1..K one pivot reciprocal (7),
 // Dc multipliers (6 each), Dc^2 multiply-adds (8 each) and the path
 // determinant (6); per configuration the bordered LU over M = n - K vertices,
-// det^2 s, the path determinant (6) and the signed FP64 sum
+// det^2 s, the path determinant (6) and the signed FP64 sum -- or, where the
+// kernel pairs the two choices of the last vertex, the paired LU's count per
+// pair (as pair_flops over M vertices) plus two path determinants
 static double split_flops(int N) {
   const int K = N <= 7 ? 2 : 3, M = N - K;
   double f = 0.0;
   for (int k = 0; k < K; k++) {
     const double dc = 2.0 * (N - k) - 1;
     f += std::ldexp(8 * dc * dc + 6 * dc + 13, k + 1);
+  }
+  if (kd::split_paired(N)) {
+    const int MC = M + 2;
+    double macs = 0, muls = 0, invs = 0;
+    for (int i = 0; i < M - 1; i++) {
+      for (int k = 0; k < i; k++) macs += MC - 1 - k;
+      invs += 1;
+      muls += (MC - 1 - i) + 1;
+    }
+    for (int k = 0; k < M - 1; k++) macs += MC - 1 - k;
+    for (int t = 0; t < 2; t++) {
+      for (int k = 0; k < M - 1; k++) macs += (M - 2 - k) + 2;
+      macs += 1;
+      invs += 1;
+      muls += 4 + 1; // det, u, d^2, value; the path determinant
+    }
+    return f + std::ldexp(8 * macs + 6 * muls + 7 * invs + 4, N - 1);
   }
   const double m = M;
   const double macs = m * (m + 1) * (2 * m + 1) / 6.0;

@@ -33,6 +33,8 @@ hipError_t launch_tree(int order, int precision, const KdParams &p, const PrepBu
 // vertices, then one bordered LU per configuration; one point per workgroup
 // of split_threads(order) threads (0: order not supported).
 int split_threads(int order);
+// whether it runs the configurations as pairs sharing the last vertex's elimination
+bool split_paired(int order);
 hipError_t launch_split(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // cofactor_kernel<order> (kernel method, orders 1..12, FP64): grid blocks of
 // 64 threads, same work items as the Keldysh-sum kernel.

@@ -36,6 +36,10 @@
 
 namespace kd {
 
+// paired LUs where they fit (SplitShape::PAIR)
+#ifndef KD_SPLIT_PAIR
+#define KD_SPLIT_PAIR 1
+#endif
 // threads per workgroup (one point per workgroup) at orders >= 8
 #ifndef KD_SPLIT_THREADS
 #define KD_SPLIT_THREADS 128
@@ -47,6 +51,9 @@ template <int N> struct SplitShape {
   static constexpr int TPB = (1 << N) < KD_SPLIT_THREADS ? (1 << N) : KD_SPLIT_THREADS;
   static constexpr int NW = TPB / 64;                     // waves per workgroup
   static constexpr int CPT = (1 << N) / TPB;              // configurations per thread
+  // paired LUs (split_config_pair) at 2 or more configurations per thread; with
+  // more than 5 vertices per LU the pair's registers spill
+  static constexpr bool PAIR = KD_SPLIT_PAIR && CPT >= 2 && M >= 2 && M <= 5;
   static constexpr int S0 = wick_stride(N);               // Wick matrix row stride
   static constexpr int Dk(int k) { return 2 * (N - k) + 1; }
   static constexpr int Sk(int k) { return k == 0 ? S0 : wick_stride(N - k); }
@@ -162,6 +169,108 @@ __device__ __forceinline__ double2 split_config(const PT *__restrict__ Tn, int l
   return cmul<double>(cmul<double>(dt, dt), make_double2((double)s.x, (double)s.y));
 }
 
+// det(A)^2 s of the two configurations that differ only in the Keldysh index of
+// the last vertex (local bit M-1 of loc: 0 for v[0], 1 for v[1]), from the same
+// node table.  The first M-1 vertex rows are eliminated once over the columns
+// of both choices of the last vertex (and the border), the border row likewise;
+// only the last vertex row and the corner update are per choice -- at M = 5,
+// 70 instead of 110 complex multiply-adds and 6 instead of 10 reciprocals per
+// pair, the same arithmetic per configuration as split_config.
+template <int M, typename T, typename PT>
+__device__ __forceinline__ void split_config_pair(const PT *__restrict__ Tn, int loc, const T *__restrict__ thr,
+                                                  double2 dp, double2 (&v)[2], bool (&bad)[2]) {
+  constexpr int S = wick_stride(M);
+  constexpr int MC = M + 2; // columns: vertices 0..M-2, last vertex (index 0), last vertex (index 1), border
+  int pc[MC];
+  static_for<0, M - 1>([&](auto I) {
+    constexpr int i = I;
+    pc[i] = 2 * i + ((loc >> i) & 1);
+  });
+  pc[M - 1] = 2 * M - 2;
+  pc[M] = 2 * M - 1;
+  pc[M + 1] = 2 * M;
+  cx<T> Uh[M - 1][MC];
+  cx<T> det = cmk<T>(T(1), T(0));
+  bool flag = false;
+  static_for<0, M - 1>([&](auto I) {
+    constexpr int i = I;
+    cx<T> row[MC];
+    int ri = pc[i] * S;
+    __asm__ volatile("" : "+v"(ri));
+    static_for<0, MC>([&](auto J) {
+      constexpr int j = J;
+      row[j] = ldx<T>(Tn[ri + pc[j]]);
+    });
+    static_for<0, i>([&](auto KB) {
+      constexpr int kb = KB;
+      const cx<T> l = row[kb];
+      static_for<kb + 1, MC>([&](auto J) {
+        constexpr int j = J;
+        row[j] = cfms<T>(row[j], l, Uh[kb][j]);
+      });
+    });
+    const cx<T> piv = row[i];
+    flag |= !(cabs1<T>(piv) >= thr[pc[i]]);
+    if constexpr (i == 0)
+      det = piv;
+    else
+      det = cmul<T>(det, piv);
+    const cx<T> inv = cinv<T>(piv);
+    static_for<i + 1, MC>([&](auto J) {
+      constexpr int j = J;
+      Uh[i][j] = cmul<T>(row[j], inv);
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  // border row over the same columns (its border entry is the node's corner)
+  cx<T> rb[MC];
+  {
+    int ri = 2 * M * S;
+    __asm__ volatile("" : "+v"(ri));
+    static_for<0, MC>([&](auto J) {
+      constexpr int j = J;
+      rb[j] = ldx<T>(Tn[ri + pc[j]]);
+    });
+  }
+  static_for<0, M - 1>([&](auto KB) {
+    constexpr int kb = KB;
+    const cx<T> l = rb[kb];
+    static_for<kb + 1, MC>([&](auto J) {
+      constexpr int j = J;
+      rb[j] = cfms<T>(rb[j], l, Uh[kb][j]);
+    });
+  });
+  __builtin_amdgcn_sched_barrier(0);
+  // the last vertex row for each Keldysh index t: its own column and the border
+  static_for<0, 2>([&](auto TT) {
+    constexpr int t = TT;
+    constexpr int ct = M - 1 + t; // its column
+    cx<T> row[MC];
+    int ri = pc[ct] * S;
+    __asm__ volatile("" : "+v"(ri));
+    static_for<0, MC>([&](auto J) {
+      constexpr int j = J;
+      if constexpr (j < M - 1 || j == ct || j == MC - 1) row[j] = ldx<T>(Tn[ri + pc[j]]);
+    });
+    static_for<0, M - 1>([&](auto KB) {
+      constexpr int kb = KB;
+      const cx<T> l = row[kb];
+      static_for<kb + 1, MC>([&](auto J) {
+        constexpr int j = J;
+        if constexpr (j < M - 1 || j == ct || j == MC - 1) row[j] = cfms<T>(row[j], l, Uh[kb][j]);
+      });
+    });
+    const cx<T> piv = row[ct];
+    bad[t] = flag | !(cabs1<T>(piv) >= thr[pc[ct]]);
+    const cx<T> dt = cmul<T>(det, piv);
+    const cx<T> u = cmul<T>(row[MC - 1], cinv<T>(piv));
+    const cx<T> sc = cfms<T>(rb[MC - 1], rb[ct], u);
+    const double2 dd = make_double2(dp.x * (double)dt.x - dp.y * (double)dt.y, dp.x * (double)dt.y + dp.y * (double)dt.x);
+    v[t] = cmul<double>(cmul<double>(dd, dd), make_double2((double)sc.x, (double)sc.y));
+    __builtin_amdgcn_sched_barrier(0);
+  });
+}
+
 #ifndef KD_SPLIT_WAVES_PER_EU
 #define KD_SPLIT_WAVES_PER_EU 2
 #endif
@@ -258,30 +367,65 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
     });
     const cx<T> *LK = K % 2 ? R1 : R0;
 
-    // the configurations: c = mm * TPB + t, local bits c mod 2^M, node c >> M
+    // The configurations: slot m of this thread.  Single: c = m TPB + t, local
+    // bits c mod 2^M, node c >> M.  Paired (2 or more per thread): pair
+    // c = (m / 2) TPB + t, local bits 0..M-2 from c mod 2^(M-1), node
+    // c >> (M-1), last vertex's index m mod 2.
+    constexpr bool PAIR = S::PAIR;
+    auto cfg_of = [&](int m) {
+      if constexpr (PAIR) {
+        const int c = (m >> 1) * TPB + tq;
+        const int loc = (c & ((1 << (M - 1)) - 1)) | ((m & 1) << (M - 1));
+        return (c >> (M - 1)) | (loc << K);
+      } else {
+        const int c = m * TPB + tq;
+        return (c >> M) | ((c & ((1 << M) - 1)) << K);
+      }
+    };
     double sre = 0.0, sim = 0.0;
     uint32_t badmask = 0;
+    if constexpr (PAIR) {
 #pragma unroll 1
-    for (int mm = 0; mm < CPT; mm++) {
-      const int c = mm * TPB + tq;
-      const int loc = c & ((1 << M) - 1), node = c >> M;
-      const int cfg = node | (loc << K);
-      bool bad;
-      const double2 v = split_config<M, T>(LK + node * (S::Dk(K) * S::Sk(K)), loc, thr + 2 * K,
-                                           dpath[(1 << K) + node], bad);
-      bad = bad || fpath[(1 << K) + node] != 0;
-      const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
-      sre += bad ? 0.0 : sg * v.x;
-      sim += bad ? 0.0 : sg * v.y;
-      badmask |= (uint32_t)bad << mm;
+      for (int mm = 0; mm < CPT / 2; mm++) {
+        const int c = mm * TPB + tq;
+        const int loc = c & ((1 << (M - 1)) - 1), node = c >> (M - 1);
+        const int cfg0 = node | (loc << K);
+        double2 v[2];
+        bool bad[2];
+        split_config_pair<M, T>(LK + node * (S::Dk(K) * S::Sk(K)), loc, thr + 2 * K, dpath[(1 << K) + node], v,
+                                bad);
+        const bool fp = fpath[(1 << K) + node] != 0;
+        bad[0] = bad[0] || fp;
+        bad[1] = bad[1] || fp;
+        const double sg = (__popc(cfg0) & 1) ? -1.0 : 1.0; // the index-1 configuration has the other sign
+        sre += bad[0] ? 0.0 : sg * v[0].x;
+        sim += bad[0] ? 0.0 : sg * v[0].y;
+        sre += bad[1] ? 0.0 : -sg * v[1].x;
+        sim += bad[1] ? 0.0 : -sg * v[1].y;
+        badmask |= ((uint32_t)bad[0] << (2 * mm)) | ((uint32_t)bad[1] << (2 * mm + 1));
+      }
+    } else {
+#pragma unroll 1
+      for (int mm = 0; mm < CPT; mm++) {
+        const int c = mm * TPB + tq;
+        const int loc = c & ((1 << M) - 1), node = c >> M;
+        const int cfg = node | (loc << K);
+        bool bad;
+        const double2 v = split_config<M, T>(LK + node * (S::Dk(K) * S::Sk(K)), loc, thr + 2 * K,
+                                             dpath[(1 << K) + node], bad);
+        bad = bad || fpath[(1 << K) + node] != 0;
+        const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
+        sre += bad ? 0.0 : sg * v.x;
+        sim += bad ? 0.0 : sg * v.y;
+        badmask |= (uint32_t)bad << mm;
+      }
     }
     // flagged configurations: partial-pivoting FP64 LU, outside the hot loop
     int nfb = 0;
     while (badmask) {
-      const int mm = __builtin_ctz(badmask);
+      const int m = __builtin_ctz(badmask);
       badmask &= badmask - 1;
-      const int c = mm * TPB + tq;
-      const int cfg = (c >> M) | ((c & ((1 << M) - 1)) << K);
+      const int cfg = cfg_of(m);
       const double2 v = config_value_pivoted(W, cfg, N, S0);
       const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
       sre += sg * v.x;
@@ -378,6 +522,23 @@ int split_threads(int order) {
     return SplitShape<10>::TPB;
   default:
     return 0;
+  }
+}
+
+bool split_paired(int order) {
+  switch (order) {
+  case 6:
+    return SplitShape<6>::PAIR;
+  case 7:
+    return SplitShape<7>::PAIR;
+  case 8:
+    return SplitShape<8>::PAIR;
+  case 9:
+    return SplitShape<9>::PAIR;
+  case 10:
+    return SplitShape<10>::PAIR;
+  default:
+    return false;
   }
 }
 

@@ -31,6 +31,12 @@ last two M.  Points per second come from the timed steps (`--points` per step,
 replica seeds 0..steps-1, the same Sobol points for both estimators).  value =
 the selected estimator's time (the plain one's with --no-flow).  One-off set-up
 (g0 tabulation, pilot fit, flow training) is reported separately.
+
+Side figure (estimator.plain_fp32_lu, skipped by --no-fp32): the plain
+estimator with the determinants LU'd in FP32 and the Keldysh sum and the
+accumulation in FP64, on the same replicas -- its points per second, its time
+to 1e-3 at the FP64 search's points needed, and each replica's deviation from
+the FP64 estimate of the same points.
 """
 import argparse
 import json
@@ -256,6 +262,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flow", action="store_true", help="plain estimator only")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the FP32-LU side measurement of the plain estimator")
     ap.add_argument("--flow-steps", type=int, default=2000)
     ap.add_argument("--flow-samples", type=int, default=1 << 20)
     ap.add_argument("--flow-batch", type=int, default=1 << 14)
@@ -324,6 +332,24 @@ def main():
                 "in_domain_fraction": m["in_domain_fraction"]}
 
     plain = measure("plain")
+    # The north star's precision split (LU in FP32, the Keldysh sum and the
+    # accumulation in FP64) on the same chain and the same scrambles as the
+    # plain FP64 replicas: throughput, and each replica's deviation from its
+    # FP64 estimate.  The points needed are the FP64 search's (the two
+    # estimators differ by far less than their spread).
+    fp32_side = None
+    if args.precision == "fp64" and not args.no_fp32:
+        comp.set_precision("fp32")
+        run_replicas(comp, 1000, 1, args.points)
+        est32, _, lu32, walls32 = run_replicas(comp, 0, args.steps, args.points)
+        comp.set_precision("fp64")
+        pps32 = args.steps * args.points / float(walls32.sum())
+        dev = np.abs(est32 - plain["est"]) / abs(plain["mean"])
+        fp32_side = {"dtype": "f32-lu/f64-sum", "time_to_1e-3_s": plain["n_needed"] / pps32,
+                     "points_per_s": pps32, "speedup_vs_fp64": pps32 / plain["pps"],
+                     "lu_ms_per_launch": float(lu32.mean()) / max(1, -(-args.points // (1 << 21))),
+                     "max_rel_deviation_from_fp64_per_replica": float(dev.max()),
+                     "rel_spread_per_replica": plain["rel_rep"]}
     learned = None
     learned_selected = False
     selection = []
@@ -392,6 +418,8 @@ def main():
     traffic = pmc_traffic(kname, min(args.points, 1 << 21))
 
     estimator = {"plain": summary(plain), "setup_s": setup_s}
+    if fp32_side:
+        estimator["plain_fp32_lu"] = fp32_side
     if selection:
         estimator["flow_selection"] = {"pilot_points": PILOT_M, "pilot_replicas": PILOT_REPLICAS, "candidates": selection,
                                        "selected": head["warper"], "select_margin": SELECT_MARGIN,

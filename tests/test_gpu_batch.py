@@ -155,13 +155,14 @@ def test_pair_matches_per_configuration_and_oracle(flat_g0, order, prec):
     assert abs(complex(ra.re, ra.im) - complex(rb.re, rb.im)) <= tol * abs(complex(rb.re, rb.im)) * 10
 
 
-@pytest.mark.parametrize("order,prec", [(6, 0), (7, 0), (8, 0), (9, 0), (10, 0), (7, 1), (8, 1)])
+@pytest.mark.parametrize("order,prec", [(6, 0), (7, 0), (8, 0), (9, 0), (10, 0), (7, 1), (8, 1), (9, 1), (10, 2)])
 def test_split_matches_per_configuration_and_oracle(flat_g0, order, prec):
     """The split algorithm (Schur tree over the leading vertices, then one
     bordered LU per configuration) against the per-configuration LU on the same
-    points: FP64 to 1e-12 normwise, FP32 to 1e-4; against the oracle at order
-    8; the same domain flags; and the same accumulated sum through the driver,
-    with time bins."""
+    points: FP64 to 1e-12 normwise, FP32 to 1e-4 (1e-2 with FP32 at order 10,
+    where both FP32 results carry the cancellation of 1024 terms); against the
+    oracle at order 8; the same domain flags; and the same accumulated sum
+    through the driver, with time bins."""
     t = 10.0
     rng = np.random.default_rng(70 + order)
     n = 256 if order <= 9 else 96
@@ -177,7 +178,7 @@ def test_split_matches_per_configuration_and_oracle(flat_g0, order, prec):
     va, da = a.eval_batch(u)
     vb, db = b.eval_batch(u)
     assert np.array_equal(da, db)
-    tol = 1e-12 if prec == 0 else 1e-4
+    tol = 1e-12 if prec == 0 else 1e-4 if prec == 1 else 1e-2
     assert np.max(np.abs(va - vb)) <= tol * np.max(np.abs(vb))
     fa, fb = a.flops_per_point(), b.flops_per_point()
     assert fa < fb

@@ -141,22 +141,39 @@ __device__ __forceinline__ int compare_3way(const CPt &a, const CPt &b) {
   return (1 - 2 * a.k_idx) * (a.split - b.split);
 }
 
+// table_eval in two halves: the position of t on the mesh (a 32-bit index),
+// then the interpolation.
+struct TabPos {
+  int i;     // left mesh point, clamped to [0, n_pts - 2]
+  double w;  // weight of the right one
+  bool out;  // outside the mesh by more than rounding: NaN (the reference's gf evaluation throws)
+};
+
+__device__ __forceinline__ TabPos table_pos(const KdParams &p, double t) {
+  const double a = (t - p.t_min) / p.dt;
+  long i = (long)floor(a);
+  TabPos q;
+  q.out = !(a >= -1e-9) || !(a <= (double)(p.n_pts - 1) + 1e-9);
+  i = i > p.n_pts - 2 ? p.n_pts - 2 : i;
+  i = i < 0 ? 0 : i;
+  q.i = (int)i;
+  q.w = a - (double)i;
+  return q;
+}
+
+__device__ __forceinline__ double2 table_mix(double2 g0, double2 g1, const TabPos &q) {
+  double2 r;
+  r.x = (1.0 - q.w) * g0.x + q.w * g1.x;
+  r.y = (1.0 - q.w) * g0.y + q.w * g1.y;
+  if (q.out) r.x = r.y = __builtin_nan("");
+  return r;
+}
+
 __device__ __forceinline__ double2 table_eval(const KdParams &p, const double2 *tab, int oa, int ob,
                                               double t) {
   const double2 *g = tab + (size_t)(oa * p.n_orb + ob) * p.n_pts;
-  double a = (t - p.t_min) / p.dt;
-  long i = (long)floor(a);
-  // outside the mesh by more than rounding: NaN (the reference's gf evaluation throws)
-  const bool out = !(a >= -1e-9) || !(a <= (double)(p.n_pts - 1) + 1e-9);
-  i = i > p.n_pts - 2 ? p.n_pts - 2 : i;
-  i = i < 0 ? 0 : i;
-  double w = a - (double)i;
-  double2 g0 = g[i], g1 = g[i + 1];
-  double2 r;
-  r.x = (1.0 - w) * g0.x + w * g1.x;
-  r.y = (1.0 - w) * g0.y + w * g1.y;
-  if (out) r.x = r.y = __builtin_nan("");
-  return r;
+  const TabPos q = table_pos(p, t);
+  return table_mix(g[q.i], g[q.i + 1], q);
 }
 
 __device__ __forceinline__ double2 g0_eval(const KdParams &p, const CPt &a, const CPt &b, bool internal) {
@@ -177,47 +194,109 @@ __device__ __forceinline__ double2 g0_eval(const KdParams &p, const CPt &a, cons
 // contour (d > 0, or d = 0 and i > j), else lesser; (1,1): the reverse; equal
 // points (i = j) give lesser(0) - i alpha -- so a pair needs two table
 // lookups instead of four.  Items N^2 .. N^2+4N are the external row (q - N^2
-// < 2N), the external column, and the corner (0; at order 0 g0(A, B)).  Returns the largest
-// |Re|+|Im| of the vertex-block entries written (0 for the others).
+// < 2N), the external column, and the corner (0; at order 0 g0(A, B)).
+// wick_store and wick_item return the largest |Re|+|Im| of the vertex-block
+// entries written (0 for the others).
+//
+// The build item in two halves: wick_fetch loads the table entries the item
+// needs from its two times (ta = the row side's, A's for the external row; tb =
+// the column side's, B's for the external column), wick_store interpolates and
+// writes the entries; wick_item does both.
+struct WickPre {
+  double2 l0, l1, g0, g1; // pair: lesser and greater entries around the mesh position; external: the table's
+  TabPos pos;
+  int o; // pair: the (0,0) order (compare_3way) and equal points; external: the alpha correction applies
+};
+
+template <int N> __device__ __forceinline__ void wick_item_vertices(int q, int &ia, int &ib) {
+  if (q < N * N) {
+    ia = q / N;
+    ib = q - ia * N;
+    return;
+  }
+  const int e = q - N * N;
+  ia = e < 2 * N ? -1 : e < 4 * N ? (e - 2 * N) >> 1 : -1;
+  ib = e < 2 * N ? e >> 1 : -1;
+}
+
 template <int N>
-__device__ __forceinline__ double wick_item(const KdParams &p, const double *__restrict__ Us, double2 *__restrict__ W,
-                                            int S, int q) {
+__device__ __forceinline__ void wick_fetch(const KdParams &p, int q, double ta, double tb, WickPre &w) {
   if (q < N * N) {
     const int i = q / N, j = q - i * N;
-    const double d = Us[i] - Us[j];
-    const double2 gl = table_eval(p, p.lesser, 0, 0, d);
-    const double2 gg = table_eval(p, p.greater, 0, 0, d);
-    const int o = d > 0.0 ? 1 : d < 0.0 ? -1 : (i > j) - (i < j);
-    double2 g00 = o > 0 ? gg : gl;
-    double2 g11 = o < 0 ? gg : gl;
-    if (o == 0) {
+    const double d = ta - tb;
+    w.pos = table_pos(p, d);
+    w.l0 = p.lesser[w.pos.i];
+    w.l1 = p.lesser[w.pos.i + 1];
+    w.g0 = p.greater[w.pos.i];
+    w.g1 = p.greater[w.pos.i + 1];
+    w.o = d > 0.0 ? 1 : d < 0.0 ? -1 : (i > j) - (i < j);
+    return;
+  }
+  const int e = q - N * N;
+  if (e >= 4 * N) return;
+  CPt a, b;
+  if (e < 2 * N) {
+    a = CPt{ta, p.A.k_idx, N, p.A.orbital};
+    b = CPt{tb, e & 1, e >> 1, 0};
+  } else {
+    const int r = e - 2 * N;
+    a = CPt{ta, r & 1, r >> 1, 0};
+    b = CPt{tb, p.B.k_idx, N, p.B.orbital};
+  }
+  // g0_eval(p, a, b, internal = true)
+  const int ord = compare_3way(a, b);
+  const double2 *g = (ord > 0 ? p.greater : p.lesser) + (size_t)(a.orb * p.n_orb + b.orb) * p.n_pts;
+  w.pos = table_pos(p, ord == 0 ? 0.0 : a.time - b.time);
+  w.l0 = g[w.pos.i];
+  w.l1 = g[w.pos.i + 1];
+  w.o = ord == 0 && a.orb == b.orb;
+}
+
+template <int N>
+__device__ __forceinline__ double wick_store(const KdParams &p, double2 *__restrict__ W, int S, int q,
+                                             const WickPre &w) {
+  if (q < N * N) {
+    const int i = q / N, j = q - i * N;
+    const double2 gl = table_mix(w.l0, w.l1, w.pos);
+    const double2 gg = table_mix(w.g0, w.g1, w.pos);
+    double2 g00 = w.o > 0 ? gg : gl;
+    double2 g11 = w.o < 0 ? gg : gl;
+    if (w.o == 0) {
       g00.y -= p.alpha;
       g11.y -= p.alpha;
     }
-    double2 *w = W + 2 * i * S + 2 * j;
-    w[0] = g00;
-    w[1] = gl;
-    w[S] = gg;
-    w[S + 1] = g11;
+    double2 *d = W + 2 * i * S + 2 * j;
+    d[0] = g00;
+    d[1] = gl;
+    d[S] = gg;
+    d[S + 1] = g11;
     return fmax(fmax(fabs(g00.x) + fabs(g00.y), fabs(gl.x) + fabs(gl.y)),
                 fmax(fabs(gg.x) + fabs(gg.y), fabs(g11.x) + fabs(g11.y)));
   }
   const int e = q - N * N;
-  if (e < 2 * N) {
-    const CPt a{p.A.time, p.A.k_idx, N, p.A.orbital};
-    const CPt b{Us[e >> 1], e & 1, e >> 1, 0};
-    W[2 * N * S + e] = g0_eval(p, a, b, true);
-  } else if (e < 4 * N) {
-    const int r = e - 2 * N;
-    const CPt a{Us[r >> 1], r & 1, r >> 1, 0};
-    const CPt b{p.B.time, p.B.k_idx, N, p.B.orbital};
-    W[r * S + 2 * N] = g0_eval(p, a, b, true);
+  if (e < 4 * N) {
+    double2 v = table_mix(w.l0, w.l1, w.pos);
+    if (w.o) v.y -= p.alpha;
+    if (e < 2 * N)
+      W[2 * N * S + e] = v;
+    else
+      W[(e - 2 * N) * S + 2 * N] = v;
   } else if constexpr (N > 0) {
     W[2 * N * S + 2 * N] = make_double2(0.0, 0.0);
   } else { // order 0: the value is g0(A, B) itself
     W[0] = g0_eval(p, CPt{p.A.time, p.A.k_idx, 0, p.A.orbital}, CPt{p.B.time, p.B.k_idx, 0, p.B.orbital}, false);
   }
   return 0.0;
+}
+
+template <int N>
+__device__ __forceinline__ double wick_item(const KdParams &p, const double *__restrict__ Us, double2 *__restrict__ W,
+                                            int S, int q) {
+  int ia, ib;
+  wick_item_vertices<N>(q, ia, ib);
+  WickPre w;
+  wick_fetch<N>(p, q, ia >= 0 ? Us[ia] : p.A.time, ib >= 0 ? Us[ib] : p.B.time, w);
+  return wick_store<N>(p, W, S, q, w);
 }
 // items of the build above
 template <int N> constexpr int wick_items() { return N * N + 4 * N + 1; }

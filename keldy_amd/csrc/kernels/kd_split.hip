@@ -321,10 +321,11 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
       b -= nbatch;
       grp++;
     }
+    const int64_t slot_n = b * 64 + grp;
     if (grp < 64) {
       cnt_n = buf.cnt[b];
-      if (t < N) u_n = buf.u[(b * 64 + grp) * N + t];
-      if (t == 0) jac_n = buf.jac[b * 64 + grp];
+      if (t < N) u_n = buf.u[slot_n * N + t];
+      if (t == 0) jac_n = buf.jac[slot_n];
     }
     if (g >= cnt) continue;
     // Opaque per-point copy of the thread id: keeps the thread-only index math

@@ -18,13 +18,13 @@ products) appended to the plain chain (applied first to the Sobol point),
 trained on the device for 2000 Adam steps of 2^14 points on 2^20 samples of
 the plain warped integrand (scramble seed 1000003, disjoint from the
 replicas), toward the density |f|^alpha for alpha in 1, 1/2, 1/4.  The plain
-estimator and the candidates are scored on 32 pilot replicas of 2^16 points
+estimator and the candidates are scored on 128 pilot replicas of 2^16 points
 (seeds 20000+) by spread^2 x seconds per point; the best flow is measured, and
 it is the selected estimator only if its score is below 0.8 x the plain one's
 (plain is the fallback).
 
 For each estimator: time to 1e-3 = (points needed for 1e-3 relative error) /
-(points per second).  The points needed come from a doubling search: 64
+(points per second).  The points needed come from a doubling search: 256
 (--target-replicas) scrambled replicas at M points, M = 2^12, 2^13, ... until the relative spread
 of one replica's estimate is <= 1e-3, then log-log interpolation between the
 last two M.  Points per second come from the timed steps (`--points` per step,
@@ -52,7 +52,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 TARGET_REL = 1e-3
 PILOT_M = 1 << 16  # points per pilot replica when judging flow candidates
-PILOT_REPLICAS = 32
+PILOT_REPLICAS = 128
 SELECT_MARGIN = 0.8  # a flow is selected when its pilot score is below this fraction of the plain one
 # committed rocprofv3 --pmc summaries holding "pmc-traffic <kernel> ..." lines
 PMC_SUMMARIES = [os.path.join(ROOT, "profiles", f) for f in
@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--order", type=int, default=8)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--pilot-points", type=int, default=1 << 14)
-    ap.add_argument("--target-replicas", type=int, default=64,
+    ap.add_argument("--target-replicas", type=int, default=256,
                     help="RQMC replicas per size in the doubling search for the points needed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")

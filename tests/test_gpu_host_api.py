@@ -199,3 +199,32 @@ def test_pilot_histograms_on_device_match_host_binning(model30):
     ref.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
     ref.run(100)
     assert c.reduce_result() == ref.reduce_result()
+
+
+def test_default_algorithm_per_order(model30):
+    """A new computer uses the split algorithm at orders 8-10 and one LU per
+    configuration below; an explicit choice holds until 'default' restores the
+    per-order rule; unknown names raise; both give the same run on the same
+    points (1e-12 relative)."""
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+    t = 10.0
+    for order, want in ((5, "config"), (7, "config"), (8, "split"), (10, "split")):
+        assert kio.ComputeChargeQDirect(model30, t, order, 0.0).get_algorithm() == want, order
+    c = kio.ComputeChargeQDirect(model30, t, 8, 0.0)
+    c.set_algorithm("config")
+    assert c.get_algorithm() == "config"
+    c.set_algorithm("default")
+    assert c.get_algorithm() == "split"
+    with pytest.raises(RuntimeError, match="algorithm must be"):
+        c.set_algorithm("gray")
+    res = []
+    for algo in ("default", "config"):
+        c = kio.ComputeChargeQDirect(model30, t, 8, 0.0)
+        c.set_algorithm(algo)
+        c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+        c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+        c.reset_rng("sobol", 0, True, True, 3)
+        c.run(1 << 14)
+        res.append(c.reduce_result())
+    assert abs(res[0] - res[1]) <= 1e-12 * abs(res[1])

@@ -51,7 +51,7 @@ extern "C" {
 #define KD_PREC_FP32_LU_ALL 2
 
 /* Keldysh-sum algorithm.  KD_ALGO_DEFAULT (a new ctx's setting): KD_ALGO_SPLIT
- * at orders 8..10, KD_ALGO_CONFIG otherwise -- the faster of the two on MI355X
+ * at orders 7..10, KD_ALGO_CONFIG otherwise -- the faster of the two on MI355X
  * for full-size launches; kd_autotune measures every candidate instead.
  * KD_ALGO_CONFIG: one bordered LU per
  * configuration sigma (2^n of them, sharing the staged Wick matrix).

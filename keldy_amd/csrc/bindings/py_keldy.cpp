@@ -165,7 +165,7 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               throw keldy_error("algorithm must be 'default', 'config', 'pair', 'split' or 'tree'");
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
-          py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (split at orders 8..10, config otherwise), 'config', 'pair' "
+          py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (split at orders 7..10, config otherwise), 'config', 'pair' "
           "(orders >= 7), 'split' (orders 6..10) or 'tree' (orders >= 6)")
       .def("get_algorithm",
            [](T &s) {

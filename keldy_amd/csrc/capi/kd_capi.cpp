@@ -78,7 +78,7 @@ struct kd_ctx {
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
   // point and waves per CU of the persistent grid
   int lu_lanes_max = 64, lu_waves_per_cu = 16;
-  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: split at orders 8..10)
+  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: split at orders 7..10)
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
   // time binning (kd_set_time_bins)
@@ -125,7 +125,7 @@ bool use_pair(const kd_ctx *c) {
 
 // the split tree + per-configuration LU (orders 6..10)
 bool use_split(const kd_ctx *c) {
-  const bool chosen = c->algo == KD_ALGO_SPLIT || (c->algo == KD_ALGO_DEFAULT && c->order >= 8);
+  const bool chosen = c->algo == KD_ALGO_SPLIT || (c->algo == KD_ALGO_DEFAULT && c->order >= 7);
   return chosen && c->obs == KD_OBS_DIRECT && kd::split_threads(c->order) > 0;
 }
 

@@ -16,13 +16,14 @@
 // product of the K path pivots and its own n-K pivots, and the bordered Schur
 // complement is the corner left at the end.
 //
-// One point per workgroup of TPB = min(2^n, 128) threads: the workgroup builds
-// the Wick matrix in LDS, computes levels 1..K breadth first (one child row
-// per thread; the levels alternate between two LDS regions), then each thread
-// runs the bordered LUs of its 2^n / TPB configurations from its node's table
-// in registers.  Order 8 (K = 3): 2,094 complex multiply-adds in the tree and
-// 256 x 55 in the configurations' LUs, 16.2k in all against 52.2k for a full
-// bordered LU per configuration.
+// One point per workgroup of TPB threads (64 up to order 7, 128 above): the
+// workgroup builds the Wick matrix in LDS, computes levels 1..K breadth first
+// (one (child, row, column chunk) per thread; the levels alternate between two
+// LDS regions), then each thread runs the bordered LUs of its 2^n / TPB
+// configurations from its node's table in registers -- at orders 7 and 8 two
+// of them, paired on the last vertex's Keldysh index.  Order 8 (K = 3): 2,094
+// complex multiply-adds in the tree and 128 x 80 in the paired LUs, 12.3k in
+// all against 52.2k for a full bordered LU per configuration.
 //
 // Pivot check as in the per-configuration kernel: every pivot on a
 // configuration's path (the tree's and its own) against PivotTol times
@@ -48,7 +49,8 @@ namespace kd {
 template <int N> struct SplitShape {
   static constexpr int K = N <= 7 ? 2 : 3;                // tree levels
   static constexpr int M = N - K;                         // vertices per configuration LU
-  static constexpr int TPB = (1 << N) < KD_SPLIT_THREADS ? (1 << N) : KD_SPLIT_THREADS;
+  // threads per workgroup: 2 configurations per thread from order 7 on (paired)
+  static constexpr int TPB = N <= 7 ? 64 : (1 << N) < KD_SPLIT_THREADS ? (1 << N) : KD_SPLIT_THREADS;
   static constexpr int NW = TPB / 64;                     // waves per workgroup
   static constexpr int CPT = (1 << N) / TPB;              // configurations per thread
   // paired LUs (split_config_pair) at 2 or more configurations per thread; with

@@ -202,14 +202,14 @@ def test_pilot_histograms_on_device_match_host_binning(model30):
 
 
 def test_default_algorithm_per_order(model30):
-    """A new computer uses the split algorithm at orders 8-10 and one LU per
+    """A new computer uses the split algorithm at orders 7-10 and one LU per
     configuration below; an explicit choice holds until 'default' restores the
     per-order rule; unknown names raise; both give the same run on the same
     points (1e-12 relative)."""
     from keldy_amd import impurity_oneband as kio
     from keldy_amd import warpers as kw
     t = 10.0
-    for order, want in ((5, "config"), (7, "config"), (8, "split"), (10, "split")):
+    for order, want in ((5, "config"), (6, "config"), (7, "split"), (8, "split"), (10, "split")):
         assert kio.ComputeChargeQDirect(model30, t, order, 0.0).get_algorithm() == want, order
     c = kio.ComputeChargeQDirect(model30, t, 8, 0.0)
     c.set_algorithm("config")

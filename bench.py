@@ -332,7 +332,7 @@ def main():
                 "in_domain_fraction": m["in_domain_fraction"]}
 
     plain = measure("plain")
-    # The north star's precision split (LU in FP32, the Keldysh sum and the
+    # The mixed-precision variant (LU in FP32, the Keldysh sum and the
     # accumulation in FP64) on the same chain and the same scrambles as the
     # plain FP64 replicas: throughput, and each replica's deviation from its
     # FP64 estimate.  The points needed are the FP64 search's (the two

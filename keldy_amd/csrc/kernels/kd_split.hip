@@ -305,7 +305,7 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
   // item.  The next item's batch count, vertex times and Jacobian are loaded
   // while the current one is computed (every slot < nbatch * 64 lies inside
   // the compacted buffer; an empty slot's stale record is never used).
-  int grp = (int)(blockIdx.x / nbatch);
+  int grp = nbatch > 0 ? (int)(blockIdx.x / nbatch) : 64; // no points: no items (zero partials)
   int64_t b = blockIdx.x - (int64_t)grp * nbatch;
   int cnt_n = 0;
   double u_n = 0.0, jac_n = 0.0;

@@ -351,10 +351,17 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
   // the compacted points sit at the front of each batch, so the items that
   // hold work come first and are dealt evenly over the workgroups.
   constexpr int GPB = 64 / PPW; // groups per batch
-  const int64_t nitems = nbatch * GPB;
-  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const int64_t grp = it / nbatch;
-    const int64_t b = it - grp * nbatch;
+  // item = grp * nbatch + b, stepped by the grid size without a division per item
+  int grp_i = nbatch > 0 ? (int)(blockIdx.x / nbatch) : GPB; // no points: no items
+  int64_t b_i = blockIdx.x - (int64_t)grp_i * nbatch;
+  while (grp_i < GPB) {
+    const int64_t grp = grp_i;
+    const int64_t b = b_i;
+    b_i += gridDim.x;
+    while (b_i >= nbatch) {
+      b_i -= nbatch;
+      grp_i++;
+    }
     const int g0i = (int)grp * PPW;
     const int cnt = buf.cnt[b];
     if (g0i >= cnt) continue;

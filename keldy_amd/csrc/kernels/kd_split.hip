@@ -301,35 +301,48 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
   const int64_t nbatch = (p.n_points + 63) / 64;
   double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
   // One compacted point per item, numbered slot-major (see integrand_kernel):
-  // item = grp * nbatch + b, stepped by the grid size without a division per
-  // item.  The next item's batch count, vertex times and Jacobian are loaded
-  // while the current one is computed (every slot < nbatch * 64 lies inside
-  // the compacted buffer; an empty slot's stale record is never used).
-  int grp = nbatch > 0 ? (int)(blockIdx.x / nbatch) : 64; // no points: no items (zero partials)
-  int64_t b = blockIdx.x - (int64_t)grp * nbatch;
-  int cnt_n = 0;
+  // item = grp * nbatch + b; a workgroup takes every gridDim.x-th item.  Each
+  // wave scans a window of the workgroup's next 64 items at once -- lane l
+  // loads the batch count of item l of the window -- and a ballot leaves the
+  // live ones (slot below its batch's count), which are then processed in
+  // order: a sparse domain costs one vector load per 64 items, not a dependent
+  // load per item.  Both waves compute the same window.  The next live item's
+  // vertex times and Jacobian are loaded while the current one is computed.
+  const int lane = t & 63;
+  const int64_t G = gridDim.x;
+  int wgrp = nbatch > 0 ? (int)(blockIdx.x / nbatch) : 64; // no points: no items (zero partials)
+  int64_t wb = blockIdx.x - (int64_t)wgrp * nbatch;        // the window's first item
+  uint64_t mask = 0;                                        // live items of the window not yet done
+  int gl = 0, bl = 0;                                       // this lane's item of the window
   double u_n = 0.0, jac_n = 0.0;
-  if (grp < 64) {
-    cnt_n = buf.cnt[b];
-    if (t < N) u_n = buf.u[(b * 64 + grp) * N + t];
-    if (t == 0) jac_n = buf.jac[b * 64 + grp];
-  }
-  while (grp < 64) {
-    const int g = grp, cnt = cnt_n;
-    const int64_t slot = b * 64 + grp;
+  for (;;) {
+    if (!mask) {
+      if (wgrp >= 64) break;
+      const int64_t bw = wb + (int64_t)lane * G;
+      gl = wgrp + (int)(bw / nbatch);
+      bl = (int)(bw % nbatch);
+      mask = __ballot(gl < 64 && gl < buf.cnt[bl]);
+      const int64_t adv = wb + 64 * G;
+      wgrp += (int)(adv / nbatch);
+      wb = adv % nbatch;
+      if (!mask) continue;
+      const int j1 = __builtin_ctzll(mask);
+      const int64_t s1 = (int64_t)__shfl(bl, j1, 64) * 64 + __shfl(gl, j1, 64);
+      if (t < N) u_n = buf.u[s1 * N + t];
+      if (t == 0) jac_n = buf.jac[s1];
+    }
+    const int j = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    const int g = __shfl(gl, j, 64);
+    const int64_t b = __shfl(bl, j, 64);
+    const int64_t slot = b * 64 + g;
     const double uc = u_n, jj = jac_n;
-    b += gridDim.x;
-    while (b >= nbatch) {
-      b -= nbatch;
-      grp++;
+    if (mask) {
+      const int j2 = __builtin_ctzll(mask);
+      const int64_t s2 = (int64_t)__shfl(bl, j2, 64) * 64 + __shfl(gl, j2, 64);
+      if (t < N) u_n = buf.u[s2 * N + t];
+      if (t == 0) jac_n = buf.jac[s2];
     }
-    const int64_t slot_n = b * 64 + grp;
-    if (grp < 64) {
-      cnt_n = buf.cnt[b];
-      if (t < N) u_n = buf.u[slot_n * N + t];
-      if (t == 0) jac_n = buf.jac[slot_n];
-    }
-    if (g >= cnt) continue;
     // Opaque per-point copy of the thread id: keeps the thread-only index math
     // of the matrix build, the tree levels and the configurations' LDS
     // addresses from being hoisted out of the point loop into registers.

@@ -71,8 +71,10 @@ extern "C" {
 /* KD_ALGO_SPLIT (orders 6..10; otherwise KD_ALGO_CONFIG): the tree of Schur
  * complements over the leading 2 (orders 6, 7) or 3 vertices, shared by the
  * configurations below each node, then one bordered LU per configuration over
- * the remaining vertices; one point per workgroup.  Order 8: ~3x fewer
- * multiply-adds than KD_ALGO_CONFIG, same arithmetic up to reassociation. */
+ * the remaining vertices (at orders 7 and 8 two configurations per thread,
+ * sharing the elimination of all but the last vertex); one point per
+ * workgroup.  Order 8: ~3.9x fewer multiply-adds than KD_ALGO_CONFIG, same
+ * arithmetic up to reassociation (FP64 results agree to ~1e-13). */
 #define KD_ALGO_SPLIT 3
 
 typedef struct kd_ctx kd_ctx;

@@ -394,46 +394,56 @@ __global__ __launch_bounds__(64) void integrand_kernel(KdParams p, PrepBuf buf) 
           sre = v.x;
           sim = v.y;
         } else {
-          uint64_t badmask = 0;
-          if constexpr (PR && CPL >= 2) {
+          // Flags of the lane's configurations, one 64-bit word per 64 of
+          // them (CPL reaches 256 at order 12 with 16 lanes per point); each
+          // word's flagged configurations are recomputed before the next word.
+          constexpr int CPW = CPL < 64 ? CPL : 64; // configurations per word
 #pragma unroll 1
-            for (int mp = 0; mp < CPL / 2; mp++) {
-              const int cfg0 = (lane % G) + G * mp; // last vertex at index 0; + 2^(N-1) for index 1
-              double2 v[2];
-              bool bad[2];
-              config_value_pair<N, T>(PPW == 1 ? W[0] : W[q], cfg0,
-                                      (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), v, bad);
-              const double sg = (__popc(cfg0) & 1) ? -1.0 : 1.0;
-              sre += bad[0] ? 0.0 : sg * v[0].x;
-              sim += bad[0] ? 0.0 : sg * v[0].y;
-              sre += bad[1] ? 0.0 : -sg * v[1].x;
-              sim += bad[1] ? 0.0 : -sg * v[1].y;
-              badmask |= ((uint64_t)bad[0] << mp) | ((uint64_t)bad[1] << (mp + CPL / 2));
+          for (int w0 = 0; w0 < CPL; w0 += CPW) {
+            uint64_t badmask = 0;
+            if constexpr (PR && CPL >= 2) {
+              // bit 2 (mp - w0 / 2) + t: configuration mp + t CPL / 2, mp in [w0 / 2, (w0 + CPW) / 2)
+#pragma unroll 1
+              for (int mp = w0 / 2; mp < (w0 + CPW) / 2; mp++) {
+                const int cfg0 = (lane % G) + G * mp; // last vertex at index 0; + 2^(N-1) for index 1
+                double2 v[2];
+                bool bad[2];
+                config_value_pair<N, T>(PPW == 1 ? W[0] : W[q], cfg0,
+                                        (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), v, bad);
+                const double sg = (__popc(cfg0) & 1) ? -1.0 : 1.0;
+                sre += bad[0] ? 0.0 : sg * v[0].x;
+                sim += bad[0] ? 0.0 : sg * v[0].y;
+                sre += bad[1] ? 0.0 : -sg * v[1].x;
+                sim += bad[1] ? 0.0 : -sg * v[1].y;
+                const int bit = 2 * (mp - w0 / 2);
+                badmask |= ((uint64_t)bad[0] << bit) | ((uint64_t)bad[1] << (bit + 1));
+              }
+            } else {
+#pragma unroll 1
+              for (int m = w0; m < w0 + CPW; m++) {
+                const int cfg = (lane % G) + G * m;
+                bool bad;
+                const double2 v = config_value<N, T>(PPW == 1 ? W[0] : W[q], cfg,
+                                                     (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), bad);
+                // branch-free: a divergent branch here doubles the register footprint
+                const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
+                sre += bad ? 0.0 : sg * v.x;
+                sim += bad ? 0.0 : sg * v.y;
+                badmask |= (uint64_t)bad << (m - w0);
+              }
             }
-          } else {
-#pragma unroll 1
-            for (int m = 0; m < CPL; m++) {
+            // Flagged configurations: partial-pivoting FP64 LU, outside the hot loop.
+            while (badmask) {
+              const int bit = __builtin_ctzll(badmask);
+              badmask &= badmask - 1;
+              const int m = (PR && CPL >= 2) ? w0 / 2 + (bit >> 1) + (bit & 1) * (CPL / 2) : w0 + bit;
               const int cfg = (lane % G) + G * m;
-              bool bad;
-              const double2 v = config_value<N, T>(PPW == 1 ? W[0] : W[q], cfg,
-                                                   (T)__longlong_as_double((long long)Ssc[PPW == 1 ? 0 : q]), bad);
-              // branch-free: a divergent branch here doubles the register footprint
+              const double2 v = config_value_pivoted(W[q], cfg, N, DS);
               const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
-              sre += bad ? 0.0 : sg * v.x;
-              sim += bad ? 0.0 : sg * v.y;
-              badmask |= (uint64_t)bad << m;
+              sre += sg * v.x;
+              sim += sg * v.y;
+              nfb++;
             }
-          }
-          // Flagged configurations: partial-pivoting FP64 LU, outside the hot loop.
-          while (badmask) {
-            const int m = __builtin_ctzll(badmask);
-            badmask &= badmask - 1;
-            const int cfg = (lane % G) + G * m;
-            const double2 v = config_value_pivoted(W[q], cfg, N, DS);
-            const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
-            sre += sg * v.x;
-            sim += sg * v.y;
-            nfb++;
           }
         }
       }

@@ -245,14 +245,29 @@ int ko_compare_3way(const ko_gf_index *a, const ko_gf_index *b) {
   return (1 - 2 * a->k_idx) * (a->timesplit - b->timesplit);
 }
 
+/* KO_PLANT selects a deliberate mistake (0: none, the restatement itself).
+ * Only tests/test_oracle_pins.py builds the nonzero variants, to show that the
+ * checks which pin the oracle fail on a plausible error:
+ *   1: linear interpolation off by one mesh point      2: its weights swapped
+ *   3: Steffen limiter without the factor 1/2         4: Steffen's p_i with h swapped */
+#ifndef KO_PLANT
+#define KO_PLANT 0
+#endif
+
+/* TRIQS gf<retime> evaluation off the mesh (call sites model.hpp:253-263):
+ * linear interpolation between the two neighbouring mesh points, the index
+ * clamped to the table. */
 static void table_eval(const ko_g0 *g, const double *tab, int oa, int ob, double t, double out[2]) {
   const double *p = tab + ((size_t)(oa * g->n_orb + ob) * g->n_pts) * 2;
   double a = (t - g->t_min) / g->dt;
   double fl = floor(a);
-  long i = (long)fl;
+  long i = (long)fl + (KO_PLANT == 1);
   if (i > g->n_pts - 2) i = g->n_pts - 2;
   if (i < 0) i = 0;
   double w = a - (double)i;
+#if KO_PLANT == 2
+  w = 1.0 - w;
+#endif
   out[0] = (1.0 - w) * p[2 * i] + w * p[2 * i + 2];
   out[1] = (1.0 - w) * p[2 * i + 1] + w * p[2 * i + 3];
 }
@@ -313,10 +328,18 @@ void ko_steffen_coeffs(const double *x, const double *y, int n, double *co) {
   for (int i = 1; i < n - 1; i++) {
     double hi = x[i + 1] - x[i], him1 = x[i] - x[i - 1];
     double si = (y[i + 1] - y[i]) / hi, sim1 = (y[i] - y[i - 1]) / him1;
+#if KO_PLANT == 4
+    double pi = (sim1 * him1 + si * hi) / (him1 + hi);
+#else
     double pi = (sim1 * hi + si * him1) / (him1 + hi);
+#endif
     double mn = fabs(sim1);
     if (fabs(si) < mn) mn = fabs(si);
+#if KO_PLANT == 3
+    if (fabs(pi) < mn) mn = fabs(pi);
+#else
     if (0.5 * fabs(pi) < mn) mn = 0.5 * fabs(pi);
+#endif
     yp[i] = (steffen_sign(1.0, sim1) + steffen_sign(1.0, si)) * mn;
   }
   yp[n - 1] = (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]);

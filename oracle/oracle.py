@@ -70,43 +70,49 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        dp = C.POINTER(C.c_double)
-        L.ko_mt64_seed.argtypes = [C.c_void_p, C.c_uint64]
-        L.ko_mt64_next.argtypes = [C.c_void_p]
-        L.ko_mt64_next.restype = C.c_uint64
-        L.ko_jk2008_matrices.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.c_int]
-        L.ko_sobol_init.argtypes = [C.POINTER(_Sobol), C.c_int, C.c_uint32, C.c_int, C.c_int,
-                                    C.c_int64]
-        L.ko_sobol_next.argtypes = [C.POINTER(_Sobol), dp]
-        L.ko_sobol_set_state.argtypes = [C.POINTER(_Sobol), C.c_uint32]
-        L.ko_compare_3way.argtypes = [C.POINTER(_GfIndex), C.POINTER(_GfIndex)]
-        L.ko_g0_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.POINTER(_GfIndex), C.c_int, dp]
-        L.ko_map1d_reverse.argtypes = [C.POINTER(_Map1d), C.c_double, dp]
-        L.ko_map1d_reverse.restype = C.c_double
-        L.ko_warp_reverse.argtypes = [C.POINTER(_Warper), C.c_int, C.c_int, dp, dp]
-        L.ko_warp_reverse.restype = C.c_double
-        L.ko_steffen_coeffs.argtypes = [dp, dp, C.c_int, dp]
-        L.ko_steffen_eval.argtypes = [dp, dp, C.c_int, C.c_double]
-        L.ko_steffen_eval.restype = C.c_double
-        L.ko_integrand_eval.argtypes = [C.POINTER(_Integrand), C.c_int, dp, C.c_int, dp]
-        L.ko_integrand_eval_no_det.argtypes = [C.POINTER(_Integrand), C.c_int, dp, C.c_int, dp]
-        L.ko_det.argtypes = [dp, C.c_int, dp]
-        L.ko_eval_warped.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int, C.c_int,
-                                     dp, C.c_int, dp]
-        L.ko_run.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int, C.POINTER(_Sobol),
-                             C.c_uint64, C.POINTER(_Accum)]
-        L.ko_run_parallel.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int,
-                                      C.POINTER(_Sobol), C.c_uint64, C.c_int, C.POINTER(_Accum)]
-        L.ko_first_row_expansion.argtypes = [dp, C.c_int, dp, dp]
-        L.ko_kernel_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int, dp]
-        L.ko_kernel_omega_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int,
-                                           C.c_double, dp]
-        u64p = C.POINTER(C.c_uint64)
-        L.ko_kernel_run.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.POINTER(_Warper), C.c_int,
-                                    C.POINTER(_Sobol), C.c_uint64, C.c_double, C.c_int, dp, u64p, u64p, u64p]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path):
+    """The oracle library at `path` with its ctypes signatures (lib() loads the
+    restatement; tests load variants built with a planted mistake)."""
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    L.ko_mt64_seed.argtypes = [C.c_void_p, C.c_uint64]
+    L.ko_mt64_next.argtypes = [C.c_void_p]
+    L.ko_mt64_next.restype = C.c_uint64
+    L.ko_jk2008_matrices.argtypes = [C.POINTER(C.c_uint32), C.c_int, C.c_int]
+    L.ko_sobol_init.argtypes = [C.POINTER(_Sobol), C.c_int, C.c_uint32, C.c_int, C.c_int,
+                                C.c_int64]
+    L.ko_sobol_next.argtypes = [C.POINTER(_Sobol), dp]
+    L.ko_sobol_set_state.argtypes = [C.POINTER(_Sobol), C.c_uint32]
+    L.ko_compare_3way.argtypes = [C.POINTER(_GfIndex), C.POINTER(_GfIndex)]
+    L.ko_g0_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.POINTER(_GfIndex), C.c_int, dp]
+    L.ko_map1d_reverse.argtypes = [C.POINTER(_Map1d), C.c_double, dp]
+    L.ko_map1d_reverse.restype = C.c_double
+    L.ko_warp_reverse.argtypes = [C.POINTER(_Warper), C.c_int, C.c_int, dp, dp]
+    L.ko_warp_reverse.restype = C.c_double
+    L.ko_steffen_coeffs.argtypes = [dp, dp, C.c_int, dp]
+    L.ko_steffen_eval.argtypes = [dp, dp, C.c_int, C.c_double]
+    L.ko_steffen_eval.restype = C.c_double
+    L.ko_integrand_eval.argtypes = [C.POINTER(_Integrand), C.c_int, dp, C.c_int, dp]
+    L.ko_integrand_eval_no_det.argtypes = [C.POINTER(_Integrand), C.c_int, dp, C.c_int, dp]
+    L.ko_det.argtypes = [dp, C.c_int, dp]
+    L.ko_eval_warped.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int, C.c_int,
+                                 dp, C.c_int, dp]
+    L.ko_run.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int, C.POINTER(_Sobol),
+                         C.c_uint64, C.POINTER(_Accum)]
+    L.ko_run_parallel.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int,
+                                  C.POINTER(_Sobol), C.c_uint64, C.c_int, C.POINTER(_Accum)]
+    L.ko_first_row_expansion.argtypes = [dp, C.c_int, dp, dp]
+    L.ko_kernel_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int, dp]
+    L.ko_kernel_omega_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int,
+                                       C.c_double, dp]
+    u64p = C.POINTER(C.c_uint64)
+    L.ko_kernel_run.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.POINTER(_Warper), C.c_int,
+                                C.POINTER(_Sobol), C.c_uint64, C.c_double, C.c_int, dp, u64p, u64p, u64p]
+    return L
 
 
 def _dp(a):

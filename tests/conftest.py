@@ -14,17 +14,26 @@ def pytest_configure(config):
 
 
 def _have_gpu():
+    """True when the library loads and sees a HIP device.  A library that is
+    missing or fails to load is an error on a machine with a GPU driver
+    (/dev/kfd), so a broken build cannot pass as a run of skipped tests."""
     try:
         from keldy_amd import capi
-        return capi.device_count() > 0
-    except Exception:
-        return False
+        capi.lib()
+    except (ImportError, OSError) as e:
+        if os.path.exists("/dev/kfd"):
+            raise
+        return f"library not loadable here ({e})"
+    return capi.device_count() > 0 or "no HIP device"
 
 
 def pytest_collection_modifyitems(config, items):
-    if _have_gpu():
+    if not any("gpu" in it.keywords for it in items):
         return
-    skip = pytest.mark.skip(reason="no HIP device")
+    have = _have_gpu()
+    if have is True:
+        return
+    skip = pytest.mark.skip(reason=have)
     for it in items:
         if "gpu" in it.keywords:
             it.add_marker(skip)

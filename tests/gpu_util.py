@@ -28,12 +28,15 @@ KIND = {"uniform": capi.KD_MAP_UNIFORM, "exponential": capi.KD_MAP_EXPONENTIAL,
 
 
 def chain(spec):
-    """spec: list of ("plasma", t) | ("map", kind, u_max, w, power) (same map on
+    """spec: list of ("identity",) | ("plasma", t) | ("map", kind, u_max, w, power) (same map on
     every axis) in the order warpers are added.  Returns (oracle Train,
     (descs, maps, tables)) for the device."""
     o_ws, descs, maps = [], [], []
     for s in spec:
-        if s[0] == "plasma":
+        if s[0] == "identity":
+            o_ws.append(O.Warper(O.WARP_IDENTITY))
+            descs.append(capi.WarperDesc(capi.KD_WARP_IDENTITY, 0, 0.0, 0))
+        elif s[0] == "plasma":
             o_ws.append(O.plasma(s[1]))
             descs.append(capi.WarperDesc(capi.KD_WARP_PLASMA, 0, s[1], 0))
         else:

@@ -81,6 +81,35 @@ def test_eval_warped_fp64(flat_g0, order):
     assert _normwise(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("order,spec", [
+    (3, [("identity",)]),
+    (4, [("plasma", 10.0), ("identity",), ("map", "exponential", 10.0, 2.0, 1)]),
+    (5, [("plasma", 10.0), ("map", "inverse_power", 10.0, 1.5, 3)]),
+    (6, [("map", "exponential", 10.0, 3.0, 1), ("identity",)]),
+    (8, [("plasma", 10.0), ("map", "inverse_power", 10.0, 0.7, 4)]),
+])
+def test_eval_warped_identity_exponential_inverse_power(flat_g0, order, spec):
+    """The identity warper (identity.hpp:30-49: u = l, J = 1, domain [0, 1]^n)
+    and the exponential and inverse_power makers (product_1d_simple.hpp:560-630)
+    on the device: values, domain flags and Jacobians against the oracle's
+    chain on the same scrambled Sobol points."""
+    t = 10.0
+    train, (descs, maps, tabs) = U.chain(spec)
+    ctx = U.charge_ctx(flat_g0, order, t)
+    ctx.set_warper_chain(descs, maps, tabs)
+    l = O.Sobol(order, 0, True, True, 11).points(64)
+    got, dom, jac = ctx.eval_warped_batch(l)
+    f = O.charge_integrand(U.oracle_charge(flat_g0), t)
+    res = [O.eval_warped(f, train, li) for li in l]
+    ref = np.array([r[0] for r in res])
+    rdom = np.array([r[1] for r in res])
+    rjac = np.array([train.reverse(li)[1] for li in l])
+    assert np.array_equal(dom, rdom)
+    assert np.any(dom == 1)
+    assert _normwise(got, ref) <= TOL
+    np.testing.assert_allclose(jac, rjac, rtol=1e-13, atol=0)
+
+
 def test_run_config1(flat_g0):
     """n=3, 2^16 scrambled Sobol points (seed 0), uniform product map on [0, 10]^3."""
     t, order, npts = 10.0, 3, 1 << 16

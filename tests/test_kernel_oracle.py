@@ -117,7 +117,7 @@ def test_domain_and_single_omega(dot_lead_default):
     for w in (0.0, 1.3):
         ref = sum(e[c, i] * np.exp(1j * w * t[i]) for c in range(8) for i in range(3))
         got, dom = K.omega(t, w)
-        assert dom == 1 and abs(got - ref) <= 1e-15 * abs(ref)
+        assert dom == 1 and abs(got - ref) <= 1e-14 * abs(ref)
 
 
 def _value_kernel_params():

@@ -107,7 +107,8 @@ def test_eval_warped_identity_exponential_inverse_power(flat_g0, order, spec):
     assert np.array_equal(dom, rdom)
     assert np.any(dom == 1)
     assert _normwise(got, ref) <= TOL
-    np.testing.assert_allclose(jac, rjac, rtol=1e-13, atol=0)
+    # a product of up to 8 per-axis factors from pow/log1p: device and libm differ by ulps
+    np.testing.assert_allclose(jac, rjac, rtol=1e-12, atol=0)
 
 
 def test_run_config1(flat_g0):

@@ -102,18 +102,53 @@ def test_eval_batch_fp32_lu_vs_oracle(flat_g0, oracle_ref, order):
 
 
 @pytest.mark.parametrize("order", [8, 9, 10])
+def test_near_coincident_vertex_pairs_vs_oracle(flat_g0, order):
+    """Vertices in pairs 1e-7 or 1e-9 apart: the configurations that give both
+    vertices of a pair the same Keldysh index have two nearly equal rows, an
+    unpivoted pivot of ~delta that fails the pivot check, and go through the
+    partial-pivoting FP64 LU; the sum stays well conditioned (the oracle and a
+    40-digit evaluation of the same matrices agree to ~5e-12 at order 8).
+    Every algorithm against the oracle at 1e-10."""
+    t = 10.0
+    rng = np.random.default_rng(960 + order)
+    rows = []
+    for d in (1e-7, 1e-9):
+        for _ in range(4):
+            taus = rng.uniform(0.5, 9.5, (order + 1) // 2)
+            x = np.empty(2 * len(taus))
+            x[0::2], x[1::2] = taus, taus + d
+            rows.append(rng.permutation(x[:order]))
+    u = np.array(rows)
+    ref, rdom = O.charge_integrand(U.oracle_charge(flat_g0), t).batch(u)
+    for label, algo, lanes in _shapes(order):
+        ctx = U.charge_ctx(flat_g0, order, t)
+        ctx.set_algorithm(algo)
+        ctx.set_launch_shape(lanes, 16)
+        got, dom = ctx.eval_batch(u)
+        assert np.array_equal(dom, rdom), label
+        err = _normwise(got, ref)
+        print(f"order {order} {label}: near-coincident pairs, normwise error vs oracle {err:.2e}")
+        assert err <= TOL, label
+
+
+@pytest.mark.parametrize("order", [8, 9, 10])
 def test_half_filling_alpha_half_all_fallback(flat_g0, order):
     """Keldysh shift alpha = 1/2 at half filling: every diagonal entry
-    g^<(0) - i alpha = 0 (model.hpp:244-265), so no unpivoted LU may be
-    trusted; every configuration must fail the pivot check and take the
-    partial-pivoting FP64 path.  Values against the oracle at 1e-10 with every
-    algorithm, and the run's fallback count is every configuration of every
-    in-domain point."""
+    g^<(0) - i alpha is exactly 0 (model.hpp:244-265), so every configuration
+    must fail the pivot check and take the partial-pivoting FP64 path: a run's
+    fallback count is every configuration of every in-domain point.  The
+    integrand then vanishes to ~1e-20 of its alpha = 0 size, below what any
+    FP64 determinant resolves (the oracle's own values are off by tens of
+    percent against a 40-digit evaluation), so values are compared on the
+    alpha = 0 integrand's scale: device and oracle agree to 1e-10 of it and are
+    finite."""
     t = 10.0
     g0 = dict(flat_g0)
     u = _points(order, 16, t, 950 + order)
+    u = u[np.all(u >= 0, axis=1)]
     tab = O.G0Table(g0["lesser"], g0["greater"], g0["time_max"], alpha=0.5)
     ref, rdom = O.charge_integrand(tab, t).batch(u)
+    scale = np.max(np.abs(O.charge_integrand(U.oracle_charge(flat_g0), t).batch(u)[0]))
     for label, algo, lanes in _shapes(order):
         ctx = capi.Context(0)
         ctx.set_g0_table(g0["lesser"], g0["greater"], g0["time_max"], alpha=0.5)
@@ -122,14 +157,16 @@ def test_half_filling_alpha_half_all_fallback(flat_g0, order):
         ctx.set_launch_shape(lanes, 16)
         got, dom = ctx.eval_batch(u)
         assert np.array_equal(dom, rdom), label
-        err = _normwise(got, ref)
-        assert err <= TOL, f"{label}: {err:.2e}"
-        if label in ("split", "config64"):
-            _, (descs, maps, tabs) = U.chain([("plasma", t), ("map", "uniform", t, 1.0, 1)])
+        assert np.all(np.isfinite(got)), label
+        assert np.max(np.abs(got - ref)) <= TOL * scale, label
+        if label in ("split", "config64", "tree"):
+            # the uniform map on [0, t]^n: every point in the domain
+            _, (descs, maps, tabs) = U.chain([("map", "uniform", t, 1.0, 1)])
             ctx.set_warper_chain(descs, maps, tabs)
             Csr, shfr = O.Sobol(order, 0, True, True, 1).tables()
             ctx.set_sobol(Csr, shfr, 0)
-            acc = ctx.run(1 << 10)
+            acc = ctx.run(256)
+            assert acc.n_in_domain == 256
             print(f"order {order} {label}, alpha=1/2: {acc.n_fallback} fallbacks for "
                   f"{acc.n_in_domain} in-domain points x {1 << order} configurations")
             assert acc.n_fallback == acc.n_in_domain << order

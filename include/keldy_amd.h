@@ -153,6 +153,19 @@ int kd_eval_batch(kd_ctx *ctx, const double *u, int64_t n_points, int32_t keep_u
 /* Same with device pointers (inputs already in HBM); asynchronous on the ctx stream. */
 int kd_eval_batch_device(kd_ctx *ctx, const double *d_u, int64_t n_points, int32_t keep_u_hypercube,
                          double *d_out, int32_t *d_in_domain);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): kd_eval_batch_device with the same arguments and results,
+ * but its compacted-point buffer is a static device array of the library
+ * (KD_NOALLOC_MAX_POINTS points at order KD_MAX_ORDER, about 3.5 MB, held by
+ * each device the library runs on until the process ends) instead of ctx
+ * memory; the call allocates nothing.  n_points above KD_NOALLOC_MAX_POINTS
+ * returns KD_ERR_RANGE.  Every call on one device shares that array: calls
+ * that overlap on a device (other contexts or streams) race.  Asynchronous on
+ * the ctx stream, as kd_eval_batch_device. */
+#define KD_NOALLOC_MAX_POINTS 32768
+int kd_eval_batch_device_noalloc(kd_ctx *ctx, const double *d_u, int64_t n_points, int32_t keep_u_hypercube,
+                                 double *d_out, int32_t *d_in_domain);
 
 /* Warper chain, in the order warpers were added (applied last-to-first).
  * maps: all kd_map1d of PRODUCT warpers; tables: the doubles they index. */

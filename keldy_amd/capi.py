@@ -56,7 +56,8 @@ SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_se
            "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch",
            "kd_set_time_bins", "kd_get_time_bins", "kd_set_algorithm", "kd_get_algorithm",
            "kd_pilot_histograms", "kd_set_kernel_integrand", "kd_set_kernel_omega_integrand",
-           "kd_set_kernel_bins", "kd_get_kernel_bins", "kd_eval_kernel_batch"]
+           "kd_set_kernel_bins", "kd_get_kernel_bins", "kd_eval_kernel_batch",
+           "kd_eval_batch_device_noalloc"]
 
 _lib = None
 
@@ -80,6 +81,7 @@ def lib():
         L.kd_set_precision.argtypes = [vp, C.c_int32]
         L.kd_eval_batch.argtypes = [vp, dp, C.c_int64, C.c_int32, dp, ip]
         L.kd_eval_batch_device.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
+        L.kd_eval_batch_device_noalloc.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
         L.kd_set_warper_chain.argtypes = [vp, C.POINTER(WarperDesc), C.c_int32, C.POINTER(Map1d),
                                           C.c_int32, dp, C.c_int64]
         L.kd_eval_warped_batch.argtypes = [vp, dp, C.c_int64, C.c_int32, dp, ip, dp]

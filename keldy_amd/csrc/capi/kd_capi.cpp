@@ -9,6 +9,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -90,7 +91,35 @@ struct kd_ctx {
   int bin_part_cap = 0;
 };
 
+// Scratch of kd_eval_batch_device_noalloc: the compacted-point buffer
+// (u [n][order], jac [n], idx [n], cnt [n / 64]) for up to
+// KD_NOALLOC_MAX_POINTS points at order KD_MAX_ORDER, held by every device the
+// library runs on until the process ends.  Host code never uses the symbol's
+// name as an address: kd_create resolves the device's own copy once per device
+// (hipGetSymbolAddress) and the entry point passes that address to the kernels.
+constexpr size_t kNoallocScratchBytes =
+    (size_t)KD_NOALLOC_MAX_POINTS * (sizeof(double) * KD_MAX_ORDER + sizeof(double) + sizeof(int)) +
+    (size_t)KD_NOALLOC_MAX_POINTS / 64 * sizeof(int) + 256;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};       // device address of `scratch`, per device
+hipError_t g_scratch_rc[kMaxDevices] = {};   // result of its lookup (hipSuccess before the first)
+bool g_scratch_done[kMaxDevices] = {};
+
+// Resolves `scratch` on `device` (the current device) once per process.
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void *ptr = nullptr;
+  g_scratch_rc[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_ptr[device] = g_scratch_rc[device] == hipSuccess ? ptr : nullptr;
+  g_scratch_done[device] = true;
+}
 
 int fail(kd_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
@@ -261,7 +290,7 @@ int flow_grid(const kd_ctx *c, int64_t n) {
   return (int)std::max<int64_t>(1, std::min(nb, cap));
 }
 
-int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed);
+int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepBuf *ext = nullptr);
 int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) { return launch_chunks(c, p0, true); }
 
 // Allocate every buffer a launch of p0 needs (before a graph capture).
@@ -279,10 +308,15 @@ int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
 // Prep + integrand kernels over p0.n_points points in chunks (and, in
 // MODE_RUN, the deterministic partial reduction into d_acc); with `timed`
 // each kernel is bracketed by HIP events on the ctx stream (kd_last_timing).
-int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed) {
+// `ext`: a compacted-point buffer of the caller's for up to kChunk points (no
+// allocation of the ctx's own); only for evaluation modes without the flow stage.
+int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepBuf *ext) {
   kd::PrepBuf buf;
-  int rc = ensure_prep(c, c->order, p0.n_points, &buf);
-  if (rc) return rc;
+  int rc = KD_OK;
+  if (ext)
+    buf = *ext;
+  else if ((rc = ensure_prep(c, c->order, p0.n_points, &buf)))
+    return rc;
   const int N = c->order;
   // The flow warper, when last in the chain, is applied first: its own stage
   // maps l -> x (and the Jacobian) before the rest of the chain.
@@ -435,6 +469,7 @@ int kd_create(kd_ctx **out, int device) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+  resolve_scratch(device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc(&c->d_acc, 4 * sizeof(double)) != hipSuccess ||
@@ -557,6 +592,49 @@ int kd_eval_batch_device(kd_ctx *c, const double *d_u, int64_t n_points, int32_t
   p.out_vals = d_out;
   p.out_dom = d_in_domain;
   return launch_timed(c, p, grid_for(c, n_points));
+}
+
+int kd_eval_batch_device_noalloc(kd_ctx *c, const double *d_u, int64_t n_points, int32_t keep_u_hypercube,
+                                 double *d_out, int32_t *d_in_domain) {
+  if (!c) return KD_ERR_ARG;
+  int rc = need_integrand(c);
+  if (rc) return rc;
+  if (c->obs == KD_OBS_KERNEL) return fail(c, KD_ERR_STATE, "kernel-method entries: use kd_eval_kernel_batch");
+  if (n_points < 0) return fail(c, KD_ERR_ARG, "negative point count");
+  if (n_points > KD_NOALLOC_MAX_POINTS)
+    return fail(c, KD_ERR_RANGE, "kd_eval_batch_device_noalloc: at most KD_NOALLOC_MAX_POINTS (" +
+                                     std::to_string(KD_NOALLOC_MAX_POINTS) + ") points per call, got " +
+                                     std::to_string(n_points));
+  if (n_points == 0) return KD_OK;
+  if ((!d_u && c->order > 0) || !d_out) return fail(c, KD_ERR_ARG, "null buffer");
+  if (c->device < 0 || c->device >= kMaxDevices || !g_scratch_ptr[c->device])
+    return fail(c, KD_ERR_HIP, std::string("scratch of kd_eval_batch_device_noalloc unavailable: ") +
+                                   hipGetErrorString(c->device >= 0 && c->device < kMaxDevices
+                                                         ? g_scratch_rc[c->device]
+                                                         : hipErrorInvalidDevice));
+  KD_HIP(c, hipSetDevice(c->device));
+  // the compacted-point buffer carved from this device's scratch, laid out as
+  // ensure_prep lays out the ctx's own (every slot read is written first by
+  // the prep kernel of the same call, so nothing needs clearing)
+  const size_t np = ((size_t)n_points + 63) / 64 * 64;
+  const int N = c->order > 0 ? c->order : 1;
+  char *base = (char *)g_scratch_ptr[c->device];
+  kd::PrepBuf buf;
+  buf.u = (double *)base;
+  base += np * sizeof(double) * (size_t)N;
+  buf.jac = (double *)base;
+  base += np * sizeof(double);
+  buf.idx = (int *)base;
+  base += np * sizeof(int);
+  buf.cnt = (int *)base;
+  kd::KdParams p = base_params(c);
+  p.mode = kd::MODE_EVAL_U;
+  p.keep_u = keep_u_hypercube ? 1 : 0;
+  p.in_pts = d_u;
+  p.n_points = n_points;
+  p.out_vals = d_out;
+  p.out_dom = d_in_domain;
+  return launch_chunks(c, p, false, &buf);
 }
 
 int kd_eval_batch(kd_ctx *c, const double *u, int64_t n_points, int32_t keep_u_hypercube, double *out,

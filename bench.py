@@ -56,7 +56,8 @@ PILOT_REPLICAS = 128
 SELECT_MARGIN = 0.8  # a flow is selected when its pilot score is below this fraction of the plain one
 # committed rocprofv3 --pmc summaries holding "pmc-traffic <kernel> ..." lines
 PMC_SUMMARIES = [os.path.join(ROOT, "profiles", f) for f in
-                 ("integrand_split_kernel_n8_fp64_pmc.md", "integrand_kernel_n8_fp64_pmc.md")]
+                 ("integrand_wsplit_kernel_n8_fp64_pmc.md", "integrand_split_kernel_n8_fp64_pmc.md",
+                  "integrand_kernel_n8_fp64_pmc.md")]
 
 
 class _stdout_to_stderr:
@@ -218,14 +219,38 @@ def kernel_method_line(args, model, proj, t_max):
     }))
 
 
+def cpu_threads():
+    """Threads for the CPU baseline and what they come from: the host cores this
+    process may run on (sched_getaffinity), capped by OMP_NUM_THREADS when the
+    environment sets it (the share of a shared GPU host's cores given to one
+    GPU's jobs)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return threads, {"nproc": nproc, "affinity": affinity, "omp_num_threads": omp}
+
+
 def cpu_baseline(model, proj, order, t_max, gpu_points_needed, seconds):
-    """The CPU oracle (a serial-per-point C port of the reference path) on the
-    same warper chain, timed on the host cores."""
+    """The CPU oracle (a C port of the reference path, FP64, partial-pivoting
+    LUs per configuration) on the same warper chain and Sobol points, timed on
+    the host cores three ways: points split over the threads in contiguous Sobol
+    ranges (the analogue of the reference's MPI round robin,
+    integrator.hpp:82-88); the reference's own OpenMP structure, every point in
+    turn with its Wick rows and 2^n configurations spread over the threads
+    (wick_direct.cpp:81,93,110); and one core.  The baseline is the faster of
+    the two parallel structures.  Each is timed on a bounded sample sized to
+    about its share of `seconds` from a calibration run."""
     import numpy as np
 
     from oracle import oracle as O
 
-    cores = min(os.cpu_count() or 1, 16)
+    threads, where = cpu_threads()
     g0 = O.G0Table(model.g0_lesser[0, 0], model.g0_greater[0, 0], model.time_max)
     f = O.charge_integrand(g0, t_max)
     maps = []
@@ -233,19 +258,103 @@ def cpu_baseline(model, proj, order, t_max, gpu_points_needed, seconds):
         t, f1, F, cut = proj.get_1d_warper(a).knots()
         maps.append(O.Map1d("nearest", 1.0, t=np.array(t), f=np.array(f1), F=np.array(F), n_inv=cut + 1))
     train = O.Train([O.plasma(t_max), O.product(O.Map1d("inverse", t_max, w=1.0)), O.product(maps)])
-    calib = 4096
-    s = O.Sobol(order, 0, True, True, 0)
-    t0 = time.perf_counter()
-    O.run(f, train, s, calib, threads=cores)
-    dt = time.perf_counter() - t0
-    n = int(max(calib, min(1 << 22, calib * seconds / max(dt, 1e-6))))
-    s = O.Sobol(order, 0, True, True, 0)
-    t0 = time.perf_counter()
-    val, npts, ndom = O.run(f, train, s, n, threads=cores)
-    dt = time.perf_counter() - t0
-    pps = n / dt
-    return {"pps": pps, "cores": cores, "sample_points": n, "seconds": dt,
-            "value": gpu_points_needed / pps, "sum": val, "n_in_domain": ndom}
+
+    def timed(n, **kw):
+        s = O.Sobol(order, 0, True, True, 0)
+        t0 = time.perf_counter()
+        val, _, ndom = O.run(f, train, s, n, **kw)
+        return time.perf_counter() - t0, val, ndom
+
+    out = {}
+    for name, kw, share in (("points_split", dict(threads=threads), 0.55),
+                            ("intrapoint_openmp", dict(threads=threads, intrapoint=True), 0.3),
+                            ("single_core", dict(threads=1), 0.15)):
+        calib = 1024
+        dt, _, _ = timed(calib, **kw)
+        n = int(max(calib, min(1 << 22, calib * seconds * share / max(dt, 1e-6))))
+        dt, val, ndom = timed(n, **kw)
+        out[name] = {"points_per_s": n / dt, "sample_points": n, "seconds": dt, "threads": kw["threads"],
+                     "sum": val, "n_in_domain": ndom}
+    best = max(("points_split", "intrapoint_openmp"), key=lambda k: out[k]["points_per_s"])
+    pps = out[best]["points_per_s"]
+    return {"pps": pps, "cores": threads, "structure": best, "where": where, "structures": out,
+            "value": gpu_points_needed / pps, "sample_points": out[best]["sample_points"],
+            "seconds": out[best]["seconds"]}
+
+
+def workload_rates():
+    """Keldysh-sum throughput of the other two device workloads, per order or
+    time, counting only the points inside the integration domain (the rest cost
+    the sampling stage only):
+    - orders 1..10, floor(2^24 / n^2) points each, [plasma_uv(10), inverse(10, 1)],
+      FP32 LU (FP64 from order 10), each order's launch shape autotuned; every
+      order run on its own (in-domain points / Keldysh-sum kernel time), then
+      all orders as one hipGraph launch;
+    - the time-resolved current at order 10: 128 external times in (0, 20],
+      2^20 points each, dot-lead semicircle g0 by contour integration,
+      [plasma_uv(t), inverse(t, 1)]; summed over the times, then all times as
+      one hipGraph launch."""
+    import numpy as np
+
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+
+    def one(c, m):
+        d0, f0 = c.reduce_nr_points_in_domain(), c.reduce_nr_fallback()
+        c.run(m)
+        lu_ms = c.last_timing()[1]
+        return c.reduce_nr_points_in_domain() - d0, c.reduce_nr_fallback() - f0, lu_ms
+
+    t = 10.0
+    with _stdout_to_stderr():
+        model = kio.G0Model(kio.G0ModelOmega(dict(model_params())), False)
+    orders = list(range(1, 11))
+    counts = [(1 << 24) // (n * n) for n in orders]
+    comps, per_order = [], []
+    for n, m in zip(orders, counts):
+        c = kio.ComputeChargeQDirect(model, t, n, 0.0)
+        c.set_precision("fp32")
+        c.warper.emplace_back(kw.WarperPlasmaUvT(t))
+        c.warper.emplace_back(kw.make_product_1d_simple_inverse(t, 1.0))
+        c.reset_rng("sobol", 0, True, True, 11)
+        lanes, wpc, _ = c.autotune(1 << 16)
+        one(c, m)  # warm-up
+        dom, fb, lu_ms = one(c, m)
+        fpp = c.flops_per_point()
+        per_order.append({"order": n, "points": m, "algorithm": c.get_algorithm(), "lanes": int(lanes),
+                          "waves_per_cu": int(wpc), "in_domain": int(dom), "lu_ms": lu_ms,
+                          "in_domain_points_per_s": dom / (lu_ms * 1e-3), "fallback_configurations": int(fb),
+                          "tflops": dom * fpp / (lu_ms * 1e-3) / 1e12})
+        c.reset_rng("sobol", 0, True, True, 11)
+        comps.append(c)
+    kio.run_batch(comps, counts)
+    graph_ms = kio.run_batch(comps, counts)
+
+    p = dict(beta=-1.0, bias_V=0.6, eps_d=-0.5, Gamma=1.0, time_max=20.0, nr_time_points_gf=2001, alpha=0.0,
+             half_bandwidth=2.0, bath_type="semicircle", ft_method="contour")
+    with _stdout_to_stderr():
+        dot = kio.G0Model(kio.G0ModelOmega(p), True)
+    times = np.linspace(20.0 / 128, 20.0, 128)
+    cur, m = [], 1 << 20
+    dom_t, fb_t, lu_t = 0, 0, 0.0
+    for tt in times:
+        c = kio.ComputeCurrentJDirect(dot, float(tt), 10, 0.0)
+        c.warper.emplace_back(kw.WarperPlasmaUvT(float(tt)))
+        c.warper.emplace_back(kw.make_product_1d_simple_inverse(float(tt), 1.0))
+        c.reset_rng("sobol", 0, True, True, 5)
+        dom, fb, lu_ms = one(c, m)
+        dom_t, fb_t, lu_t = dom_t + dom, fb_t + fb, lu_t + lu_ms
+        c.reset_rng("sobol", 0, True, True, 5)
+        cur.append(c)
+    kio.run_batch(cur, [m] * len(cur))
+    cur_ms = kio.run_batch(cur, [m] * len(cur))
+    return {
+        "orders_1_to_10": {"per_order": per_order, "one_graph_launch_ms": graph_ms},
+        "current_order10_128_times": {"points": 128 * m, "in_domain": int(dom_t), "lu_ms_sum": lu_t,
+                                      "in_domain_points_per_s": dom_t / (lu_t * 1e-3),
+                                      "fallback_configurations": int(fb_t), "algorithm": cur[0].get_algorithm(),
+                                      "one_graph_launch_ms": cur_ms},
+    }
 
 
 def main():
@@ -262,6 +371,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flow", action="store_true", help="plain estimator only")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the per-order rates of the orders 1..10 and time-resolved current workloads")
     ap.add_argument("--no-fp32", action="store_true",
                     help="skip the FP32-LU side measurement of the plain estimator")
     ap.add_argument("--flow-steps", type=int, default=2000)
@@ -270,7 +381,7 @@ def main():
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-alphas", type=float, nargs="+", default=[1.0, 0.5, 0.25],
                     help="weight powers of the flow candidates (density ~ |f|^alpha)")
-    ap.add_argument("--algorithm", default=None, choices=["config", "pair", "split", "tree"],
+    ap.add_argument("--algorithm", default=None, choices=["config", "pair", "split", "wsplit", "tree"],
                     help="Keldysh-sum algorithm (default: the library's)")
     ap.add_argument("--observable", default="charge", choices=["charge", "kernel"],
                     help="kernel: only time the kernel method (compute_gf_kernel, 10 bins) on the same chain")
@@ -309,14 +420,20 @@ def main():
     def measure(label):
         if args.warmup:
             run_replicas(comp, 1000, args.warmup, args.points)
+        fb0 = comp.reduce_nr_fallback()
         est, dom, lu_ms, walls = run_replicas(comp, 0, args.steps, args.points)
+        fb = comp.reduce_nr_fallback() - fb0
         wall = float(walls.sum())
         n_total = args.steps * args.points
         pps = n_total / wall
         n_needed, table, wall_at_target = points_for_target(comp, args.target_replicas, TARGET_REL)
+        # the time to the target measured directly: one run of the points needed
+        # (median over replicas), launches and host round trips included
+        walls_at = run_replicas(comp, 30_000, 32, int(np.ceil(n_needed)))[3]
         mean = est.mean()
         return {"warper": label, "est": est, "dom": dom, "lu_ms": lu_ms, "wall": wall, "pps": pps,
-                "n_needed": n_needed, "t_needed": n_needed / pps, "table": table,
+                "n_needed": n_needed, "t_needed": float(np.median(walls_at)), "t_throughput": n_needed / pps,
+                "fallback": fb, "table": table,
                 "wall_at_target": wall_at_target, "mean": mean,
                 "charge": (-1j * (1j ** args.order) * mean).real,
                 "rel_rep": est.std(ddof=1) / abs(mean) if len(est) > 1 else float("nan"),
@@ -324,12 +441,14 @@ def main():
                 "timing": comp.last_timing()}
 
     def summary(m):
-        return {"warper": m["warper"], "time_to_1e-3_s": m["t_needed"], "charge_estimate": m["charge"],
+        return {"warper": m["warper"], "time_to_1e-3_s": m["t_needed"],
+                "time_to_1e-3_from_throughput_s": m["t_throughput"], "charge_estimate": m["charge"],
                 "rel_err_per_replica_at_points": m["rel_rep"], "points_needed_1e-3": m["n_needed"],
                 "points_per_s": m["pps"], "ms_per_replica": m["wall"] / args.steps * 1e3,
                 "rqmc_spread_table": [[a, r] for a, r, _ in m["table"]],
                 "wall_s_one_replica_at_target": m["wall_at_target"],
-                "in_domain_fraction": m["in_domain_fraction"]}
+                "in_domain_fraction": m["in_domain_fraction"],
+                "fallback_configurations_in_timed_replicas": int(m["fallback"])}
 
     plain = measure("plain")
     # The mixed-precision variant (LU in FP32, the Keldysh sum and the
@@ -342,10 +461,12 @@ def main():
         comp.set_precision("fp32")
         run_replicas(comp, 1000, 1, args.points)
         est32, _, lu32, walls32 = run_replicas(comp, 0, args.steps, args.points)
+        walls32_at = run_replicas(comp, 30_000, 32, int(np.ceil(plain["n_needed"])))[3]
         comp.set_precision("fp64")
         pps32 = args.steps * args.points / float(walls32.sum())
         dev = np.abs(est32 - plain["est"]) / abs(plain["mean"])
-        fp32_side = {"dtype": "f32-lu/f64-sum", "time_to_1e-3_s": plain["n_needed"] / pps32,
+        fp32_side = {"dtype": "f32-lu/f64-sum", "time_to_1e-3_s": float(np.median(walls32_at)),
+                     "time_to_1e-3_from_throughput_s": plain["n_needed"] / pps32,
                      "points_per_s": pps32, "speedup_vs_fp64": pps32 / plain["pps"],
                      "lu_ms_per_launch": float(lu32.mean()) / max(1, -(-args.points // (1 << 21))),
                      "max_rel_deviation_from_fp64_per_replica": float(dev.max()),
@@ -413,11 +534,13 @@ def main():
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     launches_per_step = max(1, -(-args.points // (1 << 21)))
     algo = comp.get_algorithm()
-    kname = (f"integrand_{'tree_' if algo == 'tree' else 'split_' if algo == 'split' else ''}kernel"
+    prefix = {"tree": "tree_", "split": "split_", "wsplit": "wsplit_"}.get(algo, "")
+    kname = (f"integrand_{prefix}kernel"
              f"<{args.order},{'double' if args.precision == 'fp64' else 'float'}{',pair' if algo == 'pair' else ''}>")
     traffic = pmc_traffic(kname, min(args.points, 1 << 21))
 
-    estimator = {"plain": summary(plain), "setup_s": setup_s}
+    estimator = {"plain": summary(plain), "setup_s": setup_s,
+                 "end_to_end_s": {"plain": setup_s + plain["t_needed"]}}
     if fp32_side:
         estimator["plain_fp32_lu"] = fp32_side
     if selection:
@@ -425,14 +548,19 @@ def main():
                                        "selected": head["warper"], "select_margin": SELECT_MARGIN,
                                        "training_s_total": train_s}
     if learned:
+        estimator["end_to_end_s"]["learned"] = setup_s + train_s + learned["t_needed"]
         pe, le = plain["est"], learned["est"]
-        err = np.hypot(pe.std(ddof=1), le.std(ddof=1)) / np.sqrt(len(pe))
+        # the charge per replica: Q_n = -i i^n I_n (compute_obs_test.cpp:133)
+        qp, ql = (-1j * (1j ** args.order) * pe).real, (-1j * (1j ** args.order) * le).real
+        err = np.hypot(qp.std(ddof=1), ql.std(ddof=1)) / np.sqrt(len(qp))
         estimator.update({
             "learned": summary(learned),
             "speedup_learned_vs_plain": plain["t_needed"] / learned["t_needed"],
+            "speedup_learned_vs_plain_from_throughput": plain["t_throughput"] / learned["t_throughput"],
             "variance_ratio_plain_over_learned_at_points": float(pe.var(ddof=1) / le.var(ddof=1)),
-            "unbiasedness": {"mean_plain": complex(pe.mean()).real, "mean_learned": complex(le.mean()).real,
-                             "difference_over_sigma": float(abs(le.mean() - pe.mean()) / err)},
+            "unbiasedness": {"charge_mean_plain": float(qp.mean()), "charge_mean_learned": float(ql.mean()),
+                             "difference_over_sigma": float(abs(ql.mean() - qp.mean()) / err),
+                             "replicas": len(qp), "points_per_replica": args.points},
             "flow_training": {"steps": args.flow_steps, "samples": args.flow_samples, "batch": args.flow_batch,
                               "lr": args.flow_lr, "layers": 6, "hidden": 64},
         })
@@ -462,13 +590,18 @@ def main():
                      "traffic_detail": traffic, "flops_per_point": fpp,
                      "lu_ms_per_launch": float(head["lu_ms"].mean()) / launches_per_step},
     }
+    if not args.no_workloads:
+        out["workloads"] = workload_rates()
     if not args.no_cpu:
         # the CPU runs the plain estimator: its time to 1e-3 = plain points needed / CPU points/s
         cb = cpu_baseline(model, proj, args.order, t_max, plain["n_needed"], args.cpu_seconds)
         out["cpu_baseline"] = {"value": cb["value"], "unit": "s", "cores": cb["cores"], "kind": "port",
                                "sample": f"{cb['sample_points']} scrambled Sobol points (seed 0) of the plain "
-                                         f"chain in {cb['seconds']:.1f} s ({cb['pps']:.0f} points/s)",
-                               "points_per_s": cb["pps"]}
+                                         f"chain in {cb['seconds']:.1f} s ({cb['pps']:.0f} points/s, "
+                                         f"{cb['structure']})",
+                               "points_per_s": cb["pps"], "structure": cb["structure"], "host": cb["where"],
+                               "structures": {k: {kk: vv for kk, vv in v.items() if kk != "sum"}
+                                              for k, v in cb["structures"].items()}}
         out["plain_gpu_vs_cpu"] = cb["value"] / plain["t_needed"]
     print(json.dumps(out))
 

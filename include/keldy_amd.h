@@ -50,9 +50,9 @@ extern "C" {
 #define KD_PREC_FP32_LU 1
 #define KD_PREC_FP32_LU_ALL 2
 
-/* Keldysh-sum algorithm.  KD_ALGO_DEFAULT (a new ctx's setting): KD_ALGO_SPLIT
- * at orders 7..10, KD_ALGO_CONFIG otherwise -- the faster of the two on MI355X
- * for full-size launches; kd_autotune measures every candidate instead.
+/* Keldysh-sum algorithm.  KD_ALGO_DEFAULT (a new ctx's setting): KD_ALGO_WSPLIT
+ * at orders 6..10, KD_ALGO_CONFIG otherwise -- the fastest on MI355X for
+ * full-size launches; kd_autotune measures every candidate instead.
  * KD_ALGO_CONFIG: one bordered LU per
  * configuration sigma (2^n of them, sharing the staged Wick matrix).
  * KD_ALGO_TREE (orders >= 6; lower orders use KD_ALGO_CONFIG): the
@@ -76,6 +76,14 @@ extern "C" {
  * workgroup.  Order 8: ~3.9x fewer multiply-adds than KD_ALGO_CONFIG, same
  * arithmetic up to reassociation (FP64 results agree to ~1e-13). */
 #define KD_ALGO_SPLIT 3
+/* KD_ALGO_WSPLIT (orders 6..10; otherwise KD_ALGO_CONFIG): the split
+ * algorithm with one point per wavefront and no workgroup barriers; the tree
+ * covers the leading n - 5 vertices from order 8 (2 below), its last two
+ * levels formed in one step, so each configuration finishes with a bordered
+ * LU over five vertices (four at order 6), the two choices of the last vertex
+ * paired.  Same arithmetic up to reassociation (FP64 results agree to
+ * ~1e-13). */
+#define KD_ALGO_WSPLIT 4
 
 typedef struct kd_ctx kd_ctx;
 

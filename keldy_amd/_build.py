@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "kernels/kd_flow.hip",
     "kernels/kd_cofactor.hip",
     "kernels/kd_split.hip",
+    "kernels/kd_wsplit.hip",
     "capi/kd_capi.cpp",
 ]
 

@@ -159,14 +159,16 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               v = KD_ALGO_PAIR;
             else if (a == "split")
               v = KD_ALGO_SPLIT;
+            else if (a == "wsplit")
+              v = KD_ALGO_WSPLIT;
             else if (a == "default")
               v = KD_ALGO_DEFAULT;
             else
-              throw keldy_error("algorithm must be 'default', 'config', 'pair', 'split' or 'tree'");
+              throw keldy_error("algorithm must be 'default', 'config', 'pair', 'split', 'wsplit' or 'tree'");
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
-          py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (split at orders 7..10, config otherwise), 'config', 'pair' "
-          "(orders >= 7), 'split' (orders 6..10) or 'tree' (orders >= 6)")
+          py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (wsplit at orders 6..10, config otherwise), 'config', 'pair' "
+          "(orders >= 7), 'split' or 'wsplit' (orders 6..10) or 'tree' (orders >= 6)")
       .def("get_algorithm",
            [](T &s) {
              int32_t v = KD_ALGO_CONFIG;
@@ -174,6 +176,7 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
              return std::string(v == KD_ALGO_TREE    ? "tree"
                                 : v == KD_ALGO_PAIR  ? "pair"
                                 : v == KD_ALGO_SPLIT ? "split"
+                                : v == KD_ALGO_WSPLIT ? "wsplit"
                                                      : "config");
            })
       .def("last_timing", &T::last_timing)

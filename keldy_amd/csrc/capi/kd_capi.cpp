@@ -79,7 +79,7 @@ struct kd_ctx {
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
   // point and waves per CU of the persistent grid
   int lu_lanes_max = 64, lu_waves_per_cu = 16;
-  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: split at orders 7..10)
+  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: wave-per-point split at orders 6..10)
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
   // time binning (kd_set_time_bins)
@@ -152,10 +152,16 @@ bool use_pair(const kd_ctx *c) {
   return c->algo == KD_ALGO_PAIR && c->order >= 7 && c->obs == KD_OBS_DIRECT && c->lu_lanes_max == 64;
 }
 
-// the split tree + per-configuration LU (orders 6..10)
+// the split tree + per-configuration LU with one point per wave (orders 6..10;
+// the default there)
+bool use_wsplit(const kd_ctx *c) {
+  const bool chosen = c->algo == KD_ALGO_WSPLIT || c->algo == KD_ALGO_DEFAULT;
+  return chosen && c->obs == KD_OBS_DIRECT && kd::wsplit_supported(c->order);
+}
+
+// the split tree + per-configuration LU with one point per workgroup (orders 6..10)
 bool use_split(const kd_ctx *c) {
-  const bool chosen = c->algo == KD_ALGO_SPLIT || (c->algo == KD_ALGO_DEFAULT && c->order >= 7);
-  return chosen && c->obs == KD_OBS_DIRECT && kd::split_threads(c->order) > 0;
+  return c->algo == KD_ALGO_SPLIT && c->obs == KD_OBS_DIRECT && kd::split_threads(c->order) > 0;
 }
 
 // complex values per point of an evaluation output
@@ -167,7 +173,7 @@ int64_t vals_per_point(const kd_ctx *c) {
 // group), capped at lu_waves_per_cu waves per CU -- the kernel is persistent
 // over the rest.
 int grid_for(const kd_ctx *c, int64_t n_points) {
-  const bool per_point = use_tree(c) || use_split(c); // one workgroup item per point
+  const bool per_point = use_tree(c) || use_split(c) || use_wsplit(c); // one workgroup item per point
   const int64_t per_batch = per_point ? 64 : kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
   int64_t nb = (n_points + 63) / 64 * per_batch;
   // waves per CU -> workgroups per CU (the split kernel's workgroups hold several waves)
@@ -380,6 +386,8 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepB
       KD_HIP(c, kd::launch_tree(N, c->precision, p, buf, grid, c->stream));
     else if (use_split(c))
       KD_HIP(c, kd::launch_split(N, c->precision, p, buf, grid, c->stream));
+    else if (use_wsplit(c))
+      KD_HIP(c, kd::launch_wsplit(N, c->precision, p, buf, grid, c->stream));
     else
       KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, use_pair(c), p, buf, grid, c->stream));
     if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
@@ -917,14 +925,14 @@ static double tree_flops(int N) {
 // det^2 s, the path determinant (6) and the signed FP64 sum -- or, where the
 // kernel pairs the two choices of the last vertex, the paired LU's count per
 // pair (as pair_flops over M vertices) plus two path determinants
-static double split_flops(int N) {
-  const int K = N <= 7 ? 2 : 3, M = N - K;
+static double split_like_flops(int N, int K, bool paired) {
+  const int M = N - K;
   double f = 0.0;
   for (int k = 0; k < K; k++) {
     const double dc = 2.0 * (N - k) - 1;
     f += std::ldexp(8 * dc * dc + 6 * dc + 13, k + 1);
   }
-  if (kd::split_paired(N)) {
+  if (paired) {
     const int MC = M + 2;
     double macs = 0, muls = 0, invs = 0;
     for (int i = 0; i < M - 1; i++) {
@@ -948,6 +956,24 @@ static double split_flops(int N) {
   return f;
 }
 
+static double split_flops(int N) { return split_like_flops(N, N <= 7 ? 2 : 3, kd::split_paired(N)); }
+// the wave-per-point split: K = n - 5 tree levels from order 8 (2 below), the
+// last two formed in one step from level K - 2 -- per level-K node and row of
+// its Dc = 2(n-K)+1, the 2x2 pivot block's determinant and inverse (21) and
+// the row's two coefficients (40), two multiply-adds per entry, and per node
+// the second pivot and the path determinant (19) -- then the five remaining
+// vertices paired (order 6: four, unpaired)
+static double wsplit_flops(int N) {
+  const int K = N <= 7 ? 2 : N - 5;
+  double f = split_like_flops(N, K, N >= 7);
+  for (int k = K - 2; k < K; k++) { // the two single steps split_like_flops counted
+    const double dc = 2.0 * (N - k) - 1;
+    f -= std::ldexp(8 * dc * dc + 6 * dc + 13, k + 1);
+  }
+  const double dc = 2.0 * (N - K) + 1;
+  return f + std::ldexp(16 * dc * dc + 61 * dc + 19, K);
+}
+
 int kd_flops_per_point(const kd_ctx *c, double *flops) {
   if (!c || !flops) return KD_ERR_ARG;
   if (c->obs != KD_OBS_DIRECT) {
@@ -964,6 +990,10 @@ int kd_flops_per_point(const kd_ctx *c, double *flops) {
   }
   if (use_split(c)) {
     *flops = split_flops(c->order);
+    return KD_OK;
+  }
+  if (use_wsplit(c)) {
+    *flops = wsplit_flops(c->order);
     return KD_OK;
   }
   const double N = c->order < 0 ? 0 : c->order;
@@ -1271,13 +1301,23 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16
   // or 32 waves per CU
   const bool direct = c->obs == KD_OBS_DIRECT;
-  const int li0 = !direct ? 0 : kd::split_threads(c->order) > 0 ? -3 : c->order >= 7 ? -2 : c->order >= 6 ? -1 : 0;
+  const int li0 = !direct                           ? 0
+                  : kd::wsplit_supported(c->order)   ? -4
+                  : kd::split_threads(c->order) > 0 ? -3
+                  : c->order >= 7                   ? -2
+                  : c->order >= 6                   ? -1
+                                                    : 0;
   for (int li = li0; li < (direct ? 3 : 1) && !rc; li++) {
+    if (li == -3 && kd::split_threads(c->order) == 0) continue;
     if (li == -2 && c->order < 7) continue;
     const int L = li < 0 ? 64 : lane_opts[li];
     if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
     for (int wi = 0; wi < 3 && !rc; wi++) {
-      c->algo = li == -3 ? KD_ALGO_SPLIT : li == -2 ? KD_ALGO_PAIR : li == -1 ? KD_ALGO_TREE : KD_ALGO_CONFIG;
+      c->algo = li == -4   ? KD_ALGO_WSPLIT
+                : li == -3 ? KD_ALGO_SPLIT
+                : li == -2 ? KD_ALGO_PAIR
+                : li == -1 ? KD_ALGO_TREE
+                           : KD_ALGO_CONFIG;
       c->lu_lanes_max = L;
       c->lu_waves_per_cu = wave_opts[wi];
       float t = 1e30f;
@@ -1304,7 +1344,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   c->lu_waves_per_cu = best_w;
   c->algo = best_a;
   c->tune_ms_per_mpt = best * 1048576.f / (float)sample_points;
-  if (lanes) *lanes = use_tree(c) || use_split(c) ? 0 : kd::lu_lanes(c->order, best_l);
+  if (lanes) *lanes = use_tree(c) || use_split(c) || use_wsplit(c) ? 0 : kd::lu_lanes(c->order, best_l);
   if (waves_per_cu) *waves_per_cu = best_w;
   if (ms_per_mpoint) *ms_per_mpoint = c->tune_ms_per_mpt;
   return KD_OK;
@@ -1322,7 +1362,7 @@ int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
 int kd_set_algorithm(kd_ctx *c, int32_t algo) {
   if (!c) return KD_ERR_ARG;
   if (algo != KD_ALGO_DEFAULT && algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE && algo != KD_ALGO_PAIR &&
-      algo != KD_ALGO_SPLIT)
+      algo != KD_ALGO_SPLIT && algo != KD_ALGO_WSPLIT)
     return fail(c, KD_ERR_ARG, "unknown Keldysh-sum algorithm");
   c->algo = algo;
   return KD_OK;
@@ -1330,7 +1370,11 @@ int kd_set_algorithm(kd_ctx *c, int32_t algo) {
 
 int kd_get_algorithm(const kd_ctx *c, int32_t *algo) {
   if (!c || !algo) return KD_ERR_ARG;
-  *algo = use_tree(c) ? KD_ALGO_TREE : use_pair(c) ? KD_ALGO_PAIR : use_split(c) ? KD_ALGO_SPLIT : KD_ALGO_CONFIG;
+  *algo = use_tree(c)     ? KD_ALGO_TREE
+          : use_pair(c)   ? KD_ALGO_PAIR
+          : use_split(c)  ? KD_ALGO_SPLIT
+          : use_wsplit(c) ? KD_ALGO_WSPLIT
+                          : KD_ALGO_CONFIG;
   return KD_OK;
 }
 

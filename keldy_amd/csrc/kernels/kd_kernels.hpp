@@ -36,6 +36,10 @@ int split_threads(int order);
 // whether it runs the configurations as pairs sharing the last vertex's elimination
 bool split_paired(int order);
 hipError_t launch_split(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
+// integrand_wsplit_kernel<order, T> (orders 6..10): the split algorithm with
+// one point per 64-thread workgroup (one wave), no workgroup barriers.
+bool wsplit_supported(int order);
+hipError_t launch_wsplit(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // cofactor_kernel<order> (kernel method, orders 1..12, FP64): grid blocks of
 // 64 threads, same work items as the Keldysh-sum kernel.
 hipError_t launch_cofactor(int order, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);

@@ -34,6 +34,7 @@
 #include "kd_kernels.hpp"
 #include "kd_lu.hpp"
 #include "kd_pivoted.hpp"
+#include "kd_split_lu.hpp"
 
 namespace kd {
 
@@ -69,9 +70,6 @@ template <int N> struct SplitShape {
   }
 };
 
-template <typename T> __device__ __forceinline__ cx<T> ldx(const double2 &v) { return cmk<T>((T)v.x, (T)v.y); }
-template <typename T> __device__ __forceinline__ cx<T> ldx(const float2 &v) { return cmk<T>((T)v.x, (T)v.y); }
-
 // Level k -> k+1: NCH = 2 NPAR children of NPAR = 2^k parent tables (Dp rows at
 // stride Sp).  Child m = pm + NPAR s eliminates vertex k with Keldysh index s
 // from parent pm.  Work item = (child, row, column chunk): each child row is
@@ -92,14 +90,24 @@ __device__ __forceinline__ void split_level(const PT *__restrict__ par, cx<T> *_
     const int m = q / Dc, a = q - m * Dc;
     const int pm = m & (NPAR - 1), s = m / NPAR;
     const PT *P = par + pm * (Dp * Sp);
-    const cx<T> piv = ldx<T>(P[s * Sp + s]);
-    const cx<T> l = cmul<T>(ldx<T>(P[(a + 2) * Sp + s]), cinv<T>(piv));
     const PT *Pa = P + (a + 2) * Sp + 2 + c0;
     const PT *Ps = P + s * Sp + 2 + c0;
+    // every operand of the item loaded before any result is stored: the
+    // compiler cannot move these loads above the stores into `ch` by itself,
+    // and a load-wait-store round trip per column would serialise the item
+    cx<T> pa[CW], ps[CW];
+#pragma unroll
+    for (int b = 0; b < CW; b++)
+      if (Dc % CW == 0 || c0 + b < Dc) {
+        pa[b] = ldx<T>(Pa[b]);
+        ps[b] = ldx<T>(Ps[b]);
+      }
+    const cx<T> piv = ldx<T>(P[s * Sp + s]);
+    const cx<T> l = cmul<T>(ldx<T>(P[(a + 2) * Sp + s]), cinv<T>(piv));
     cx<T> *C = ch + m * (Dc * Sc) + a * Sc + c0;
 #pragma unroll
     for (int b = 0; b < CW; b++)
-      if (Dc % CW == 0 || c0 + b < Dc) C[b] = cfms<T>(ldx<T>(Pa[b]), l, ldx<T>(Ps[b]));
+      if (Dc % CW == 0 || c0 + b < Dc) C[b] = cfms<T>(pa[b], l, ps[b]);
     if (a == 0 && c0 == 0) {
       const double2 d = dpath[NPAR + pm];
       dpath[2 * NPAR + m] = make_double2(d.x * (double)piv.x - d.y * (double)piv.y,
@@ -108,169 +116,6 @@ __device__ __forceinline__ void split_level(const PT *__restrict__ par, cx<T> *_
     }
   }
   __syncthreads();
-}
-
-// det(A)^2 s of one configuration from its level-K node table Tn (2M+1 rows at
-// stride wick_stride(M)): loc holds sigma_K..sigma_{n-1} in bits 0..M-1; thr
-// the pivot thresholds from index 2K on; dp the node's path determinant.
-template <int M, typename T, typename PT>
-__device__ __forceinline__ double2 split_config(const PT *__restrict__ Tn, int loc, const T *__restrict__ thr,
-                                                double2 dp, bool &bad) {
-  constexpr int S = wick_stride(M);
-  int pr[M + 1], pc[M + 1];
-  static_for<0, M>([&](auto I) {
-    constexpr int i = I;
-    const int pi = 2 * i + ((loc >> i) & 1);
-    pr[i] = pi * S;
-    pc[i] = pi;
-  });
-  pr[M] = 2 * M * S;
-  pc[M] = 2 * M;
-  cx<T> Uh[M + 1][M + 1];
-  cx<T> det = cmk<T>(T(1), T(0));
-  cx<T> s = cmk<T>(T(0), T(0));
-  bool flag = false;
-  static_for<0, M + 1>([&](auto I) {
-    constexpr int i = I;
-    cx<T> row[M + 1];
-    // opaque row offset: the row's addresses are formed here, not all up front
-    int ri = pr[i];
-    __asm__ volatile("" : "+v"(ri));
-    static_for<0, M + 1>([&](auto J) {
-      constexpr int j = J;
-      row[j] = ldx<T>(Tn[ri + pc[j]]);
-    });
-    static_for<0, i>([&](auto KB) {
-      constexpr int kb = KB;
-      const cx<T> l = row[kb];
-      static_for<kb + 1, M + 1>([&](auto J) {
-        constexpr int j = J;
-        row[j] = cfms<T>(row[j], l, Uh[kb][j]);
-      });
-    });
-    if constexpr (i < M) {
-      const cx<T> piv = row[i];
-      flag |= !(cabs1<T>(piv) >= thr[pc[i]]);
-      if constexpr (i == 0)
-        det = piv;
-      else
-        det = cmul<T>(det, piv);
-      const cx<T> inv = cinv<T>(piv);
-      static_for<i + 1, M + 1>([&](auto J) {
-        constexpr int j = J;
-        Uh[i][j] = cmul<T>(row[j], inv);
-      });
-    } else {
-      s = row[M];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  });
-  bad = flag;
-  const double2 dt = make_double2(dp.x * (double)det.x - dp.y * (double)det.y,
-                                  dp.x * (double)det.y + dp.y * (double)det.x);
-  return cmul<double>(cmul<double>(dt, dt), make_double2((double)s.x, (double)s.y));
-}
-
-// det(A)^2 s of the two configurations that differ only in the Keldysh index of
-// the last vertex (local bit M-1 of loc: 0 for v[0], 1 for v[1]), from the same
-// node table.  The first M-1 vertex rows are eliminated once over the columns
-// of both choices of the last vertex (and the border), the border row likewise;
-// only the last vertex row and the corner update are per choice -- at M = 5,
-// 70 instead of 110 complex multiply-adds and 6 instead of 10 reciprocals per
-// pair, the same arithmetic per configuration as split_config.
-template <int M, typename T, typename PT>
-__device__ __forceinline__ void split_config_pair(const PT *__restrict__ Tn, int loc, const T *__restrict__ thr,
-                                                  double2 dp, double2 (&v)[2], bool (&bad)[2]) {
-  constexpr int S = wick_stride(M);
-  constexpr int MC = M + 2; // columns: vertices 0..M-2, last vertex (index 0), last vertex (index 1), border
-  int pc[MC];
-  static_for<0, M - 1>([&](auto I) {
-    constexpr int i = I;
-    pc[i] = 2 * i + ((loc >> i) & 1);
-  });
-  pc[M - 1] = 2 * M - 2;
-  pc[M] = 2 * M - 1;
-  pc[M + 1] = 2 * M;
-  cx<T> Uh[M - 1][MC];
-  cx<T> det = cmk<T>(T(1), T(0));
-  bool flag = false;
-  static_for<0, M - 1>([&](auto I) {
-    constexpr int i = I;
-    cx<T> row[MC];
-    int ri = pc[i] * S;
-    __asm__ volatile("" : "+v"(ri));
-    static_for<0, MC>([&](auto J) {
-      constexpr int j = J;
-      row[j] = ldx<T>(Tn[ri + pc[j]]);
-    });
-    static_for<0, i>([&](auto KB) {
-      constexpr int kb = KB;
-      const cx<T> l = row[kb];
-      static_for<kb + 1, MC>([&](auto J) {
-        constexpr int j = J;
-        row[j] = cfms<T>(row[j], l, Uh[kb][j]);
-      });
-    });
-    const cx<T> piv = row[i];
-    flag |= !(cabs1<T>(piv) >= thr[pc[i]]);
-    if constexpr (i == 0)
-      det = piv;
-    else
-      det = cmul<T>(det, piv);
-    const cx<T> inv = cinv<T>(piv);
-    static_for<i + 1, MC>([&](auto J) {
-      constexpr int j = J;
-      Uh[i][j] = cmul<T>(row[j], inv);
-    });
-    __builtin_amdgcn_sched_barrier(0);
-  });
-  // border row over the same columns (its border entry is the node's corner)
-  cx<T> rb[MC];
-  {
-    int ri = 2 * M * S;
-    __asm__ volatile("" : "+v"(ri));
-    static_for<0, MC>([&](auto J) {
-      constexpr int j = J;
-      rb[j] = ldx<T>(Tn[ri + pc[j]]);
-    });
-  }
-  static_for<0, M - 1>([&](auto KB) {
-    constexpr int kb = KB;
-    const cx<T> l = rb[kb];
-    static_for<kb + 1, MC>([&](auto J) {
-      constexpr int j = J;
-      rb[j] = cfms<T>(rb[j], l, Uh[kb][j]);
-    });
-  });
-  __builtin_amdgcn_sched_barrier(0);
-  // the last vertex row for each Keldysh index t: its own column and the border
-  static_for<0, 2>([&](auto TT) {
-    constexpr int t = TT;
-    constexpr int ct = M - 1 + t; // its column
-    cx<T> row[MC];
-    int ri = pc[ct] * S;
-    __asm__ volatile("" : "+v"(ri));
-    static_for<0, MC>([&](auto J) {
-      constexpr int j = J;
-      if constexpr (j < M - 1 || j == ct || j == MC - 1) row[j] = ldx<T>(Tn[ri + pc[j]]);
-    });
-    static_for<0, M - 1>([&](auto KB) {
-      constexpr int kb = KB;
-      const cx<T> l = row[kb];
-      static_for<kb + 1, MC>([&](auto J) {
-        constexpr int j = J;
-        if constexpr (j < M - 1 || j == ct || j == MC - 1) row[j] = cfms<T>(row[j], l, Uh[kb][j]);
-      });
-    });
-    const cx<T> piv = row[ct];
-    bad[t] = flag | !(cabs1<T>(piv) >= thr[pc[ct]]);
-    const cx<T> dt = cmul<T>(det, piv);
-    const cx<T> u = cmul<T>(row[MC - 1], cinv<T>(piv));
-    const cx<T> sc = cfms<T>(rb[MC - 1], rb[ct], u);
-    const double2 dd = make_double2(dp.x * (double)dt.x - dp.y * (double)dt.y, dp.x * (double)dt.y + dp.y * (double)dt.x);
-    v[t] = cmul<double>(cmul<double>(dd, dd), make_double2((double)sc.x, (double)sc.y));
-    __builtin_amdgcn_sched_barrier(0);
-  });
 }
 
 #ifndef KD_SPLIT_WAVES_PER_EU
@@ -328,8 +173,8 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
       if (!mask) continue;
       const int j1 = __builtin_ctzll(mask);
       const int64_t s1 = (int64_t)__shfl(bl, j1, 64) * 64 + __shfl(gl, j1, 64);
-      if (t < N) u_n = buf.u[s1 * N + t];
-      if (t == 0) jac_n = buf.jac[s1];
+      u_n = buf.u[s1 * N + (t < N ? t : N - 1)];
+      jac_n = buf.jac[s1];
     }
     const int j = __builtin_ctzll(mask);
     mask &= mask - 1;
@@ -340,8 +185,10 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
     if (mask) {
       const int j2 = __builtin_ctzll(mask);
       const int64_t s2 = (int64_t)__shfl(bl, j2, 64) * 64 + __shfl(gl, j2, 64);
-      if (t < N) u_n = buf.u[s2 * N + t];
-      if (t == 0) jac_n = buf.jac[s2];
+      // every thread loads, so that no thread keeps an old value: a partial
+      // load makes the compiler copy the register and wait for the load at once
+      u_n = buf.u[s2 * N + (t < N ? t : N - 1)];
+      jac_n = buf.jac[s2];
     }
     // Opaque per-point copy of the thread id: keeps the thread-only index math
     // of the matrix build, the tree levels and the configurations' LDS
@@ -408,8 +255,8 @@ integrand_split_kernel(KdParams p, PrepBuf buf) {
         const int cfg0 = node | (loc << K);
         double2 v[2];
         bool bad[2];
-        split_config_pair<M, T>(LK + node * (S::Dk(K) * S::Sk(K)), loc, thr + 2 * K, dpath[(1 << K) + node], v,
-                                bad);
+        split_config_pair<M, T, cx<T>, false>(LK + node * (S::Dk(K) * S::Sk(K)), loc, thr + 2 * K,
+                                              dpath[(1 << K) + node], v, bad);
         const bool fp = fpath[(1 << K) + node] != 0;
         bad[0] = bad[0] || fp;
         bad[1] = bad[1] || fp;

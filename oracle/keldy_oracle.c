@@ -498,9 +498,14 @@ static cplx g0c(const ko_g0 *g, const ko_gf_index *a, const ko_gf_index *b, int 
   return o[0] + I * o[1];
 }
 
-/* integrand_g_direct::operator() (wick_direct.cpp:43-158) */
-int ko_integrand_eval(const ko_integrand *f, int n, const double *times, int keep_u_hypercube,
-                      double out[2]) {
+/* integrand_g_direct::operator() (wick_direct.cpp:43-158).  threads > 1
+ * follows the reference's own parallel structure: OpenMP over the rows of the
+ * Wick matrices (wick_direct.cpp:81,93) and over the 2^n configurations
+ * (wick_direct.cpp:110), each configuration with its own heap-allocated
+ * matrices as the reference's TRIQS matrices are; threads <= 1 is the serial
+ * loop, configurations summed in order. */
+static int integrand_eval_impl(const ko_integrand *f, int n, const double *times, int keep_u_hypercube,
+                               double out[2], int threads) {
   out[0] = out[1] = 0.0;
   if (f->A.spin != f->B.spin) return 0;
   if (keep_u_hypercube)
@@ -532,6 +537,9 @@ int ko_integrand_eval(const ko_integrand *f, int n, const double *times, int kee
   cplx *W1 = (cplx *)malloc(sizeof(cplx) * d1 * d1);
   cplx *W2 = (cplx *)malloc(sizeof(cplx) * n2 * n2);
   W1[ext * d1 + ext] = g0c(&f->g0, &a, &b, 0);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) if (threads > 1)
+#endif
   for (int i = 0; i < n2; i++) {
     W1[ext * d1 + i] = g0c(&f->g0, &a, &c1[i], 1);
     W1[i * d1 + ext] = g0c(&f->g0, &c1[i], &b, 1);
@@ -540,33 +548,68 @@ int ko_integrand_eval(const ko_integrand *f, int n, const double *times, int kee
       W2[i * n2 + j] = g0c(&f->g0, &c2[i], &c2[j], 1);
     }
   }
-  cplx sum = 0.0;
-  cplx *M1 = (cplx *)malloc(sizeof(cplx) * (n + 1) * (n + 1));
-  cplx *M2 = (cplx *)malloc(sizeof(cplx) * n * n);
-  int pick[KO_MAX_DIM + 1];
   uint64_t nconf = (uint64_t)1 << n;
-  for (uint64_t idx = 0; idx < nconf; idx++) {
-    for (int i = 0; i < n; i++) pick[i] = i + (int)((idx >> i) & 1u) * n;
-    pick[n] = ext;
-    for (int i = 0; i <= n; i++)
-      for (int j = 0; j <= n; j++) M1[i * (n + 1) + j] = W1[pick[i] * d1 + pick[j]];
-    M1[n * (n + 1) + n] = 0.0;
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) M2[i * n + j] = W2[pick[i] * n2 + pick[j]];
-    double d1v[2], d2v[2];
-    ko_det((double *)M1, n + 1, d1v);
-    ko_det((double *)M2, n, d2v);
-    int par = 1 - 2 * __builtin_parityll(idx);
-    sum += (double)par * (d1v[0] + I * d1v[1]) * (d2v[0] + I * d2v[1]);
+  if (threads <= 1) {
+    cplx sum = 0.0;
+    cplx *M1 = (cplx *)malloc(sizeof(cplx) * (n + 1) * (n + 1));
+    cplx *M2 = (cplx *)malloc(sizeof(cplx) * n * n);
+    int pick[KO_MAX_DIM + 1];
+    for (uint64_t idx = 0; idx < nconf; idx++) {
+      for (int i = 0; i < n; i++) pick[i] = i + (int)((idx >> i) & 1u) * n;
+      pick[n] = ext;
+      for (int i = 0; i <= n; i++)
+        for (int j = 0; j <= n; j++) M1[i * (n + 1) + j] = W1[pick[i] * d1 + pick[j]];
+      M1[n * (n + 1) + n] = 0.0;
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) M2[i * n + j] = W2[pick[i] * n2 + pick[j]];
+      double d1v[2], d2v[2];
+      ko_det((double *)M1, n + 1, d1v);
+      ko_det((double *)M2, n, d2v);
+      int par = 1 - 2 * __builtin_parityll(idx);
+      sum += (double)par * (d1v[0] + I * d1v[1]) * (d2v[0] + I * d2v[1]);
+    }
+    free(M1);
+    free(M2);
+    out[0] = creal(sum);
+    out[1] = cimag(sum);
+  } else {
+    double sre = 0.0, sim = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) reduction(+ : sre, sim)
+#endif
+    for (int64_t idx = 0; idx < (int64_t)nconf; idx++) {
+      int pick[KO_MAX_DIM + 1];
+      cplx *M1 = (cplx *)malloc(sizeof(cplx) * (n + 1) * (n + 1));
+      cplx *M2 = (cplx *)malloc(sizeof(cplx) * n * n);
+      for (int i = 0; i < n; i++) pick[i] = i + (int)((idx >> i) & 1u) * n;
+      pick[n] = ext;
+      for (int i = 0; i <= n; i++)
+        for (int j = 0; j <= n; j++) M1[i * (n + 1) + j] = W1[pick[i] * d1 + pick[j]];
+      M1[n * (n + 1) + n] = 0.0;
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) M2[i * n + j] = W2[pick[i] * n2 + pick[j]];
+      double d1v[2], d2v[2];
+      ko_det((double *)M1, n + 1, d1v);
+      ko_det((double *)M2, n, d2v);
+      const int par = 1 - 2 * __builtin_parityll((uint64_t)idx);
+      const cplx term = (double)par * (d1v[0] + I * d1v[1]) * (d2v[0] + I * d2v[1]);
+      sre += creal(term);
+      sim += cimag(term);
+      free(M1);
+      free(M2);
+    }
+    out[0] = sre;
+    out[1] = sim;
   }
   free(W1);
   free(W2);
-  free(M1);
-  free(M2);
-  if (cabs(sum) < f->cutoff) sum = 0.0;
-  out[0] = creal(sum);
-  out[1] = cimag(sum);
+  if (cabs(out[0] + I * out[1]) < f->cutoff) out[0] = out[1] = 0.0;
   return 1;
+}
+
+int ko_integrand_eval(const ko_integrand *f, int n, const double *times, int keep_u_hypercube,
+                      double out[2]) {
+  return integrand_eval_impl(f, n, times, keep_u_hypercube, out, 1);
 }
 
 /* integrand_g_direct::eval_no_det (wick_direct.cpp:161-228) */
@@ -639,6 +682,25 @@ int ko_run(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint6
     acc->n_in_domain += (uint64_t)dom;
     acc->re += v[0];
     acc->im += v[1];
+    acc->n_points++;
+  }
+  return 0;
+}
+
+/* integrator::run with the reference's intra-point parallelism: points in
+ * sequence, each integrand evaluated with `threads` OpenMP threads over its
+ * Wick rows and configurations (integrand_eval_impl). */
+int ko_run_intrapoint(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint64_t n_steps,
+                      int threads, ko_accum *acc) {
+  double l[KO_MAX_DIM], u[KO_MAX_DIM], v[2];
+  int n = s->dim;
+  for (uint64_t i = 0; i < n_steps; i++) {
+    if (ko_sobol_next(s, l) != 0) return -1;
+    double jac = ko_warp_reverse(w, nw, n, l, u);
+    int dom = integrand_eval_impl(f, n, u, 1, v, threads);
+    acc->n_in_domain += (uint64_t)dom;
+    acc->re += v[0] * jac;
+    acc->im += v[1] * jac;
     acc->n_points++;
   }
   return 0;

@@ -145,6 +145,10 @@ int ko_eval_warped(const ko_integrand *f, const ko_warper *w, int nw, int n, con
 int ko_run(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint64_t n_steps,
            ko_accum *acc);
 /* same sum, points split over `threads` contiguous Sobol ranges (OpenMP) */
+/* integrator::run with OpenMP over each point's Wick rows and configurations
+ * (the reference's own structure, wick_direct.cpp:81,93,110). */
+int ko_run_intrapoint(const ko_integrand *f, const ko_warper *w, int nw, ko_sobol *s, uint64_t n_steps,
+                      int threads, ko_accum *acc);
 int ko_run_parallel(const ko_integrand *f, const ko_warper *w, int nw, const ko_sobol *s0,
                     uint64_t n_steps, int threads, ko_accum *acc);
 

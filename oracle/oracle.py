@@ -105,6 +105,8 @@ def load(path):
                          C.c_uint64, C.POINTER(_Accum)]
     L.ko_run_parallel.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int,
                                   C.POINTER(_Sobol), C.c_uint64, C.c_int, C.POINTER(_Accum)]
+    L.ko_run_intrapoint.argtypes = [C.POINTER(_Integrand), C.POINTER(_Warper), C.c_int,
+                                    C.POINTER(_Sobol), C.c_uint64, C.c_int, C.POINTER(_Accum)]
     L.ko_first_row_expansion.argtypes = [dp, C.c_int, dp, dp]
     L.ko_kernel_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int, dp]
     L.ko_kernel_omega_eval.argtypes = [C.POINTER(_G0), C.POINTER(_GfIndex), C.c_int, dp, C.c_int,
@@ -370,10 +372,16 @@ def eval_warped(integrand, train, l, keep_u_hypercube=True):
     return complex(out[0], out[1]), dom
 
 
-def run(integrand, train, sobol, n_steps, threads=1):
-    """integrator::run restated; returns (sum, n_points, n_in_domain)."""
+def run(integrand, train, sobol, n_steps, threads=1, intrapoint=False):
+    """integrator::run restated; returns (sum, n_points, n_in_domain).
+    threads > 1 splits the points over the threads (contiguous Sobol ranges),
+    or, with intrapoint, parallelises inside each point over its Wick rows and
+    configurations as the reference does (wick_direct.cpp:81,93,110)."""
     acc = _Accum()
-    if threads == 1:
+    if intrapoint:
+        rc = lib().ko_run_intrapoint(C.byref(integrand.c), train.arr, len(train.warpers), C.byref(sobol.s),
+                                     n_steps, threads, C.byref(acc))
+    elif threads == 1:
         rc = lib().ko_run(C.byref(integrand.c), train.arr, len(train.warpers), C.byref(sobol.s),
                           n_steps, C.byref(acc))
     else:

@@ -13,7 +13,8 @@ sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
 
-@pytest.mark.parametrize("kernel", ["integrand_split_kernel<8,double>", "integrand_kernel<8,double>"])
+@pytest.mark.parametrize("kernel", ["integrand_wsplit_kernel<8,double>", "integrand_wsplit_kernel<8,float>",
+                                    "integrand_split_kernel<8,double>", "integrand_kernel<8,double>"])
 def test_pmc_traffic_from_committed_summaries(kernel):
     t = bench.pmc_traffic(kernel, 1 << 21)
     assert t is not None, kernel

@@ -155,8 +155,10 @@ def test_pair_matches_per_configuration_and_oracle(flat_g0, order, prec):
     assert abs(complex(ra.re, ra.im) - complex(rb.re, rb.im)) <= tol * abs(complex(rb.re, rb.im)) * 10
 
 
-@pytest.mark.parametrize("order,prec", [(6, 0), (7, 0), (8, 0), (9, 0), (10, 0), (7, 1), (8, 1), (9, 1), (10, 2)])
-def test_split_matches_per_configuration_and_oracle(flat_g0, order, prec):
+@pytest.mark.parametrize("algo", [capi.KD_ALGO_SPLIT, capi.KD_ALGO_WSPLIT], ids=["split", "wsplit"])
+@pytest.mark.parametrize("order,prec", [(6, 0), (7, 0), (8, 0), (9, 0), (10, 0), (6, 1), (7, 1), (8, 1), (9, 1),
+                                        (10, 2)])
+def test_split_matches_per_configuration_and_oracle(flat_g0, order, prec, algo):
     """The split algorithm (Schur tree over the leading vertices, then one
     bordered LU per configuration) against the per-configuration LU on the same
     points: FP64 to 1e-12 normwise, FP32 to 1e-4 (1e-2 with FP32 at order 10,
@@ -172,9 +174,9 @@ def test_split_matches_per_configuration_and_oracle(flat_g0, order, prec):
     u[5, :] = u[5, 0]  # coinciding vertex times
     a = U.charge_ctx(flat_g0, order, t, prec)
     b = U.charge_ctx(flat_g0, order, t, prec)
-    a.set_algorithm(capi.KD_ALGO_SPLIT)
+    a.set_algorithm(algo)
     b.set_algorithm(capi.KD_ALGO_CONFIG)
-    assert a.algorithm() == capi.KD_ALGO_SPLIT and b.algorithm() == capi.KD_ALGO_CONFIG
+    assert a.algorithm() == algo and b.algorithm() == capi.KD_ALGO_CONFIG
     va, da = a.eval_batch(u)
     vb, db = b.eval_batch(u)
     assert np.array_equal(da, db)
@@ -182,7 +184,8 @@ def test_split_matches_per_configuration_and_oracle(flat_g0, order, prec):
     assert np.max(np.abs(va - vb)) <= tol * np.max(np.abs(vb))
     fa, fb = a.flops_per_point(), b.flops_per_point()
     assert fa < fb
-    print(f"order {order}: split {fa:.0f} flops/point vs per-configuration {fb:.0f} ({fb / fa:.2f}x)")
+    print(f"order {order}: {'wsplit' if algo == capi.KD_ALGO_WSPLIT else 'split'} {fa:.0f} flops/point "
+          f"vs per-configuration {fb:.0f} ({fb / fa:.2f}x)")
     if order == 8 and prec == 0:
         ref, rdom = O.charge_integrand(U.oracle_charge(flat_g0), t).batch(u[:32])
         assert np.array_equal(da[:32], rdom)

@@ -202,20 +202,22 @@ def test_pilot_histograms_on_device_match_host_binning(model30):
 
 
 def test_default_algorithm_per_order(model30):
-    """A new computer uses the split algorithm at orders 7-10 and one LU per
-    configuration below; an explicit choice holds until 'default' restores the
+    """A new computer uses the wave-per-point split algorithm at orders 6-10
+    and one LU per configuration below; an explicit choice holds until 'default' restores the
     per-order rule; unknown names raise; both give the same run on the same
     points (1e-12 relative)."""
     from keldy_amd import impurity_oneband as kio
     from keldy_amd import warpers as kw
     t = 10.0
-    for order, want in ((5, "config"), (6, "config"), (7, "split"), (8, "split"), (10, "split")):
+    for order, want in ((5, "config"), (6, "wsplit"), (7, "wsplit"), (8, "wsplit"), (10, "wsplit"), (11, "config")):
         assert kio.ComputeChargeQDirect(model30, t, order, 0.0).get_algorithm() == want, order
     c = kio.ComputeChargeQDirect(model30, t, 8, 0.0)
     c.set_algorithm("config")
     assert c.get_algorithm() == "config"
-    c.set_algorithm("default")
+    c.set_algorithm("split")
     assert c.get_algorithm() == "split"
+    c.set_algorithm("default")
+    assert c.get_algorithm() == "wsplit"
     with pytest.raises(RuntimeError, match="algorithm must be"):
         c.set_algorithm("gray")
     res = []

@@ -47,6 +47,7 @@ def _shapes(order):
            ("tree", capi.KD_ALGO_TREE, 64)]
     if order <= 10:
         out.append(("split", capi.KD_ALGO_SPLIT, 64))
+        out.append(("wsplit", capi.KD_ALGO_WSPLIT, 64))
     return out
 
 
@@ -67,7 +68,7 @@ def oracle_ref(flat_g0):
 
 @pytest.mark.parametrize("order", [9, 10, 11, 12])
 def test_eval_batch_fp64_every_shape_vs_oracle(flat_g0, oracle_ref, order):
-    """Every algorithm (split, tree, paired, per-configuration at 64, 32 and 16
+    """Every algorithm (split, wave-per-point split, tree, paired, per-configuration at 64, 32 and 16
     lanes per point) against the oracle.  At orders 11-12 with 16 or 32 lanes a
     lane holds 128-256 configurations; the coinciding-vertex rows flag some of
     them for the pivoted fallback, which must recompute each exactly once."""
@@ -159,7 +160,7 @@ def test_half_filling_alpha_half_all_fallback(flat_g0, order):
         assert np.array_equal(dom, rdom), label
         assert np.all(np.isfinite(got)), label
         assert np.max(np.abs(got - ref)) <= TOL * scale, label
-        if label in ("split", "config64", "tree"):
+        if label in ("split", "wsplit", "config64", "tree"):
             # the uniform map on [0, t]^n: every point in the domain
             _, (descs, maps, tabs) = U.chain([("map", "uniform", t, 1.0, 1)])
             ctx.set_warper_chain(descs, maps, tabs)

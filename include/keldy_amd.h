@@ -104,6 +104,10 @@ typedef struct {
 #define KD_MAP_INVERSE_POW 4 /* make_product_1d_simple_inverse_power (610-630) */
 #define KD_MAP_NEAREST 5     /* warper_product_1d_simple_interp_nearest_t (174-379) */
 #define KD_MAP_HYBRID 6      /* warper_product_1d_simple_interp_hybrid_t (381-554), f = w^p/(w+u)^p */
+/* warper_projection_t (warpers/projection.hpp:176-382), one axis: u = Finv(l) and
+ * f = fn(u) / norm, Finv (n_inv values at t_off) and fn (n_pts values at f_off)
+ * on uniform meshes of [0, 1], each evaluated by linear interpolation */
+#define KD_MAP_PROJECTION 7
 
 typedef struct {
   int32_t kind;

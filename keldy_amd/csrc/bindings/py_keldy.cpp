@@ -308,6 +308,42 @@ PYBIND11_MODULE(_keldy_host, mod) {
     bind_map_methods<warper_flow_t>(c);
   }
   {
+    py::class_<warper_projection_t> c(wm, "WarperProjectionT");
+    c.def(py::init([](integrator_direct &computer, int order, int num_bins, int nr_samples, double sigma,
+                      bool optimize_sigma) {
+            if (order != computer.get_dimension()) throw keldy_error("order differs from the integrator's dimension");
+            return computer.make_warper_projection_device(num_bins, nr_samples, sigma, optimize_sigma, false);
+          }),
+          py::arg("warped_integrand"), py::arg("order"), py::arg("num_bins"), py::arg("nr_samples"), py::arg("sigma"),
+          py::arg("optimize_sigma") = true,
+          "The pilot on the device: the integrator's warped integrand (current chain, without the positive-time "
+          "restriction) at the first nr_samples unshifted Sobol points.")
+        .def(py::init<std::function<dcomplex(vec)>, int, int, int, double, bool>(), py::arg("warped_integrand"),
+             py::arg("order"), py::arg("num_bins"), py::arg("nr_samples"), py::arg("sigma"),
+             py::arg("optimize_sigma") = true)
+        .def_static(
+            "from_samples",
+            [](py::array_t<double, py::array::c_style | py::array::forcecast> l, std::vector<double> const &vals,
+               int order, int num_bins, double sigma, bool optimize_sigma) {
+              if (l.ndim() != 2 || l.shape(1) != order) throw keldy_error("l must be [n][order]");
+              std::vector<vec> pts(size_t(l.shape(0)));
+              for (py::ssize_t i = 0; i < l.shape(0); i++) pts[size_t(i)].assign(l.data(i, 0), l.data(i, 0) + order);
+              return warper_projection_t::from_samples(pts, vals, order, num_bins, sigma, optimize_sigma);
+            },
+            py::arg("l"), py::arg("abs_values"), py::arg("order"), py::arg("num_bins"), py::arg("sigma"),
+            py::arg("optimize_sigma") = true)
+        .def("get_xi", &warper_projection_t::get_xi, py::arg("axis"))
+        .def("get_xi_counts", &warper_projection_t::get_xi_counts, py::arg("axis"))
+        .def("get_xi_bin_coord", &warper_projection_t::get_xi_bin_coord, py::arg("axis"))
+        .def("get_sigmas", &warper_projection_t::get_sigmas)
+        .def("get_fi", &warper_projection_t::get_fi, py::arg("axis"))
+        .def("get_fi_integrated", &warper_projection_t::get_fi_integrated, py::arg("axis"))
+        .def("get_fi_integrated_inverse", &warper_projection_t::get_fi_integrated_inverse, py::arg("axis"))
+        .def("order", &warper_projection_t::order)
+        .def("num_bins", &warper_projection_t::num_bins);
+    bind_map_methods<warper_projection_t>(c);
+  }
+  {
     py::class_<warper_train_t> c(wm, "WarperTrainT");
     c.def(py::init<>())
         .def("emplace_back", &warper_train_t::emplace_back<warper_flow_t>, py::arg("w"))
@@ -319,6 +355,7 @@ PYBIND11_MODULE(_keldy_host, mod) {
         .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_t>, py::arg("w"))
         .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_interp_nearest_t>, py::arg("w"))
         .def("emplace_back", &warper_train_t::emplace_back<warper_product_1d_interp_hybrid_t>, py::arg("w"))
+        .def("emplace_back", &warper_train_t::emplace_back<warper_projection_t>, py::arg("w"))
         .def("reserve", &warper_train_t::reserve)
         .def("clear", &warper_train_t::clear)
         .def("size", &warper_train_t::size)

@@ -689,7 +689,13 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
   }
   for (int i = 0; i < n_maps; i++) {
     const kd_map1d &m = maps[i];
-    if (m.kind < KD_MAP_UNIFORM || m.kind > KD_MAP_HYBRID) return fail(c, KD_ERR_ARG, "unknown map kind");
+    if (m.kind < KD_MAP_UNIFORM || m.kind > KD_MAP_PROJECTION) return fail(c, KD_ERR_ARG, "unknown map kind");
+    if (m.kind == KD_MAP_PROJECTION) {
+      auto in = [&](int64_t off, int64_t len) { return off >= 0 && off + len <= n_tables; };
+      if (m.n_pts < 2 || m.n_inv < 2) return fail(c, KD_ERR_ARG, "bad mesh sizes");
+      if (!in(m.t_off, m.n_inv) || !in(m.f_off, m.n_pts)) return fail(c, KD_ERR_ARG, "map table out of bounds");
+      if (!(m.norm > 0.0)) return fail(c, KD_ERR_ARG, "projection map normalisation must be positive");
+    }
     if (m.kind == KD_MAP_NEAREST || m.kind == KD_MAP_HYBRID) {
       if (m.n_pts < 2 || m.n_inv < 2 || m.n_inv > m.n_pts) return fail(c, KD_ERR_ARG, "bad knot counts");
       auto in = [&](int64_t off, int64_t len) { return off >= 0 && off + len <= n_tables; };

@@ -305,6 +305,27 @@ integrator_direct::make_warper_from_proj_device(int num_bins, long nr_samples, d
   return warpers::make_warper_from_proj_histograms(std::move(data), count, dimension, num_bins, sigma);
 }
 
+warpers::warper_projection_t integrator_direct::make_warper_projection_device(int num_bins, long nr_samples,
+                                                                             double sigma, bool optimize_sigma,
+                                                                             bool keep_u_hypercube) {
+  if (nr_samples < 0) throw keldy_error("nr_samples must be >= 0");
+  sobol g(dimension, 0);
+  std::vector<std::vector<double>> l(static_cast<size_t>(nr_samples));
+  std::vector<double> flat(static_cast<size_t>(nr_samples) * static_cast<size_t>(dimension));
+  for (long i = 0; i < nr_samples; i++) {
+    l[size_t(i)] = g();
+    std::copy(l[size_t(i)].begin(), l[size_t(i)].end(), flat.begin() + i * dimension);
+  }
+  std::vector<dcomplex> v(static_cast<size_t>(nr_samples));
+  std::vector<int> dom(static_cast<size_t>(nr_samples));
+  std::vector<double> jac(static_cast<size_t>(nr_samples));
+  if (nr_samples > 0) evaluate_warped_integrand_batch(flat.data(), nr_samples, v.data(), dom.data(), jac.data(),
+                                                      keep_u_hypercube);
+  std::vector<double> a(static_cast<size_t>(nr_samples));
+  for (long i = 0; i < nr_samples; i++) a[size_t(i)] = std::abs(v[size_t(i)]);
+  return warpers::warper_projection_t::from_samples(l, a, dimension, num_bins, sigma, optimize_sigma);
+}
+
 std::vector<double> integrator_direct::autotune(long sample_points) {
   prepare_device_run();
   int32_t lanes = 0, wpc = 0;

@@ -197,6 +197,11 @@ class integrator_direct {
   /// make_warper_from_proj with the pilot on the device: the warped integrand
   /// (current chain) at the first nr_samples unshifted Sobol points and the
   /// per-axis histograms are computed there; the host smooths and fits.
+  /// warper_projection_t with the pilot on the device: |warped integrand|
+  /// (current chain) at the first nr_samples unshifted Sobol points, the
+  /// construction on the host
+  warpers::warper_projection_t make_warper_projection_device(int num_bins, long nr_samples, double sigma,
+                                                             bool optimize_sigma, bool keep_u_hypercube);
   warpers::warper_product_1d_interp_nearest_t make_warper_from_proj_device(int num_bins, long nr_samples, double sigma,
                                                                           bool keep_u_hypercube);
   /// device state before / after a run launched elsewhere (run_batch)

@@ -602,6 +602,399 @@ warper_product_1d_interp_nearest_t make_warper_from_proj(std::function<dcomplex(
 }
 
 // --------------------------------------------------------------------------
+// projection warper (projection.hpp:53-382)
+namespace proj_details {
+
+// Akima (1970) spline, GSL's non-periodic variant: the interval slopes m_i,
+// extended by m_-1 = 2 m_0 - m_1, m_-2 = 3 m_0 - 2 m_1 and the mirror at the
+// far end; knot slope t_i = ((1 - a) m_{i-1} + a m_i) with a = |m_{i-1} -
+// m_{i-2}| / (|m_{i+1} - m_i| + |m_{i-1} - m_{i-2}|), m_i where both jumps
+// vanish; y = y_i + dx (b_i + dx (c_i + dx d_i)).
+void akima_coeffs(vec const &x, vec const &y, vec &b, vec &c, vec &d) {
+  const size_t n = x.size();
+  if (n < 3 || y.size() != n) throw keldy_error("Akima interpolation needs at least 3 points");
+  vec mm(n + 3); // mm[k + 2] = slope of interval k, k = -2 .. n
+  for (size_t i = 0; i + 1 < n; i++) mm[i + 2] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]);
+  mm[0] = 3.0 * mm[2] - 2.0 * mm[3];
+  mm[1] = 2.0 * mm[2] - mm[3];
+  mm[n + 1] = 2.0 * mm[n] - mm[n - 1];
+  mm[n + 2] = 3.0 * mm[n] - 2.0 * mm[n - 1];
+  b.assign(n - 1, 0.0);
+  c.assign(n - 1, 0.0);
+  d.assign(n - 1, 0.0);
+  for (size_t i = 0; i + 1 < n; i++) {
+    const double *m = mm.data() + i + 2; // m[k] = slope of interval i + k
+    const double ne = std::abs(m[1] - m[0]) + std::abs(m[-1] - m[-2]);
+    if (ne == 0.0) {
+      b[i] = m[0];
+      continue;
+    }
+    const double h = x[i + 1] - x[i];
+    const double ne_next = std::abs(m[2] - m[1]) + std::abs(m[0] - m[-1]);
+    const double a_i = std::abs(m[-1] - m[-2]) / ne;
+    double t_next;
+    if (ne_next == 0.0) {
+      t_next = m[0];
+    } else {
+      const double a_n = std::abs(m[0] - m[-1]) / ne_next;
+      t_next = (1.0 - a_n) * m[0] + a_n * m[1];
+    }
+    b[i] = (1.0 - a_i) * m[-1] + a_i * m[0];
+    c[i] = (3.0 * m[0] - 2.0 * b[i] - t_next) / h;
+    d[i] = (b[i] + t_next - 2.0 * m[0]) / (h * h);
+  }
+}
+
+double akima_eval(vec const &x, vec const &y, vec const &b, vec const &c, vec const &d, double xv) {
+  check_domain(x.data(), x.size(), xv);
+  const size_t i = knot_find(x.data(), x.size(), xv);
+  const double dx = xv - x[i];
+  return y[i] + dx * (b[i] + dx * (c[i] + dx * d[i]));
+}
+
+namespace {
+
+// local_linear_reg (gsl_fit_linear_wrap.hpp:66-113) of y at x = 0..N-1 with the
+// kernel (odd size) centred on each point; dead points weighted zero
+vec local_linear(vec const &y, vec const &kernel, std::vector<long> const &dead) {
+  const long N = long(y.size()), K = long(kernel.size()), H = K / 2;
+  vec x(N), out(N), kc;
+  for (long i = 0; i < N; i++) x[i] = double(i);
+  for (long i = 0; i < N; i++) {
+    const double *kp = kernel.data();
+    if (!dead.empty()) {
+      kc = kernel;
+      for (long j : dead) {
+        const long dp = j + H - i;
+        if (dp >= 0 && dp < K) kc[dp] = 0;
+      }
+      kp = kc.data();
+    }
+    const long start_ker = std::max(0L, H - i);
+    const long end_ker = std::min(N + H - i - 1, K - 1);
+    const long start_x = start_ker - H + i;
+    auto cf = fit_wlinear(x.data() + start_x, kp + start_ker, y.data() + start_x, size_t(end_ker - start_ker + 1));
+    out[i] = cf.first + x[i] * cf.second;
+  }
+  return out;
+}
+
+// Brent's minimisation (golden section with parabolic steps) of f on
+// [lo, hi] from the guess x0, which must lie below both ends; stops when the
+// bracket is narrower than abstol or after max_iter steps.
+template <typename F> double brent_minimize(F const &f, double lo, double x0, double hi, double abstol, int max_iter) {
+  const double golden = 0.3819660112501051; // (3 - sqrt 5) / 2
+  const double f_lo = f(lo), f_x0 = f(x0), f_hi = f(hi);
+  if (!std::isfinite(f_lo) || !std::isfinite(f_x0) || !std::isfinite(f_hi))
+    throw keldy_error("Function returned Nan or Inf at the minimisation bracket");
+  if (!(f_x0 < f_lo && f_x0 < f_hi))
+    throw keldy_error("Endpoints do not enclose a minimum. Make sure f(x_lower) > f(x_guess) < f(x_upper)");
+  double z = x0, fz = f_x0;
+  double v = lo + golden * (hi - lo), fv = f(v), w = v, fw = fv;
+  double d = 0.0, e = 0.0;
+  for (int it = 0; it < max_iter; it++) {
+    const double w_lower = z - lo, w_upper = hi - z;
+    const double tol = 1.4901161193847656e-08 * std::abs(z); // sqrt(DBL_EPSILON) |z|
+    const double midpoint = 0.5 * (lo + hi);
+    double p = 0.0, q = 0.0, r = 0.0;
+    if (std::abs(e) > tol) { // parabola through (z, fz), (w, fw), (v, fv)
+      r = (z - w) * (fz - fv);
+      q = (z - v) * (fz - fw);
+      p = (z - v) * q - (z - w) * r;
+      q = 2.0 * (q - r);
+      if (q > 0.0)
+        p = -p;
+      else
+        q = -q;
+      r = e;
+      e = d;
+    }
+    double u;
+    if (std::abs(p) < std::abs(0.5 * q * r) && p < q * w_lower && p < q * w_upper) {
+      const double t2 = 2.0 * tol;
+      d = p / q;
+      u = z + d;
+      if ((u - lo) < t2 || (hi - u) < t2) d = (z < midpoint) ? tol : -tol;
+    } else {
+      e = (z < midpoint) ? hi - z : -(z - lo);
+      d = golden * e;
+    }
+    u = std::abs(d) >= tol ? z + d : z + ((d > 0) ? tol : -tol);
+    const double fu = f(u);
+    if (!std::isfinite(fu)) throw keldy_error("Function returned Nan or Inf.");
+    if (fu <= fz) {
+      if (u < z)
+        hi = z;
+      else
+        lo = z;
+      v = w;
+      fv = fw;
+      w = z;
+      fw = fz;
+      z = u;
+      fz = fu;
+    } else {
+      if (u < z)
+        lo = u;
+      else
+        hi = u;
+      if (fu <= fw || w == z) {
+        v = w;
+        fv = fw;
+        w = u;
+        fw = fu;
+      } else if (fu <= fv || v == z || v == w) {
+        v = u;
+        fv = fu;
+      }
+    }
+    if (std::abs(hi - lo) < abstol) break;
+  }
+  return z;
+}
+
+double gaussian(double x, double x0, double sigma) { return std::exp(-std::pow((x - x0) / sigma, 2) / 2); }
+
+} // namespace
+
+double optimize_sigma(vec const &y, double sigma, double upper) {
+  const long N = long(y.size());
+  const double sigma_min = 0.3; // narrow gaussians make the local fits unstable
+  vec log_y(N);
+  std::vector<long> dead;
+  for (long i = 0; i < N; i++) {
+    if (y[i] <= 0) {
+      dead.push_back(i);
+      log_y[i] = 0;
+    } else {
+      log_y[i] = std::log(y[i]);
+    }
+  }
+  const long H = std::max(std::min(long(3 * upper), N), 1L);
+  vec kernel(2 * H + 1);
+  // the leave-one-out error of the smoothing (the centre weight is zero),
+  // symmetric in sigma: searched on [-upper, upper] so that a noiseless
+  // projection still has a bracketed minimum
+  auto cost = [&](double s) {
+    s = std::max(std::abs(s), sigma_min);
+    for (long j = 0; j < long(kernel.size()); j++) kernel[j] = gaussian(double(j), double(H), s);
+    kernel[H] = 0;
+    const double k1 = kernel[H + 1];
+    for (double &kv : kernel) kv /= k1;
+    const vec out = local_linear(log_y, kernel, dead);
+    double r = 0.0;
+    for (long i = 0; i < N; i++) r += std::pow(std::exp(out[i]) - y[i], 2);
+    if (std::isnan(r))
+      throw keldy_error("Smoothing procedure returned a Nan. This may be due to a lack of data, try to increase the "
+                        "number of samples.");
+    return r;
+  };
+  return std::max(std::abs(brent_minimize(cost, -upper, sigma, upper, 1e-8, 20)), sigma_min);
+}
+
+} // namespace proj_details
+
+namespace {
+// TRIQS gf<retime> evaluation on the uniform mesh of [0, 1] with n points:
+// linear interpolation between the neighbouring mesh points
+double mesh_eval(vec const &v, double t) {
+  const size_t n = v.size();
+  if (!(t >= -1e-12 && t <= 1.0 + 1e-12)) throw keldy_error("point outside the [0, 1] mesh of the projection warper");
+  const double delta = 1.0 / double(n - 1);
+  const double a = t / delta;
+  long i = long(std::floor(a));
+  i = std::min<long>(std::max<long>(i, 0), long(n) - 2);
+  const double w = a - double(i);
+  return (1.0 - w) * v[size_t(i)] + w * v[size_t(i) + 1];
+}
+} // namespace
+
+warper_projection_t::warper_projection_t(std::function<dcomplex(vec)> const &warped_integrand, int order, int num_bins,
+                                         int nr_samples, double sigma, bool optimize_sigma) {
+  if (order < 1) throw keldy_error("order must be >= 1");
+  if (nr_samples < 0) throw keldy_error("nr_samples must be >= 0");
+  sobol g(order, 0);
+  std::vector<vec> l;
+  vec vals;
+  l.reserve(size_t(nr_samples));
+  vals.reserve(size_t(nr_samples));
+  for (int i = 0; i < nr_samples; i++) {
+    l.push_back(g());
+    vals.push_back(std::abs(warped_integrand(l.back())));
+  }
+  order_ = order;
+  num_bins_ = num_bins;
+  build(l, vals, sigma, optimize_sigma);
+}
+
+warper_projection_t warper_projection_t::from_samples(std::vector<vec> const &l, vec const &abs_values, int order,
+                                                      int num_bins, double sigma, bool optimize_sigma) {
+  if (order < 1) throw keldy_error("order must be >= 1");
+  if (l.size() != abs_values.size()) throw keldy_error("sample and value counts differ");
+  warper_projection_t w;
+  w.order_ = order;
+  w.num_bins_ = num_bins;
+  w.build(l, abs_values, sigma, optimize_sigma);
+  return w;
+}
+
+void warper_projection_t::build(std::vector<vec> const &l, vec const &vals, double sigma, bool optimize_sigma) {
+  const int nb = num_bins_;
+  if (nb < 3) throw keldy_error("num_bins must be >= 3");
+  const double delta = 1.0 / (nb - 1.0);
+  // gather_data (projection.hpp:189-222): binner_t<1, 0, double> on [-d/2, 1 + d/2]
+  const double xmin = -delta / 2, xmax = 1 + delta / 2;
+  bin_size_ = (xmax - xmin) / nb;
+  xi_.assign(size_t(order_), vec(size_t(nb), 0.0));
+  xi_count_.assign(size_t(order_), std::vector<long>(size_t(nb), 0));
+  for (size_t s = 0; s < l.size(); s++) {
+    if (int(l[s].size()) != order_) throw keldy_error("sample of wrong dimension");
+    for (int a = 0; a < order_; a++) {
+      const double x = l[s][size_t(a)];
+      if (!(xmin <= x && x <= xmax)) continue;
+      const long b = x == xmax ? nb - 1 : long((x - xmin) / bin_size_);
+      xi_[size_t(a)][size_t(b)] += vals[s];
+      xi_count_[size_t(a)][size_t(b)]++;
+    }
+  }
+  for (int a = 0; a < order_; a++)
+    for (int b = 0; b < nb; b++)
+      if (xi_count_[size_t(a)][size_t(b)] > 0) xi_[size_t(a)][size_t(b)] /= double(xi_count_[size_t(a)][size_t(b)]);
+  // populate_sigmas (projection.hpp:247-274)
+  sigmas_.assign(size_t(order_), sigma / bin_size_);
+  if (optimize_sigma) {
+    if (bin_size_ >= 2) throw keldy_error("delta too large");
+    for (int a = 0; a < order_; a++)
+      sigmas_[size_t(a)] = proj_details::optimize_sigma(xi_[size_t(a)], sigmas_[size_t(a)], 1.0 / bin_size_);
+  }
+  // populate_functions (projection.hpp:276-317)
+  const int n_inv = 1 + 5 * (nb - 1);
+  vec mesh(static_cast<size_t>(nb)), times(static_cast<size_t>(nb));
+  for (int k = 0; k < nb; k++) mesh[size_t(k)] = k * delta;
+  for (int k = 0; k < nb; k++) times[size_t(k)] = xmin + (k + 0.5) * bin_size_;
+  times[0] = 0;      // snap boundaries
+  times[nb - 1] = 1; // snap boundaries
+  fn_.assign(size_t(order_), vec());
+  F_.assign(size_t(order_), vec());
+  Finv_.assign(size_t(order_), vec());
+  norm_.assign(size_t(order_), 0.0);
+  for (int a = 0; a < order_; a++) {
+    const vec sm = proj_details::kernel_smoothing(xi_[size_t(a)], sigmas_[size_t(a)]);
+    vec &fn = fn_[size_t(a)];
+    fn.resize(size_t(nb));
+    for (int k = 0; k < nb; k++) fn[size_t(k)] = linear_eval(times.data(), sm.data(), size_t(nb), mesh[size_t(k)]);
+    // midpoint rule on the mesh: F_k = sum_{j <= k} fn(t_j - d/2) d, F_0 = 0
+    vec &F = F_[size_t(a)];
+    F.assign(size_t(nb), 0.0);
+    for (int k = 1; k < nb; k++) F[size_t(k)] = mesh_eval(fn, mesh[size_t(k)] - delta / 2) * delta;
+    for (int k = 1; k < nb; k++) F[size_t(k)] += F[size_t(k) - 1];
+    norm_[size_t(a)] = F[size_t(nb) - 1];
+    for (double &x : F) x /= norm_[size_t(a)];
+    // the inverse CDF by Akima interpolation through (F, mesh), on 1 + 5(nb - 1) points
+    vec b, c, d;
+    proj_details::akima_coeffs(F, mesh, b, c, d);
+    vec &Fi = Finv_[size_t(a)];
+    Fi.resize(size_t(n_inv));
+    const double dl = 1.0 / double(n_inv - 1);
+    for (int j = 0; j < n_inv; j++) Fi[size_t(j)] = proj_details::akima_eval(F, mesh, b, c, d, j * dl);
+  }
+}
+
+void warper_projection_t::check_axis(int axis) const {
+  if (axis < 0 || axis >= order_) throw keldy_error("this axis does not exist.");
+}
+
+vec warper_projection_t::get_xi(int axis) const {
+  check_axis(axis);
+  return xi_[size_t(axis)];
+}
+
+std::vector<long> warper_projection_t::get_xi_counts(int axis) const {
+  check_axis(axis);
+  return xi_count_[size_t(axis)];
+}
+
+vec warper_projection_t::get_xi_bin_coord(int axis) const {
+  check_axis(axis);
+  vec t(static_cast<size_t>(num_bins_));
+  const double xmin = -0.5 / (num_bins_ - 1.0);
+  for (int k = 0; k < num_bins_; k++) t[size_t(k)] = xmin + (k + 0.5) * bin_size_;
+  return t;
+}
+
+vec warper_projection_t::get_sigmas() const {
+  vec out(sigmas_.size());
+  for (size_t i = 0; i < sigmas_.size(); i++) out[i] = sigmas_[i] * bin_size_;
+  return out;
+}
+
+vec warper_projection_t::get_fi(int axis) const {
+  check_axis(axis);
+  return fn_[size_t(axis)];
+}
+
+vec warper_projection_t::get_fi_integrated(int axis) const {
+  check_axis(axis);
+  return F_[size_t(axis)];
+}
+
+vec warper_projection_t::get_fi_integrated_inverse(int axis) const {
+  check_axis(axis);
+  return Finv_[size_t(axis)];
+}
+
+vec warper_projection_t::ui_from_li(vec const &l) const {
+  if (int(l.size()) > order_) throw keldy_error("point of higher dimension than the projection warper");
+  vec u(l.size());
+  for (size_t i = 0; i < l.size(); i++) u[i] = mesh_eval(Finv_[i], l[i]);
+  return u;
+}
+
+vec warper_projection_t::li_from_ui(vec const &u) const {
+  if (int(u.size()) > order_) throw keldy_error("point of higher dimension than the projection warper");
+  vec l(u.size());
+  for (size_t i = 0; i < u.size(); i++) l[i] = mesh_eval(F_[i], u[i]);
+  return l;
+}
+
+double warper_projection_t::jacobian_reverse(vec const &l) const {
+  if (int(l.size()) > order_) throw keldy_error("point of higher dimension than the projection warper");
+  double j = 1.0;
+  for (size_t i = 0; i < l.size(); i++) j *= norm_[i] / mesh_eval(fn_[i], mesh_eval(Finv_[i], l[i]));
+  return j;
+}
+
+double warper_projection_t::jacobian_forward(vec const &u) const {
+  if (int(u.size()) > order_) throw keldy_error("point of higher dimension than the projection warper");
+  double j = 1.0;
+  for (size_t i = 0; i < u.size(); i++) j *= mesh_eval(fn_[i], u[i]);
+  return j;
+}
+
+void warper_projection_t::to_device(device_chain &c) const {
+  if (order_ < 1) throw keldy_error("empty projection warper");
+  kd_warper_desc d{};
+  d.kind = KD_WARP_PRODUCT;
+  d.n_maps = int32_t(order_);
+  d.map_offset = int64_t(c.maps.size());
+  for (int a = 0; a < order_; a++) {
+    kd_map1d m{};
+    m.kind = KD_MAP_PROJECTION;
+    m.u_max = 1.0;
+    m.w = 1.0;
+    m.norm = norm_[size_t(a)];
+    m.n_pts = int32_t(fn_[size_t(a)].size());
+    m.n_inv = int32_t(Finv_[size_t(a)].size());
+    m.t_off = int64_t(c.tables.size());
+    c.tables.insert(c.tables.end(), Finv_[size_t(a)].begin(), Finv_[size_t(a)].end());
+    m.f_off = int64_t(c.tables.size());
+    c.tables.insert(c.tables.end(), fn_[size_t(a)].begin(), fn_[size_t(a)].end());
+    c.maps.push_back(m);
+  }
+  c.warpers.push_back(d);
+}
+
+// --------------------------------------------------------------------------
 map_pair warper_train_t::map_reverse(vec const &l, int start, int end) const {
   if (start < 0 || end < 0 || start > int(size()) || end > int(size()) || end > start)
     throw keldy_error("invalid warper domain range");

@@ -248,6 +248,59 @@ vec kernel_smoothing(vec const &y_in, double sigma);
 std::pair<double, double> fit_wlinear(const double *x, const double *w, const double *y, size_t n);
 } // namespace proj_details
 
+/// Projection warper (projection.hpp:176-382): per axis, the mean |f| of the
+/// pilot samples in num_bins bins of [-d/2, 1+d/2] (d = 1/(num_bins-1)),
+/// smoothed by local-linear regression in log space (width sigma, optionally
+/// chosen by Brent minimisation of the leave-one-out error); fn on the uniform
+/// mesh of [0, 1] (num_bins points), its midpoint-rule CDF normalised to 1, and
+/// the CDF's inverse by Akima interpolation tabulated on 1 + 5(num_bins-1)
+/// points.  The maps evaluate these tables by linear interpolation on their
+/// meshes: u = Finv(l), J = norm / fn(u) per axis.
+class warper_projection_t {
+ public:
+  warper_projection_t() = default;
+  /// pilot: |warped_integrand| at the first nr_samples unshifted Sobol points
+  warper_projection_t(std::function<dcomplex(vec)> const &warped_integrand, int order, int num_bins, int nr_samples,
+                      double sigma, bool optimize_sigma = true);
+  /// the same construction from given pilot points l [n][order] and |f| at them
+  static warper_projection_t from_samples(std::vector<vec> const &l, vec const &abs_values, int order, int num_bins,
+                                          double sigma, bool optimize_sigma = true);
+  vec get_xi(int axis) const;                      // per-bin mean |f| of the pilot
+  std::vector<long> get_xi_counts(int axis) const; // pilot samples per bin
+  vec get_xi_bin_coord(int axis) const;            // bin centres
+  vec get_sigmas() const;                          // smoothing widths per axis (natural units)
+  vec get_fi(int axis) const;                      // fn on the mesh of [0, 1]
+  vec get_fi_integrated(int axis) const;           // normalised CDF on the same mesh
+  vec get_fi_integrated_inverse(int axis) const;   // its inverse on the finer mesh
+  vec ui_from_li(vec const &l) const;
+  vec li_from_ui(vec const &u) const;
+  double jacobian_reverse(vec const &l) const;
+  double jacobian_forward(vec const &u) const;
+  map_pair map_reverse(vec const &l) const { return {ui_from_li(l), jacobian_reverse(l)}; }
+  map_pair map_forward(vec const &u) const { return {li_from_ui(u), jacobian_forward(u)}; }
+  void to_device(device_chain &c) const;
+  int order() const { return order_; }
+  int num_bins() const { return num_bins_; }
+
+ private:
+  void build(std::vector<vec> const &l, vec const &abs_values, double sigma, bool optimize_sigma);
+  void check_axis(int axis) const;
+  int order_ = 0, num_bins_ = 0;
+  double bin_size_ = 0.0;
+  std::vector<vec> xi_, fn_, F_, Finv_;
+  std::vector<std::vector<long>> xi_count_;
+  vec norm_, sigmas_; // sigmas_ in bin units
+};
+
+namespace proj_details {
+/// the leave-one-out smoothing width of projection.hpp:80-121 (bin units):
+/// Brent minimisation on [-upper, upper] from sigma, |result| >= 0.3
+double optimize_sigma(vec const &y, double sigma, double upper);
+/// Akima spline through (x, y) (GSL's non-periodic form): per interval b, c, d
+void akima_coeffs(vec const &x, vec const &y, vec &b, vec &c, vec &d);
+double akima_eval(vec const &x, vec const &y, vec const &b, vec const &c, vec const &d, double xv);
+} // namespace proj_details
+
 inline uint64_t next_stamp() {
   static std::atomic<uint64_t> counter{1};
   return counter.fetch_add(1);
@@ -286,7 +339,8 @@ class warper_flow_t {
 using warper_variant =
     std::variant<warper_identity_t, warper_plasma_uv_t, warper_product_1d_simple_interp_nearest_t,
                  warper_product_1d_simple_t, warper_product_1d_t, warper_product_1d_interp_nearest_t,
-                 warper_product_1d_interp_hybrid_t, warper_product_1d_simple_interp_hybrid_t, warper_flow_t>;
+                 warper_product_1d_interp_hybrid_t, warper_product_1d_simple_interp_hybrid_t, warper_projection_t,
+                 warper_flow_t>;
 
 /// Ordered chain of warpers; map_reverse applies them last-added first
 /// (warpers.hpp:41-137).

@@ -329,8 +329,22 @@ __device__ __forceinline__ int bsearch_knots(const double *xa, int n, double x) 
   return ilo;
 }
 
+// v [n] on the uniform mesh of [0, 1], at t: linear interpolation between the
+// neighbouring mesh points (index clamped to the first and last intervals, so
+// any t, NaN included, reads inside v)
+__device__ __forceinline__ double mesh_lin(const double *__restrict__ v, int n, double t) {
+  const double delta = 1.0 / (double)(n - 1);
+  const double a = t / delta;
+  int i = (int)floor(a);
+  i = i > n - 2 ? n - 2 : i;
+  i = i < 0 ? 0 : i;
+  const double w = a - (double)i;
+  return (1.0 - w) * v[i] + w * v[i + 1];
+}
+
 // x = F^{-1}(l), returns 1/f(x)   (product_1d_simple.hpp:109-117 and makers 560-630,
 // interp nearest 198-217, interp hybrid 403-415)
+
 __device__ inline double map1d_reverse(const kd_map1d &m, const double *__restrict__ tab, double l, double *u) {
   double x, f;
   const double w = m.w, um = m.u_max;
@@ -381,6 +395,14 @@ __device__ inline double map1d_reverse(const kd_map1d &m, const double *__restri
     for (int q = 1; q < m.power; q++) num *= w;
     for (int q = 1; q < m.power; q++) dd *= den;
     f = (num / dd) / m.norm;
+    break;
+  }
+  case KD_MAP_PROJECTION: {
+    // tables on the uniform meshes of [0, 1], evaluated by linear interpolation
+    // (the TRIQS gf<retime> evaluation of projection.hpp:331-357)
+    x = mesh_lin(tab + m.t_off, m.n_inv, l);
+    f = mesh_lin(tab + m.f_off, m.n_pts, x) / m.norm;
+    if (!(l >= 0.0 && l <= 1.0)) x = __builtin_nan("");
     break;
   }
   default:

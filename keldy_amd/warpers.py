@@ -15,6 +15,7 @@ WarperProduct1dInterpNearestT = _w.WarperProduct1dInterpNearestT
 WarperProduct1dInterpHybridT = _w.WarperProduct1dInterpHybridT
 WarperTrainT = _w.WarperTrainT
 WarperFlowT = _w.WarperFlowT
+WarperProjectionT = _w.WarperProjectionT
 vi_from_ui = _w.vi_from_ui
 ui_from_vi = _w.ui_from_vi
 make_product_1d_simple_exponential = _w.make_product_1d_simple_exponential

@@ -950,9 +950,9 @@ static double split_like_flops(int N, int K, bool paired) {
     for (int t = 0; t < 2; t++) {
       for (int k = 0; k < M - 1; k++) macs += (M - 2 - k) + 2;
       macs += 1;
-      invs += 1;
-      muls += 4 + 1; // det, u, d^2, value; the path determinant
+      muls += 3; // p rb_B, p x, (dp det)^2 times it
     }
+    muls += 2; // (dp det)^2, shared by the two rows
     return f + std::ldexp(8 * macs + 6 * muls + 7 * invs + 4, N - 1);
   }
   const double m = M;

@@ -150,7 +150,7 @@ struct TabPos {
 };
 
 __device__ __forceinline__ TabPos table_pos(const KdParams &p, double t) {
-  const double a = (t - p.t_min) / p.dt;
+  const double a = (t - p.t_min) / p.dt; // a quotient, so that mesh points land on their index exactly
   long i = (long)floor(a);
   TabPos q;
   q.out = !(a >= -1e-9) || !(a <= (double)(p.n_pts - 1) + 1e-9);
@@ -219,21 +219,25 @@ template <int N> __device__ __forceinline__ void wick_item_vertices(int q, int &
   ib = e < 2 * N ? e >> 1 : -1;
 }
 
+// Where item q's table entries are: w gets the mesh position and the order
+// flag; the entries are a[0], a[1] (pair items: lesser; external items: the
+// table compare_3way picks) and, for pair items, b[0], b[1] (greater).  The
+// corner item (no entries) returns a = b = nullptr.
+struct WickLoc {
+  const double2 *a, *b;
+};
+
 template <int N>
-__device__ __forceinline__ void wick_fetch(const KdParams &p, int q, double ta, double tb, WickPre &w) {
+__device__ __forceinline__ WickLoc wick_locate(const KdParams &p, int q, double ta, double tb, WickPre &w) {
   if (q < N * N) {
     const int i = q / N, j = q - i * N;
     const double d = ta - tb;
     w.pos = table_pos(p, d);
-    w.l0 = p.lesser[w.pos.i];
-    w.l1 = p.lesser[w.pos.i + 1];
-    w.g0 = p.greater[w.pos.i];
-    w.g1 = p.greater[w.pos.i + 1];
     w.o = d > 0.0 ? 1 : d < 0.0 ? -1 : (i > j) - (i < j);
-    return;
+    return WickLoc{p.lesser + w.pos.i, p.greater + w.pos.i};
   }
   const int e = q - N * N;
-  if (e >= 4 * N) return;
+  if (e >= 4 * N) return WickLoc{nullptr, nullptr};
   CPt a, b;
   if (e < 2 * N) {
     a = CPt{ta, p.A.k_idx, N, p.A.orbital};
@@ -247,9 +251,22 @@ __device__ __forceinline__ void wick_fetch(const KdParams &p, int q, double ta, 
   const int ord = compare_3way(a, b);
   const double2 *g = (ord > 0 ? p.greater : p.lesser) + (size_t)(a.orb * p.n_orb + b.orb) * p.n_pts;
   w.pos = table_pos(p, ord == 0 ? 0.0 : a.time - b.time);
-  w.l0 = g[w.pos.i];
-  w.l1 = g[w.pos.i + 1];
   w.o = ord == 0 && a.orb == b.orb;
+  return WickLoc{g + w.pos.i, nullptr};
+}
+
+template <int N>
+__device__ __forceinline__ void wick_fetch(const KdParams &p, int q, double ta, double tb, WickPre &w) {
+  const WickLoc L = wick_locate<N>(p, q, ta, tb, w);
+  if (q < N * N) {
+    w.l0 = L.a[0];
+    w.l1 = L.a[1];
+    w.g0 = L.b[0];
+    w.g1 = L.b[1];
+  } else if (q - N * N < 4 * N) {
+    w.l0 = L.a[0];
+    w.l1 = L.a[1];
+  }
 }
 
 template <int N>

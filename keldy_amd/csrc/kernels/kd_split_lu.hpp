@@ -79,8 +79,8 @@ __device__ __forceinline__ double2 split_config(const PT *__restrict__ Tn, int l
 // node table.  The first M-1 vertex rows are eliminated once over the columns
 // of both choices of the last vertex (and the border), the border row likewise;
 // only the last vertex row and the corner update are per choice -- at M = 5,
-// 70 instead of 110 complex multiply-adds and 6 instead of 10 reciprocals per
-// pair, the same arithmetic per configuration as split_config.
+// 70 instead of 110 complex multiply-adds and 4 instead of 10 reciprocals per
+// pair (the last pivot enters without one).
 // PF: row q+1 is loaded while row q is eliminated (one row of registers more).
 template <int M, typename T, typename PT, bool PF = true>
 __device__ __forceinline__ void split_config_pair(const PT *__restrict__ Tn, int loc, const T *__restrict__ thr,
@@ -116,6 +116,7 @@ __device__ __forceinline__ void split_config_pair(const PT *__restrict__ Tn, int
   cx<T> Uh[M - 1][MC];
   cx<T> rb[MC];
   cx<T> det = cmk<T>(T(1), T(0));
+  double2 D2;
   bool flag = false;
   cx<T> nxt[MC];
   if constexpr (PF) load_row(std::integral_constant<int, 0>{}, nxt);
@@ -154,15 +155,20 @@ __device__ __forceinline__ void split_config_pair(const PT *__restrict__ Tn, int
       // border row over the same columns (its border entry is the node's corner)
       static_for<0, MC>([&](auto J) { rb[J] = row[J]; });
     } else {
+      // det(A)^2 s with the last pivot p and the border's Schur complement
+      // s = rb_B - rb_c row_B / p:  (dp det p)^2 s = (dp det)^2 p (p rb_B - rb_c row_B),
+      // no reciprocal of the last pivot; (dp det)^2 is shared by the two rows
       constexpr int t = q - M, ct = M - 1 + t; // Keldysh index of the last vertex, its column
+      if constexpr (t == 0) {
+        const double2 D =
+            make_double2(dp.x * (double)det.x - dp.y * (double)det.y, dp.x * (double)det.y + dp.y * (double)det.x);
+        D2 = cmul<double>(D, D);
+      }
       const cx<T> piv = row[ct];
       bad[t] = flag | !(cabs1<T>(piv) >= th[M - 1 + t]);
-      const cx<T> dt = cmul<T>(det, piv);
-      const cx<T> u = cmul<T>(row[MC - 1], cinv<T>(piv));
-      const cx<T> sc = cfms<T>(rb[MC - 1], rb[ct], u);
-      const double2 dd =
-          make_double2(dp.x * (double)dt.x - dp.y * (double)dt.y, dp.x * (double)dt.y + dp.y * (double)dt.x);
-      v[t] = cmul<double>(cmul<double>(dd, dd), make_double2((double)sc.x, (double)sc.y));
+      const cx<T> x = cfms<T>(cmul<T>(piv, rb[MC - 1]), rb[ct], row[MC - 1]);
+      const cx<T> y = cmul<T>(piv, x);
+      v[t] = cmul<double>(D2, make_double2((double)y.x, (double)y.y));
     }
     __builtin_amdgcn_sched_barrier(0);
   });

@@ -276,12 +276,31 @@ int kd_flow_loss_grad(kd_ctx *ctx, const double *x, const double *w, int64_t n, 
  * advances) pushed through the warper chain WITHOUT the flow; keeps the points
  * and the weights w = |warped integrand|^weight_power on the device (1: the
  * flow's density targets |integrand|; below 1: a tempered target); returns
- * the mean weight. */
+ * the mean weight (0 when every sample is zero: kd_flow_train then fails). */
 int kd_flow_collect(kd_ctx *ctx, int64_t n_samples, double weight_power, double *w_mean);
 /* Adam steps on consecutive aligned batches of the training set (batch | n_samples,
  * 64 | batch, batch <= 2^16), learning rate cosine-decayed from lr to lr_final; per-step loss
  * into losses (steps doubles, may be NULL). */
 int kd_flow_train(kd_ctx *ctx, int32_t steps, int32_t batch, float lr, float lr_final, double *losses);
+/* The time-conditioned flow (one learned warper for the runs at many external
+ * times, the time an extra input feature; no reference counterpart): with
+ * n_cond = 1 every coupling layer's conditioner also takes one value c,
+ * h1 = relu(W1 a + Wc c + b1) (parameter layout in kd_flow_device.hpp), so one
+ * parameter set holds a family of maps T_c.  Set before kd_flow_init /
+ * kd_flow_set_params (the parameter count depends on n_cond); `value` is the c
+ * the ctx's map and chain use (may be changed at any time). */
+int kd_flow_set_conditioning(kd_ctx *ctx, int32_t n_cond, double value);
+int kd_flow_n_params_cond(int32_t order, int32_t n_layers, int32_t n_cond, int64_t *n);
+/* kd_flow_loss_grad with one conditioning value per point (cond: n doubles) */
+int kd_flow_loss_grad_cond(kd_ctx *ctx, const double *x, const double *w, const double *cond, int64_t n,
+                           double w_mean, float *grad, double *loss);
+/* The training set of the last kd_flow_collect, copied to host memory
+ * (x: n x order, w: n; n = the collected count). */
+int kd_flow_get_samples(kd_ctx *ctx, double *x, double *w, int64_t n);
+/* A training set from host memory, replacing the collected one: points x
+ * (n x order, in [0, 1]), weights w >= 0 (w_mean = their mean), and per point
+ * its conditioning value (cond, n doubles; NULL: the ctx's value). */
+int kd_flow_set_samples(kd_ctx *ctx, const double *x, const double *w, const double *cond, int64_t n);
 
 /* ---- kernel method ---------------------------------------------------------
  * Observables of the integrator besides the Keldysh sum of kd_set_integrand.

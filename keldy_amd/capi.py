@@ -58,7 +58,8 @@ SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_se
            "kd_set_time_bins", "kd_get_time_bins", "kd_set_algorithm", "kd_get_algorithm",
            "kd_pilot_histograms", "kd_set_kernel_integrand", "kd_set_kernel_omega_integrand",
            "kd_set_kernel_bins", "kd_get_kernel_bins", "kd_eval_kernel_batch",
-           "kd_eval_batch_device_noalloc"]
+           "kd_eval_batch_device_noalloc", "kd_flow_set_conditioning", "kd_flow_n_params_cond",
+           "kd_flow_loss_grad_cond", "kd_flow_get_samples", "kd_flow_set_samples"]
 
 _lib = None
 
@@ -103,6 +104,11 @@ def lib():
         L.kd_flow_loss_grad.argtypes = [vp, dp, dp, C.c_int64, C.c_double, fp, dp]
         L.kd_flow_collect.argtypes = [vp, C.c_int64, C.c_double, dp]
         L.kd_flow_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_float, dp]
+        L.kd_flow_set_conditioning.argtypes = [vp, C.c_int32, C.c_double]
+        L.kd_flow_n_params_cond.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int64)]
+        L.kd_flow_loss_grad_cond.argtypes = [vp, dp, dp, dp, C.c_int64, C.c_double, fp, dp]
+        L.kd_flow_get_samples.argtypes = [vp, dp, dp, C.c_int64]
+        L.kd_flow_set_samples.argtypes = [vp, dp, dp, dp, C.c_int64]
         L.kd_autotune.argtypes = [vp, C.c_int64, ip, ip, dp]
         L.kd_set_launch_shape.argtypes = [vp, C.c_int32, C.c_int32]
         L.kd_get_launch_shape.argtypes = [vp, ip, ip]
@@ -309,15 +315,21 @@ class Context:
         return l.value, w.value
 
     # -- learned flow warper ---------------------------------------------
+    flow_nc = 0
+
+    def flow_set_conditioning(self, n_cond, value=0.0):
+        self._check(lib().kd_flow_set_conditioning(self.h, n_cond, float(value)))
+        self.flow_nc = n_cond
+
     def flow_init(self, order, n_layers, seed=0):
         self._check(lib().kd_flow_init(self.h, order, n_layers, seed))
-        self.flow_shape = (order, n_layers)
+        self.flow_shape = (order, n_layers, self.flow_nc)
 
     def flow_set_params(self, order, n_layers, params):
         p = np.ascontiguousarray(params, dtype=np.float32)
         self._check(lib().kd_flow_set_params(self.h, order, n_layers, p.ctypes.data_as(C.POINTER(C.c_float)),
                                              p.size))
-        self.flow_shape = (order, n_layers)
+        self.flow_shape = (order, n_layers, self.flow_nc)
 
     def flow_get_params(self):
         p = np.empty(flow_n_params(*self.flow_shape), dtype=np.float32)
@@ -331,15 +343,34 @@ class Context:
         self._check(lib().kd_flow_map(self.h, _dp(l), l.shape[0], _dp(x), _dp(jac)))
         return x, jac
 
-    def flow_loss_grad(self, x, w, w_mean=None):
+    def flow_loss_grad(self, x, w, w_mean=None, cond=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
         w = np.ascontiguousarray(w, dtype=np.float64)
         g = np.empty(flow_n_params(*self.flow_shape), dtype=np.float32)
         loss = C.c_double()
-        self._check(lib().kd_flow_loss_grad(self.h, _dp(x), _dp(w), x.shape[0],
-                                            float(w.mean() if w_mean is None else w_mean),
-                                            g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
+        cp = None
+        if cond is not None:
+            cond = np.ascontiguousarray(np.broadcast_to(cond, (x.shape[0],)), dtype=np.float64)
+            cp = _dp(cond)
+        self._check(lib().kd_flow_loss_grad_cond(self.h, _dp(x), _dp(w), cp, x.shape[0],
+                                                 float(w.mean() if w_mean is None else w_mean),
+                                                 g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
         return loss.value, g
+
+    def flow_get_samples(self, n, order):
+        x = np.empty((n, order))
+        w = np.empty(n)
+        self._check(lib().kd_flow_get_samples(self.h, _dp(x), _dp(w), n))
+        return x, w
+
+    def flow_set_samples(self, x, w, cond=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        cp = None
+        if cond is not None:
+            cond = np.ascontiguousarray(cond, dtype=np.float64)
+            cp = _dp(cond)
+        self._check(lib().kd_flow_set_samples(self.h, _dp(x), _dp(w), cp, x.shape[0]))
 
     def flow_collect(self, n_samples, weight_power=1.0):
         wm = C.c_double()
@@ -365,9 +396,9 @@ def run_batch(ctxs, n_steps):
     return list(acc), ms.value
 
 
-def flow_n_params(order, n_layers):
+def flow_n_params(order, n_layers, n_cond=0):
     n = C.c_int64()
-    if lib().kd_flow_n_params(order, n_layers, C.byref(n)) != KD_OK:
+    if lib().kd_flow_n_params_cond(order, n_layers, n_cond, C.byref(n)) != KD_OK:
         raise KdError("bad flow shape")
     return n.value
 

@@ -294,12 +294,17 @@ PYBIND11_MODULE(_keldy_host, mod) {
   }
   {
     py::class_<warper_flow_t> c(wm, "WarperFlowT");
-    c.def(py::init([](int order, int n_layers, py::array_t<float, py::array::c_style | py::array::forcecast> p) {
-            return warper_flow_t(order, n_layers, std::vector<float>(p.data(), p.data() + p.size()));
+    c.def(py::init([](int order, int n_layers, py::array_t<float, py::array::c_style | py::array::forcecast> p,
+                      int n_cond, double cond) {
+            return warper_flow_t(order, n_layers, std::vector<float>(p.data(), p.data() + p.size()), n_cond, cond);
           }),
-          py::arg("order"), py::arg("n_layers"), py::arg("params"))
+          py::arg("order"), py::arg("n_layers"), py::arg("params"), py::arg("n_cond") = 0, py::arg("cond") = 0.0)
         .def("order", &warper_flow_t::order)
         .def("n_layers", &warper_flow_t::n_layers)
+        .def("n_cond", &warper_flow_t::n_cond)
+        .def("cond", &warper_flow_t::cond)
+        .def("with_cond", &warper_flow_t::with_cond, py::arg("cond"),
+             "the same (conditioned) flow at another conditioning value")
         .def("params", [](warper_flow_t const &w) {
           py::array_t<float> a(py::ssize_t(w.params().size()));
           std::copy(w.params().begin(), w.params().end(), a.mutable_data());
@@ -693,4 +698,26 @@ PYBIND11_MODULE(_keldy_host, mod) {
       },
       py::arg("integrators"), py::arg("nr_steps"),
       "run several integrators (e.g. orders 1..N) in one hipGraph launch; returns its wall time in ms");
+  im.def(
+      "train_flow_warper_conditioned",
+      [](py::list computers, std::vector<double> conds, int n_layers, long samples_per_computer, int steps, int batch,
+         double lr, double lr_final, int seed_shift, uint64_t init_seed, double weight_power) {
+        std::vector<integrator_direct *> its;
+        for (auto h : computers) its.push_back(h.cast<integrator_direct *>());
+        std::pair<warper_flow_t, std::vector<double>> r;
+        {
+          py::gil_scoped_release rel;
+          r = integrator_direct::train_flow_warper_conditioned(its, conds, n_layers, samples_per_computer, steps,
+                                                               batch, lr, lr_final, seed_shift, init_seed,
+                                                               weight_power);
+        }
+        py::array_t<double> losses(py::ssize_t(r.second.size()));
+        std::copy(r.second.begin(), r.second.end(), losses.mutable_data());
+        return py::make_tuple(r.first, losses);
+      },
+      py::arg("computers"), py::arg("conds"), py::arg("n_layers") = 6, py::arg("samples_per_computer") = 1 << 16,
+      py::arg("steps") = 2000, py::arg("batch") = 1 << 12, py::arg("lr") = 1e-3, py::arg("lr_final") = 1e-5,
+      py::arg("rng_seed_shift") = 1000003, py::arg("init_seed") = 0, py::arg("weight_power") = 1.0,
+      "Train one time-conditioned coupling flow on the samples of several computers (conds[i] tags computer i's "
+      "samples); returns (WarperFlowT at conds[0], per-step loss).  flow.with_cond(conds[i]) is computer i's warper.");
 }

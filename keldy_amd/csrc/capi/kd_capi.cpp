@@ -44,13 +44,15 @@ struct kd_ctx {
   // learned flow warper (KD_WARP_FLOW): FP32 parameters, Adam moments,
   // gradient, bf16 operand copies
   int flow_order = -1, flow_layers = 0, flow_step = 0;
+  int flow_nc = 0, flow_alloc_nc = 0; // conditioning inputs (requested; of the allocated flow)
+  double flow_cond = 0.0;             // the map's conditioning value
   float *d_fw = nullptr, *d_fm = nullptr, *d_fv = nullptr, *d_fgrad = nullptr;
   uint16_t *d_fpk = nullptr;
   bool flow_in_chain = false, flow_suppressed = false;
   double *d_fx = nullptr; // flow-stage output per chunk: x [kChunk][order], jac [kChunk]
   size_t fx_cap = 0;
   // flow training set and scratch
-  double *d_tx = nullptr, *d_tw = nullptr;
+  double *d_tx = nullptr, *d_tw = nullptr, *d_tc = nullptr; // points, weights, conditioning values
   int64_t n_train = 0;
   double w_mean = 0.0;
   float *d_fpart = nullptr;
@@ -288,7 +290,8 @@ int ensure_flow_stage(kd_ctx *c, int order) {
 }
 
 kd::FlowParams flow_params(const kd_ctx *c) {
-  return kd::FlowParams{c->d_fw, c->d_fpk, c->flow_layers, kd::flow_s_scale(c->flow_layers)};
+  return kd::FlowParams{c->d_fw, c->d_fpk, c->flow_layers, kd::flow_s_scale(c->flow_layers), c->flow_alloc_nc,
+                        (float)c->flow_cond};
 }
 
 int flow_grid(const kd_ctx *c, int64_t n) {
@@ -506,6 +509,7 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_fx);
   dfree(c->d_tx);
   dfree(c->d_tw);
+  dfree(c->d_tc);
   dfree(c->d_fpart);
   dfree(c->d_floss);
   dfree(c->d_bins);
@@ -1022,9 +1026,22 @@ static uint64_t splitmix64(uint64_t &x) {
   return z ^ (z >> 31);
 }
 
-int kd_flow_n_params(int32_t order, int32_t n_layers, int64_t *n) {
-  if (!n || order < 1 || order > KD_MAX_ORDER || n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS) return KD_ERR_ARG;
-  *n = kd::flow_n_params(order, n_layers);
+int kd_flow_n_params(int32_t order, int32_t n_layers, int64_t *n) { return kd_flow_n_params_cond(order, n_layers, 0, n); }
+
+int kd_flow_n_params_cond(int32_t order, int32_t n_layers, int32_t n_cond, int64_t *n) {
+  if (!n || order < 1 || order > KD_MAX_ORDER || n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS || n_cond < 0 ||
+      n_cond > 1)
+    return KD_ERR_ARG;
+  *n = kd::flow_n_params(order, n_layers, n_cond);
+  return KD_OK;
+}
+
+int kd_flow_set_conditioning(kd_ctx *c, int32_t n_cond, double value) {
+  if (!c) return KD_ERR_ARG;
+  if (n_cond < 0 || n_cond > 1) return fail(c, KD_ERR_ARG, "n_cond must be 0 or 1");
+  if (!std::isfinite(value)) return fail(c, KD_ERR_ARG, "non-finite conditioning value");
+  c->flow_nc = n_cond;
+  c->flow_cond = n_cond ? value : 0.0;
   return KD_OK;
 }
 
@@ -1033,14 +1050,14 @@ static int flow_alloc(kd_ctx *c, int32_t order, int32_t n_layers) {
   if (n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS) return fail(c, KD_ERR_ARG, "flow layers must be in [1, 16]");
   KD_HIP(c, hipSetDevice(c->device));
   KD_HIP(c, hipStreamSynchronize(c->stream));
-  if (c->flow_order != order || c->flow_layers != n_layers || !c->d_fw) {
+  if (c->flow_order != order || c->flow_layers != n_layers || c->flow_alloc_nc != c->flow_nc || !c->d_fw) {
     dfree(c->d_fw);
     dfree(c->d_fm);
     dfree(c->d_fv);
     dfree(c->d_fgrad);
     dfree(c->d_fpk);
     c->flow_order = -1;
-    const size_t np = (size_t)kd::flow_n_params(order, n_layers);
+    const size_t np = (size_t)kd::flow_n_params(order, n_layers, c->flow_nc);
     KD_HIP(c, hipMalloc(&c->d_fw, np * sizeof(float)));
     KD_HIP(c, hipMalloc(&c->d_fm, np * sizeof(float)));
     KD_HIP(c, hipMalloc(&c->d_fv, np * sizeof(float)));
@@ -1048,17 +1065,18 @@ static int flow_alloc(kd_ctx *c, int32_t order, int32_t n_layers) {
     KD_HIP(c, hipMalloc(&c->d_fpk, sizeof(uint16_t) * (size_t)kd::kFlowPackPerLayer * n_layers));
     c->flow_order = order;
     c->flow_layers = n_layers;
+    c->flow_alloc_nc = c->flow_nc;
   }
   return KD_OK;
 }
 
 // upload params, reset Adam, rebuild the bf16 copies
 static int flow_upload(kd_ctx *c, const float *params) {
-  const size_t np = (size_t)kd::flow_n_params(c->flow_order, c->flow_layers);
+  const size_t np = (size_t)kd::flow_n_params(c->flow_order, c->flow_layers, c->flow_alloc_nc);
   KD_HIP(c, hipMemcpy(c->d_fw, params, np * sizeof(float), hipMemcpyHostToDevice));
   KD_HIP(c, hipMemsetAsync(c->d_fm, 0, np * sizeof(float), c->stream));
   KD_HIP(c, hipMemsetAsync(c->d_fv, 0, np * sizeof(float), c->stream));
-  KD_HIP(c, kd::launch_flow_pack(c->d_fw, c->d_fpk, c->flow_order, c->flow_layers, c->stream));
+  KD_HIP(c, kd::launch_flow_pack(c->d_fw, c->d_fpk, c->flow_order, c->flow_layers, c->flow_alloc_nc, c->stream));
   KD_HIP(c, hipStreamSynchronize(c->stream));
   c->flow_step = 0;
   return KD_OK;
@@ -1068,17 +1086,22 @@ int kd_flow_init(kd_ctx *c, int32_t order, int32_t n_layers, uint64_t seed) {
   if (!c) return KD_ERR_ARG;
   int rc = flow_alloc(c, order, n_layers);
   if (rc) return rc;
-  // He-uniform W1, W2 (fan-in nA and H); zero biases; W3 = b3 = 0: the identity map
-  std::vector<float> w((size_t)kd::flow_n_params(order, n_layers), 0.f);
+  // He-uniform W1, W2 (fan-in nA and H) and Wc (as W1); zero biases; W3 = b3 = 0: the identity map
+  const int nc = c->flow_alloc_nc;
+  std::vector<float> w((size_t)kd::flow_n_params(order, n_layers, nc), 0.f);
   uint64_t st = seed;
   auto unif = [&]() { return (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0; };
   for (int k = 0; k < n_layers; k++) {
     const int nA = kd::flow_nA(order, k), H = KD_FLOW_HIDDEN;
-    float *W1 = w.data() + kd::flow_layer_offset(order, k);
+    float *W1 = w.data() + kd::flow_layer_offset(order, k, nc);
     float *W2 = W1 + H * nA + H;
     const double s1 = std::sqrt(6.0 / std::max(nA, 1)), s2 = std::sqrt(6.0 / H);
     for (int i = 0; i < H * nA; i++) W1[i] = (float)(unif() * s1);
     for (int i = 0; i < H * H; i++) W2[i] = (float)(unif() * s2);
+    if (nc) {
+      float *Wc = W1 + kd::flow_layer_size(order, k, nc) - H;
+      for (int i = 0; i < H; i++) Wc[i] = (float)(unif() * s1);
+    }
   }
   return flow_upload(c, w.data());
 }
@@ -1087,8 +1110,8 @@ int kd_flow_set_params(kd_ctx *c, int32_t order, int32_t n_layers, const float *
   if (!c) return KD_ERR_ARG;
   if (!params) return fail(c, KD_ERR_ARG, "null buffer");
   if (order < 1 || order > KD_MAX_ORDER || n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS ||
-      n != kd::flow_n_params(order, n_layers))
-    return fail(c, KD_ERR_ARG, "flow parameter count does not match (order, layers)");
+      n != kd::flow_n_params(order, n_layers, c->flow_nc))
+    return fail(c, KD_ERR_ARG, "flow parameter count does not match (order, layers, conditioning inputs)");
   for (int64_t i = 0; i < n; i++)
     if (!std::isfinite(params[i])) return fail(c, KD_ERR_ARG, "non-finite flow parameter");
   int rc = flow_alloc(c, order, n_layers);
@@ -1099,7 +1122,7 @@ int kd_flow_set_params(kd_ctx *c, int32_t order, int32_t n_layers, const float *
 int kd_flow_get_params(kd_ctx *c, float *params, int64_t n) {
   if (!c) return KD_ERR_ARG;
   if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow not initialised (kd_flow_init)");
-  if (!params || n != kd::flow_n_params(c->flow_order, c->flow_layers))
+  if (!params || n != kd::flow_n_params(c->flow_order, c->flow_layers, c->flow_alloc_nc))
     return fail(c, KD_ERR_ARG, "flow parameter buffer size does not match");
   KD_HIP(c, hipSetDevice(c->device));
   KD_HIP(c, hipStreamSynchronize(c->stream));
@@ -1133,7 +1156,7 @@ int kd_flow_map(kd_ctx *c, const double *l, int64_t n, double *x, double *jac) {
 }
 
 static int flow_scratch(kd_ctx *c, int blocks, int extra_losses) {
-  const size_t np = (size_t)kd::flow_n_params(c->flow_order, c->flow_layers);
+  const size_t np = (size_t)kd::flow_n_params(c->flow_order, c->flow_layers, c->flow_alloc_nc);
   const size_t pb = np * (size_t)blocks * sizeof(float);
   if (pb > c->fpart_cap) {
     dfree(c->d_fpart);
@@ -1153,17 +1176,18 @@ static int flow_scratch(kd_ctx *c, int blocks, int extra_losses) {
 
 // gradient of the weighted loss over points [offset, offset + 64 blocks) of
 // (x, w) into d_fgrad; the loss into d_floss[blocks + loss_slot]
-static int flow_grad_launch(kd_ctx *c, const double *dx, const double *dw, int64_t n, int64_t offset, int blocks,
-                            double inv_norm, int loss_slot) {
+static int flow_grad_launch(kd_ctx *c, const double *dx, const double *dw, const double *dc, int64_t n,
+                            int64_t offset, int blocks, double inv_norm, int loss_slot) {
   kd::FlowTrainIO io{};
   io.x = dx;
   io.w = dw;
+  io.c = c->flow_alloc_nc ? dc : nullptr;
   io.n_points = n;
   io.offset = offset;
   io.inv_norm = inv_norm;
   io.part = c->d_fpart;
   io.loss_part = c->d_floss;
-  io.n_params = kd::flow_n_params(c->flow_order, c->flow_layers);
+  io.n_params = kd::flow_n_params(c->flow_order, c->flow_layers, c->flow_alloc_nc);
   KD_HIP(c, kd::launch_flow_train(c->flow_order, flow_params(c), io, blocks, c->stream));
   KD_HIP(c, kd::launch_flow_grad_reduce(c->d_fpart, blocks, io.n_params, c->d_fgrad, c->d_floss,
                                         c->d_floss + blocks + loss_slot, c->stream));
@@ -1172,6 +1196,11 @@ static int flow_grad_launch(kd_ctx *c, const double *dx, const double *dw, int64
 
 int kd_flow_loss_grad(kd_ctx *c, const double *x, const double *w, int64_t n, double w_mean, float *grad,
                       double *loss) {
+  return kd_flow_loss_grad_cond(c, x, w, nullptr, n, w_mean, grad, loss);
+}
+
+int kd_flow_loss_grad_cond(kd_ctx *c, const double *x, const double *w, const double *cond, int64_t n,
+                           double w_mean, float *grad, double *loss) {
   if (!c) return KD_ERR_ARG;
   if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow not initialised (kd_flow_init)");
   if (n <= 0 || n > (int64_t(1) << 18)) return fail(c, KD_ERR_ARG, "point count must be in [1, 2^18]");
@@ -1180,15 +1209,17 @@ int kd_flow_loss_grad(kd_ctx *c, const double *x, const double *w, int64_t n, do
   const int N = c->flow_order;
   const size_t nx = (size_t)n * N;
   KD_HIP(c, hipSetDevice(c->device));
-  int rc = ensure_stage(c, sizeof(double) * (nx + (size_t)n) + 64);
+  int rc = ensure_stage(c, sizeof(double) * (nx + 2 * (size_t)n) + 64);
   if (rc) return rc;
   const int blocks = (int)((n + 63) / 64);
   if ((rc = flow_scratch(c, blocks, 0))) return rc;
-  double *dx = (double *)c->d_stage, *dw = dx + nx;
+  double *dx = (double *)c->d_stage, *dw = dx + nx, *dc = dw + n;
   KD_HIP(c, hipMemcpyAsync(dx, x, sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
   KD_HIP(c, hipMemcpyAsync(dw, w, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  if ((rc = flow_grad_launch(c, dx, dw, n, 0, blocks, 1.0 / (w_mean * (double)n), 0))) return rc;
-  const size_t np = (size_t)kd::flow_n_params(N, c->flow_layers);
+  if (cond) KD_HIP(c, hipMemcpyAsync(dc, cond, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if ((rc = flow_grad_launch(c, dx, dw, cond ? dc : nullptr, n, 0, blocks, 1.0 / (w_mean * (double)n), 0)))
+    return rc;
+  const size_t np = (size_t)kd::flow_n_params(N, c->flow_layers, c->flow_alloc_nc);
   KD_HIP(c, hipMemcpyAsync(grad, c->d_fgrad, sizeof(float) * np, hipMemcpyDeviceToHost, c->stream));
   KD_HIP(c, hipMemcpyAsync(loss, c->d_floss + blocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   KD_HIP(c, hipStreamSynchronize(c->stream));
@@ -1207,6 +1238,7 @@ int kd_flow_collect(kd_ctx *c, int64_t n_samples, double weight_power, double *w
   KD_HIP(c, hipStreamSynchronize(c->stream));
   dfree(c->d_tx);
   dfree(c->d_tw);
+  dfree(c->d_tc);
   c->n_train = 0;
   KD_HIP(c, hipMalloc(&c->d_tx, sizeof(double) * (size_t)n_samples * N));
   KD_HIP(c, hipMalloc(&c->d_tw, sizeof(double) * (size_t)n_samples));
@@ -1232,10 +1264,57 @@ int kd_flow_collect(kd_ctx *c, int64_t n_samples, double weight_power, double *w
   double s = 0.0;
   for (double v : part) s += v;
   if (!std::isfinite(s)) return fail(c, KD_ERR_ARG, "non-finite integrand samples (g0 window too small?)");
-  if (!(s > 0.0)) return fail(c, KD_ERR_ARG, "all training samples are zero");
+  // an all-zero set is kept (kd_flow_get_samples may still read it); training on it fails
   c->n_train = n_samples;
   c->w_mean = s / (double)n_samples;
   if (w_mean) *w_mean = c->w_mean;
+  return KD_OK;
+}
+
+int kd_flow_get_samples(kd_ctx *c, double *x, double *w, int64_t n) {
+  if (!c) return KD_ERR_ARG;
+  if (c->n_train <= 0) return fail(c, KD_ERR_STATE, "no training samples (kd_flow_collect)");
+  if (n != c->n_train) return fail(c, KD_ERR_ARG, "sample count differs from the training set's");
+  if (!x || !w) return fail(c, KD_ERR_ARG, "null buffer");
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  KD_HIP(c, hipMemcpy(x, c->d_tx, sizeof(double) * (size_t)n * c->order, hipMemcpyDeviceToHost));
+  KD_HIP(c, hipMemcpy(w, c->d_tw, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  return KD_OK;
+}
+
+int kd_flow_set_samples(kd_ctx *c, const double *x, const double *w, const double *cond, int64_t n) {
+  if (!c) return KD_ERR_ARG;
+  int rc = need_integrand(c);
+  if (rc) return rc;
+  if (n < 64 || n > (int64_t(1) << 24)) return fail(c, KD_ERR_ARG, "training sample count must be in [64, 2^24]");
+  if (!x || !w) return fail(c, KD_ERR_ARG, "null buffer");
+  const int N = c->order;
+  double s = 0.0;
+  for (int64_t i = 0; i < n; i++) {
+    if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return fail(c, KD_ERR_ARG, "weights must be finite and >= 0");
+    if (cond && !std::isfinite(cond[i])) return fail(c, KD_ERR_ARG, "non-finite conditioning value");
+    for (int d = 0; d < N; d++)
+      if (!(x[i * N + d] >= 0.0 && x[i * N + d] <= 1.0)) return fail(c, KD_ERR_ARG, "training points must lie in [0, 1]");
+    s += w[i];
+  }
+  if (!(s > 0.0)) return fail(c, KD_ERR_ARG, "all training samples are zero");
+  KD_HIP(c, hipSetDevice(c->device));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
+  dfree(c->d_tx);
+  dfree(c->d_tw);
+  dfree(c->d_tc);
+  c->n_train = 0;
+  KD_HIP(c, hipMalloc(&c->d_tx, sizeof(double) * (size_t)n * N));
+  KD_HIP(c, hipMalloc(&c->d_tw, sizeof(double) * (size_t)n));
+  KD_HIP(c, hipMemcpy(c->d_tx, x, sizeof(double) * (size_t)n * N, hipMemcpyHostToDevice));
+  KD_HIP(c, hipMemcpy(c->d_tw, w, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+  if (cond) {
+    KD_HIP(c, hipMalloc(&c->d_tc, sizeof(double) * (size_t)n));
+    KD_HIP(c, hipMemcpy(c->d_tc, cond, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+  }
+  c->n_train = n;
+  c->w_mean = s / (double)n;
   return KD_OK;
 }
 
@@ -1243,6 +1322,7 @@ int kd_flow_train(kd_ctx *c, int32_t steps, int32_t batch, float lr, float lr_fi
   if (!c) return KD_ERR_ARG;
   if (!c->d_fw) return fail(c, KD_ERR_STATE, "flow not initialised (kd_flow_init)");
   if (c->n_train <= 0) return fail(c, KD_ERR_STATE, "no training samples (kd_flow_collect)");
+  if (!(c->w_mean > 0.0)) return fail(c, KD_ERR_ARG, "all training samples are zero");
   if (c->flow_order != c->order) return fail(c, KD_ERR_STATE, "flow dimension differs from the integrand order");
   if (steps < 0) return fail(c, KD_ERR_ARG, "negative step count");
   if (batch < 64 || batch % 64 || batch > (1 << 16) || c->n_train % batch)
@@ -1253,18 +1333,18 @@ int kd_flow_train(kd_ctx *c, int32_t steps, int32_t batch, float lr, float lr_fi
   const int blocks = batch / 64;
   int rc = flow_scratch(c, blocks, steps);
   if (rc) return rc;
-  const int np = kd::flow_n_params(c->flow_order, c->flow_layers);
+  const int np = kd::flow_n_params(c->flow_order, c->flow_layers, c->flow_alloc_nc);
   const double inv_norm = 1.0 / (c->w_mean * (double)batch);
   KD_HIP(c, hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < steps; s++) {
     const int64_t off = ((int64_t)s * batch) % c->n_train;
-    if ((rc = flow_grad_launch(c, c->d_tx, c->d_tw, c->n_train, off, blocks, inv_norm, s))) return rc;
+    if ((rc = flow_grad_launch(c, c->d_tx, c->d_tw, c->d_tc, c->n_train, off, blocks, inv_norm, s))) return rc;
     // cosine decay from lr to lr_final
     const float lrs = lr_final + 0.5f * (lr - lr_final) * (1.f + std::cos(3.14159265358979f * (float)s / (float)steps));
     c->flow_step++;
     KD_HIP(c, kd::launch_flow_adam(c->d_fw, c->d_fm, c->d_fv, c->d_fgrad, np, lrs, 0.9f, 0.999f, 1e-8f,
                                    c->flow_step, c->stream));
-    KD_HIP(c, kd::launch_flow_pack(c->d_fw, c->d_fpk, c->flow_order, c->flow_layers, c->stream));
+    KD_HIP(c, kd::launch_flow_pack(c->d_fw, c->d_fpk, c->flow_order, c->flow_layers, c->flow_alloc_nc, c->stream));
   }
   KD_HIP(c, hipEventRecord(c->ev1, c->stream));
   if (losses)

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <random>
 
 namespace keldy {
 
@@ -206,6 +207,7 @@ void integrator_direct::sync_warper() {
   if (warper.stamp == uploaded_stamp_) return;
   const warpers::device_chain c = warper.to_device();
   if (c.flow_layers > 0 && c.flow_stamp != uploaded_flow_stamp_) {
+    ctx_.check(kd_flow_set_conditioning(ctx_.get(), c.flow_nc, c.flow_cond));
     ctx_.check(kd_flow_set_params(ctx_.get(), c.flow_order, c.flow_layers, c.flow_params.data(),
                                   int64_t(c.flow_params.size())));
     uploaded_flow_stamp_ = c.flow_stamp;
@@ -224,6 +226,7 @@ integrator_direct::train_flow_warper(int n_layers, long nr_samples, int steps, i
   rng_dirty_ = true; // the next run re-uploads the integrator's own generator
   double w_mean = 0.0;
   ctx_.check(kd_flow_collect(ctx_.get(), int64_t(nr_samples), weight_power, &w_mean));
+  ctx_.check(kd_flow_set_conditioning(ctx_.get(), 0, 0.0));
   ctx_.check(kd_flow_init(ctx_.get(), dimension, n_layers, init_seed));
   std::vector<double> losses(size_t(std::max(steps, 0)));
   ctx_.check(kd_flow_train(ctx_.get(), steps, batch, float(lr), float(lr_final), losses.data()));
@@ -237,6 +240,76 @@ integrator_direct::train_flow_warper(int n_layers, long nr_samples, int steps, i
   uploaded_flow_stamp_ = w.stamp();
   uploaded_stamp_ = 0;
   return {std::move(w), std::move(losses)};
+}
+
+std::pair<std::vector<double>, std::vector<double>>
+integrator_direct::collect_flow_samples(long nr_samples, int rng_seed_shift, double weight_power) {
+  sync_warper();
+  const sobol gen(dimension, 0, true, true, rng_seed_shift);
+  ctx_.check(kd_set_sobol(ctx_.get(), gen.dim(), gen.csr().data(), gen.shfr().data(), gen.state()));
+  rng_dirty_ = true; // the next run re-uploads the integrator's own generator
+  double w_mean = 0.0;
+  ctx_.check(kd_flow_collect(ctx_.get(), int64_t(nr_samples), weight_power, &w_mean));
+  std::vector<double> x(size_t(nr_samples) * size_t(dimension)), w(static_cast<size_t>(nr_samples));
+  ctx_.check(kd_flow_get_samples(ctx_.get(), x.data(), w.data(), int64_t(nr_samples)));
+  return {std::move(x), std::move(w)};
+}
+
+std::pair<warpers::warper_flow_t, std::vector<double>> integrator_direct::train_flow_warper_conditioned(
+    std::vector<integrator_direct *> const &computers, std::vector<double> const &conds, int n_layers,
+    long samples_per_computer, int steps, int batch, double lr, double lr_final, int rng_seed_shift,
+    uint64_t init_seed, double weight_power) {
+  if (computers.empty() || computers.size() != conds.size())
+    throw keldy_error("one conditioning value per computer is needed");
+  if (batch < 64 || batch % 64) throw keldy_error("batch must be a positive multiple of 64");
+  const int dim = computers[0]->dimension;
+  std::vector<double> x, w, c;
+  for (size_t i = 0; i < computers.size(); i++) {
+    integrator_direct &comp = *computers[i];
+    if (comp.dimension != dim) throw keldy_error("the computers' dimensions differ");
+    if (!std::isfinite(conds[i])) throw keldy_error("non-finite conditioning value");
+    auto [xi, wi] = comp.collect_flow_samples(samples_per_computer, rng_seed_shift, weight_power);
+    double s = 0.0;
+    for (double v : wi) s += v;
+    if (!(s > 0.0)) continue; // no weight at this value: nothing to fit
+    const double scale = double(wi.size()) / s;
+    for (size_t p = 0; p < wi.size(); p++) {
+      if (!(wi[p] > 0.0)) continue;
+      x.insert(x.end(), xi.begin() + long(p) * dim, xi.begin() + long(p + 1) * dim);
+      w.push_back(wi[p] * scale);
+      c.push_back(conds[i]);
+    }
+  }
+  const size_t n_all = w.size();
+  const size_t n = n_all / size_t(batch) * size_t(batch);
+  if (n < size_t(batch)) throw keldy_error("fewer nonzero training samples than one batch");
+  // a fixed shuffle, so that every batch mixes the conditioning values
+  std::vector<size_t> perm(n_all);
+  for (size_t i = 0; i < n_all; i++) perm[i] = i;
+  std::mt19937_64 eng(init_seed ^ 0x9E3779B97F4A7C15ull);
+  std::shuffle(perm.begin(), perm.end(), eng);
+  std::vector<double> xs(n * size_t(dim)), ws(n), cs(n);
+  for (size_t q = 0; q < n; q++) {
+    const size_t p = perm[q];
+    std::copy(x.begin() + long(p) * dim, x.begin() + long(p + 1) * dim, xs.begin() + long(q) * dim);
+    ws[q] = w[p];
+    cs[q] = c[p];
+  }
+  integrator_direct &tr = *computers[0];
+  device_ctx &ctx = tr.ctx_;
+  ctx.check(kd_flow_set_conditioning(ctx.get(), 1, conds[0]));
+  ctx.check(kd_flow_init(ctx.get(), dim, n_layers, init_seed));
+  ctx.check(kd_flow_set_samples(ctx.get(), xs.data(), ws.data(), cs.data(), int64_t(n)));
+  std::vector<double> losses(size_t(std::max(steps, 0)));
+  ctx.check(kd_flow_train(ctx.get(), steps, batch, float(lr), float(lr_final), losses.data()));
+  int64_t np = 0;
+  ctx.check(kd_flow_n_params_cond(dim, n_layers, 1, &np));
+  std::vector<float> prm(static_cast<size_t>(np));
+  ctx.check(kd_flow_get_params(ctx.get(), prm.data(), np));
+  warpers::warper_flow_t fw(dim, n_layers, std::move(prm), 1, conds[0]);
+  tr.uploaded_flow_stamp_ = fw.stamp();
+  tr.uploaded_stamp_ = 0;
+  return {std::move(fw), std::move(losses)};
 }
 
 std::pair<dcomplex, double> integrator_direct::evaluate_warped_integrand(std::vector<double> const &li_vec,

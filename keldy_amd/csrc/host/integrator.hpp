@@ -217,6 +217,22 @@ class integrator_direct {
                                                                           int batch, double lr, double lr_final,
                                                                           int rng_seed_shift, uint64_t init_seed,
                                                                           double weight_power = 1.0);
+  /// Training samples of the current chain without a flow warper: nr_samples
+  /// scrambled Sobol points (seed shift rng_seed_shift) [n][dimension] and
+  /// their weights |warped integrand|^weight_power.
+  std::pair<std::vector<double>, std::vector<double>> collect_flow_samples(long nr_samples, int rng_seed_shift,
+                                                                           double weight_power);
+  /// The time-conditioned flow (no counterpart in keldy): one coupling flow
+  /// with one conditioning input, trained on the samples of every computer
+  /// (same dimension), computer i's samples tagged with conds[i] and their
+  /// weights scaled to mean 1 per computer (so every conditional density is
+  /// fitted alike); zero-weight samples dropped, the set shuffled and cut to a
+  /// multiple of batch.  Trains on computers[0]'s device context.  Returns the
+  /// flow at conds[0] (with_cond gives the others) and the per-step loss.
+  static std::pair<warpers::warper_flow_t, std::vector<double>>
+  train_flow_warper_conditioned(std::vector<integrator_direct *> const &computers, std::vector<double> const &conds,
+                                int n_layers, long samples_per_computer, int steps, int batch, double lr,
+                                double lr_final, int rng_seed_shift, uint64_t init_seed, double weight_power);
 
  protected:
   int dimension = 1;

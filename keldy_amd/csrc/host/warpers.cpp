@@ -1057,18 +1057,22 @@ double log_sig_prime(double y) {
 }
 } // namespace
 
-warper_flow_t::warper_flow_t(int order, int n_layers, std::vector<float> params)
-   : order_(order), layers_(n_layers), params_(std::move(params)), stamp_(next_stamp()) {
+warper_flow_t::warper_flow_t(int order, int n_layers, std::vector<float> params, int n_cond, double cond)
+   : order_(order), layers_(n_layers), nc_(n_cond), cond_(cond), params_(std::move(params)), stamp_(next_stamp()) {
   if (order < 1 || order > KD_MAX_ORDER) throw keldy_error("flow dimension must be in [1, 12]");
   if (n_layers < 1 || n_layers > KD_FLOW_MAX_LAYERS) throw keldy_error("flow layers must be in [1, 16]");
-  if (int64_t(params_.size()) != kd::flow_n_params(order, n_layers))
-    throw keldy_error("flow parameter count does not match (order, layers)");
+  if (n_cond < 0 || n_cond > 1) throw keldy_error("n_cond must be 0 or 1");
+  if (!std::isfinite(cond)) throw keldy_error("non-finite conditioning value");
+  if (int64_t(params_.size()) != kd::flow_n_params(order, n_layers, n_cond))
+    throw keldy_error("flow parameter count does not match (order, layers, conditioning inputs)");
 }
 
 void warper_flow_t::net(int k, const double *y, double *s, double *t) const {
   const int n = order_, nA = fl_nA(n, k), nB = n - nA;
-  const float *W1 = params_.data() + kd::flow_layer_offset(n, k);
+  const float *W1 = params_.data() + kd::flow_layer_offset(n, k, nc_);
   const float *b1 = W1 + kH * nA, *W2 = b1 + kH, *b2 = W2 + kH * kH, *W3 = b2 + kH, *b3 = W3 + 2 * nB * kH;
+  const float *Wc = b3 + 2 * nB; // nc_ = 1 only
+  const float cv = float(cond_);
   float a[KD_MAX_ORDER];
   int na = 0;
   for (int i = 0; i < n; i++)
@@ -1076,6 +1080,7 @@ void warper_flow_t::net(int k, const double *y, double *s, double *t) const {
   float h1[kH], h2[kH];
   for (int j = 0; j < kH; j++) {
     float z = b1[j];
+    if (nc_) z = std::fmaf(Wc[j], cv, z);
     for (int i = 0; i < nA; i++) z = std::fmaf(W1[j * nA + i], a[i], z);
     h1[j] = bf16_round(std::max(z, 0.f));
   }
@@ -1156,6 +1161,8 @@ void warper_flow_t::to_device(device_chain &c) const {
   c.warpers.push_back(d);
   c.flow_order = order_;
   c.flow_layers = layers_;
+  c.flow_nc = nc_;
+  c.flow_cond = cond_;
   c.flow_params = params_;
   c.flow_stamp = stamp_;
 }

@@ -26,7 +26,8 @@ struct device_chain {
   std::vector<kd_map1d> maps;
   std::vector<double> tables;
   // learned flow warper (last in the chain), if any
-  int flow_order = 0, flow_layers = 0;
+  int flow_order = 0, flow_layers = 0, flow_nc = 0;
+  double flow_cond = 0.0;
   std::vector<float> flow_params;
   uint64_t flow_stamp = 0;
 };
@@ -313,10 +314,14 @@ inline uint64_t next_stamp() {
 /// methods evaluate the same network (bf16-rounded operands where the device
 /// uses bf16 MFMA, FP64 accumulation); runs use the device kernels.  Must be
 /// the last warper of a train (it is applied first to the Sobol point).
+/// With n_cond = 1 the flow is conditioned on one value (the time-resolved
+/// runs: a scaled external time): the map is T_cond, one member of the family
+/// the parameters hold; with_cond gives the same parameters at another value.
 class warper_flow_t {
  public:
   warper_flow_t() = default;
-  warper_flow_t(int order, int n_layers, std::vector<float> params);
+  warper_flow_t(int order, int n_layers, std::vector<float> params, int n_cond = 0, double cond = 0.0);
+  warper_flow_t with_cond(double cond) const { return warper_flow_t(order_, layers_, params_, nc_, cond); }
   map_pair map_reverse(vec const &l) const; // l -> x, dx/dl
   map_pair map_forward(vec const &x) const; // x -> l, dl/dx
   vec ui_from_li(vec const &l) const { return map_reverse(l).first; }
@@ -326,12 +331,15 @@ class warper_flow_t {
   void to_device(device_chain &c) const;
   int order() const { return order_; }
   int n_layers() const { return layers_; }
+  int n_cond() const { return nc_; }
+  double cond() const { return cond_; }
   std::vector<float> const &params() const { return params_; }
   uint64_t stamp() const { return stamp_; }
 
  private:
   void net(int k, const double *y, double *s, double *t) const;
-  int order_ = 0, layers_ = 0;
+  int order_ = 0, layers_ = 0, nc_ = 0;
+  double cond_ = 0.0;
   std::vector<float> params_;
   uint64_t stamp_ = 0;
 };

@@ -40,6 +40,7 @@ struct __align__(16) FlowLdsTrain {
   uint16_t dZ[64 * FSH];
   uint16_t dO[64 * FSO];
   uint16_t Ab[64 * FSO];
+  float Cv[64]; // the points' conditioning values (nc = 1)
 };
 
 __device__ __forceinline__ v4f mfma16(v4s a, v4s b, v4f c) {
@@ -69,16 +70,18 @@ __device__ __forceinline__ double log_sig_prime(double y) {
   return -ay - 2.0 * log1p(exp(-ay));
 }
 
-// Network of coupling layer k for the wave's 64 points: a[nA] -> s[nB], t[nB].
+// Network of coupling layer k for the wave's 64 points: a[nA] (and the
+// lane's conditioning value cv when fp.nc = 1) -> s[nB], t[nB].
 // Leaves h1 (H1) and h2 (H2) of every point in LDS as bf16.
 template <int N>
-__device__ void flow_net(const FlowParams &fp, int k, const float (&a)[N], float (&s)[N], float (&t)[N],
+__device__ void flow_net(const FlowParams &fp, int k, const float (&a)[N], float cv, float (&s)[N], float (&t)[N],
                          FlowLdsFwd &L, int lane) {
   const int nA = flow_nA(N, k), nB = flow_nB(N, k);
-  const float *W1 = fp.w + flow_layer_offset(N, k);
+  const float *W1 = fp.w + flow_layer_offset(N, k, fp.nc);
   const float *b1 = W1 + FH * nA;
   const float *b2 = b1 + FH + FH * FH;
   const float *b3 = b2 + FH + 2 * nB * FH;
+  const float *Wc = b3 + 2 * nB; // read only when fp.nc = 1
   const uint16_t *pk = fp.pk + (size_t)k * kFlowPackPerLayer;
   const int r = lane & 15, g = lane >> 4;
 
@@ -90,6 +93,7 @@ __device__ void flow_net(const FlowParams &fp, int k, const float (&a)[N], float
     for (int e = 0; e < 4; e++) {
       const int j = j0 + e;
       float z = b1[j];
+      if (fp.nc) z = fmaf(Wc[j], cv, z);
       for (int i = 0; i < nA; i++) z = fmaf(W1[j * nA + i], a[i], z);
       hb[e] = f2bf(fmaxf(z, 0.f));
     }
@@ -192,7 +196,7 @@ template <int N> __global__ __launch_bounds__(64) void flow_map_kernel(FlowParam
 #pragma unroll
       for (int i = 0; i < N; i++)
         if (((i + k) & 1) == 0) a[na++] = (float)y[i];
-      flow_net<N>(fp, k, a, s, t, L, lane);
+      flow_net<N>(fp, k, a, fp.cond, s, t, L, lane);
       int nb2 = 0;
 #pragma unroll
       for (int i = 0; i < N; i++)
@@ -223,10 +227,10 @@ __device__ void flow_backward(const FlowParams &fp, int k, const float (&a)[N], 
                               const double (&vB)[N], const float (&gB)[N], float c, float (&da)[N],
                               FlowLdsTrain &L, float *__restrict__ gout, int lane) {
   const int nA = flow_nA(N, k), nB = flow_nB(N, k);
-  const int off = flow_layer_offset(N, k);
+  const int off = flow_layer_offset(N, k, fp.nc);
   const float *W1 = fp.w + off;
   const int oW1 = off, ob1 = oW1 + FH * nA, oW2 = ob1 + FH, ob2 = oW2 + FH * FH, oW3 = ob2 + FH,
-            ob3 = oW3 + 2 * nB * FH;
+            ob3 = oW3 + 2 * nB * FH, oWc = ob3 + 2 * nB;
   const uint16_t *pk = fp.pk + (size_t)k * kFlowPackPerLayer;
   const int r = lane & 15, g = lane >> 4;
 
@@ -325,7 +329,7 @@ __device__ void flow_backward(const FlowParams &fp, int k, const float (&a)[N], 
   }
 #pragma unroll
   for (int nt = 0; nt < 4; nt++) {
-    float sb = 0.f;
+    float sb = 0.f, sc = 0.f;
 #pragma unroll
     for (int mt = 0; mt < 4; mt++)
 #pragma unroll
@@ -334,10 +338,16 @@ __device__ void flow_backward(const FlowParams &fp, int k, const float (&a)[N], 
         const float v = bf2f(L.f.H1[p * FSH + j]) > 0.f ? acc1[mt][nt][e] : 0.f;
         acc1[mt][nt][e] = v;
         sb += v;
+        if (fp.nc) sc = fmaf(v, L.Cv[p], sc); // dWc[j] = sum_p dz1[p][j] c_p
       }
     sb += __shfl_xor(sb, 16, 64);
     sb += __shfl_xor(sb, 32, 64);
     if (g == 0) gout[ob1 + 16 * nt + r] = sb;
+    if (fp.nc) {
+      sc += __shfl_xor(sc, 16, 64);
+      sc += __shfl_xor(sc, 32, 64);
+      if (g == 0) gout[oWc + 16 * nt + r] = sc;
+    }
   }
 
   // dW1[j][i] = sum_p dz1[p][j] a[p][i]
@@ -416,6 +426,9 @@ template <int N> __global__ __launch_bounds__(64) void flow_train_kernel(FlowPar
   double v[N];
   double nlq = 0.0;
   const double c = valid ? io.w[pi] * io.inv_norm : 0.0;
+  // the point's conditioning value (nc = 1), also kept in LDS for the Wc gradient
+  const float cv = io.c && valid ? (float)io.c[pi] : fp.cond;
+  L.Cv[lane] = cv;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     double x = valid ? io.x[pi * N + i] : 0.5;
@@ -429,7 +442,7 @@ template <int N> __global__ __launch_bounds__(64) void flow_train_kernel(FlowPar
 #pragma unroll
     for (int i = 0; i < N; i++)
       if (((i + k) & 1) == 0) a[na++] = (float)v[i];
-    flow_net<N>(fp, k, a, s, t, L.f, lane);
+    flow_net<N>(fp, k, a, cv, s, t, L.f, lane);
     int nb = 0;
 #pragma unroll
     for (int i = 0; i < N; i++)
@@ -462,7 +475,7 @@ template <int N> __global__ __launch_bounds__(64) void flow_train_kernel(FlowPar
         nb++;
       }
     }
-    flow_net<N>(fp, k, a, s, t, L.f, lane);
+    flow_net<N>(fp, k, a, cv, s, t, L.f, lane);
     flow_backward<N>(fp, k, a, s, vB, gB, (float)c, da, L, gout, lane);
     na = 0;
     nb = 0;
@@ -520,12 +533,12 @@ __global__ __launch_bounds__(256) void flow_adam_kernel(float *__restrict__ w, f
 
 // bf16 operand copies of W2, W2^T, W3 (padded to 16 rows), W3^T.
 __global__ __launch_bounds__(256) void flow_pack_kernel(const float *__restrict__ w, uint16_t *__restrict__ pk,
-                                                        int n, int layers) {
+                                                        int n, int layers, int nc) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= layers * kFlowPackPerLayer) return;
   const int k = idx / kFlowPackPerLayer, e = idx - k * kFlowPackPerLayer;
   const int nA = flow_nA(n, k), nB = flow_nB(n, k);
-  const float *W2 = w + flow_layer_offset(n, k) + FH * nA + FH;
+  const float *W2 = w + flow_layer_offset(n, k, nc) + FH * nA + FH;
   const float *W3 = W2 + FH * FH + FH;
   float val;
   if (e < kFlowPackW2t) {
@@ -619,9 +632,9 @@ hipError_t launch_flow_adam(float *w, float *m, float *v, const float *g, int n,
   return hipGetLastError();
 }
 
-hipError_t launch_flow_pack(const float *w, uint16_t *pk, int order, int layers, hipStream_t st) {
+hipError_t launch_flow_pack(const float *w, uint16_t *pk, int order, int layers, int nc, hipStream_t st) {
   const int tot = layers * kFlowPackPerLayer;
-  hipLaunchKernelGGL(flow_pack_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, w, pk, order, layers);
+  hipLaunchKernelGGL(flow_pack_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, w, pk, order, layers, nc);
   return hipGetLastError();
 }
 

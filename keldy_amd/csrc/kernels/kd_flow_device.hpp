@@ -6,7 +6,10 @@
 // and transforms the others (set B):
 //     a = y_A;  h1 = relu(W1 a + b1);  h2 = relu(W2 h1 + b2);  o = W3 h2 + b3
 //     s = S tanh(o[0:nB]),  t = o[nB:2nB],   y_B <- y_B * exp(s) + t
-// Hidden width KD_FLOW_HIDDEN.
+// Hidden width KD_FLOW_HIDDEN.  With one conditioning input c (nc = 1: the
+// time-conditioned flow, c a scaled external time) every layer's input layer
+// is h1 = relu(W1 a + Wc c + b1): one map T_c per value of c, each a
+// bijection with its own exact Jacobian.
 //
 // Tails.  Near a face of the cube an affine map in logit space is a power law:
 // with a = exp(sum of the s a coordinate receives), x ~ 1 - (1-l)^a, so the
@@ -36,19 +39,19 @@
 namespace kd {
 
 // Parameter layout of layer k (floats), nA = |A_k|, nB = |B_k|:
-//   W1 [H][nA], b1 [H], W2 [H][H], b2 [H], W3 [2 nB][H], b3 [2 nB]
+//   W1 [H][nA], b1 [H], W2 [H][H], b2 [H], W3 [2 nB][H], b3 [2 nB], and with nc = 1 Wc [H]
 KD_HD inline int flow_nA(int n, int k) { return (n + 1 - (k & 1)) / 2; }
 KD_HD inline int flow_nB(int n, int k) { return n - flow_nA(n, k); }
-KD_HD inline int flow_layer_size(int n, int k) {
+KD_HD inline int flow_layer_size(int n, int k, int nc = 0) {
   const int H = KD_FLOW_HIDDEN, nA = flow_nA(n, k), nB = flow_nB(n, k);
-  return H * nA + H + H * H + H + 2 * nB * H + 2 * nB;
+  return H * nA + H + H * H + H + 2 * nB * H + 2 * nB + nc * H;
 }
-KD_HD inline int flow_layer_offset(int n, int k) {
+KD_HD inline int flow_layer_offset(int n, int k, int nc = 0) {
   int off = 0;
-  for (int q = 0; q < k; q++) off += flow_layer_size(n, q);
+  for (int q = 0; q < k; q++) off += flow_layer_size(n, q, nc);
   return off;
 }
-KD_HD inline int flow_n_params(int n, int layers) { return flow_layer_offset(n, layers); }
+KD_HD inline int flow_n_params(int n, int layers, int nc = 0) { return flow_layer_offset(n, layers, nc); }
 // bound S of the per-layer log-scale s = S tanh(o) (see "Tails" above)
 KD_HD inline float flow_s_scale(int layers) { return 0.62383246f / (float)((layers + 1) / 2); } // 0.9 ln 2
 
@@ -65,6 +68,8 @@ struct FlowParams {
   const uint16_t *pk; // bf16 operand copies, kFlowPackPerLayer per layer
   int layers;
   float s_scale;      // flow_s_scale(layers)
+  int nc;             // conditioning inputs: 0 or 1
+  float cond;         // the conditioning value of the map (nc = 1)
 };
 
 // float -> bfloat16 bits, round to nearest even (finite inputs)

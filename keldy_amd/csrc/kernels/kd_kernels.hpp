@@ -64,6 +64,7 @@ struct FlowMapIO {
 struct FlowTrainIO {
   const double *x; // training points [n][order]
   const double *w; // weights |integrand| [n]
+  const double *c; // per-point conditioning values [n] (nc = 1), or null: FlowParams::cond
   int64_t n_points, offset;
   double inv_norm; // 1 / (mean(w) * batch)
   float *part;     // per-block gradients [grid][n_params]
@@ -78,7 +79,7 @@ hipError_t launch_flow_grad_reduce(const float *part, int nblk, int n_params, fl
                                    double *loss, hipStream_t st);
 hipError_t launch_flow_adam(float *w, float *m, float *v, const float *g, int n, float lr, float b1, float b2,
                             float eps, int step, hipStream_t st);
-hipError_t launch_flow_pack(const float *w, uint16_t *pk, int order, int layers, hipStream_t st);
+hipError_t launch_flow_pack(const float *w, uint16_t *pk, int order, int layers, int nc, hipStream_t st);
 hipError_t launch_flow_abs(const double *vals, int64_t n, double power, double *w, double *part, int grid,
                            hipStream_t st);
 

@@ -26,8 +26,10 @@ ComputeGfKernelSingleOmega = _m.ComputeGfKernelSingleOmega
 Binner_1_1 = _m.Binner_1_1
 SparseBinner_1_1 = _m.SparseBinner_1_1
 run_batch = _m.run_batch
+train_flow_warper_conditioned = _m.train_flow_warper_conditioned
 
 __all__ = ["GfIndexT", "G0ModelOmega", "G0Model", "G0KeldyshContourT", "IntegrandGDirect",
            "ComputeChargeQDirect", "ComputeCurrentJDirect", "IntegratorDirect", "ComputeChargeQDirectTime",
            "IntegrandGDirectTime", "Binner1", "SparseBinner1", "IntegrandGKernel", "IntegrandGKernelSingleOmega",
-           "ComputeGfKernel", "ComputeGfKernelSingleOmega", "Binner_1_1", "SparseBinner_1_1", "run_batch"]
+           "ComputeGfKernel", "ComputeGfKernelSingleOmega", "Binner_1_1", "SparseBinner_1_1", "run_batch",
+           "train_flow_warper_conditioned"]

@@ -65,3 +65,18 @@ def train_flow_warper(computer, n_layers=6, nr_samples=1 << 20, steps=2000, batc
     """
     return computer.train_flow_warper(n_layers, nr_samples, steps, batch, lr, lr_final, rng_seed_shift, init_seed,
                                       weight_power)
+
+
+def train_flow_warper_conditioned(computers, conds, n_layers=6, samples_per_computer=1 << 16, steps=2000,
+                                  batch=1 << 12, lr=1e-3, lr_final=1e-5, rng_seed_shift=1000003, init_seed=0,
+                                  weight_power=1.0):
+    """Time-conditioned learned flow (no counterpart in keldy): one coupling
+    flow with the conditioning value (e.g. the external time scaled to [0, 1])
+    as an extra input of every layer, trained on the samples of all
+    `computers` (computer i's tagged with conds[i], weights scaled to mean 1
+    per computer).  Returns (WarperFlowT at conds[0], per-step loss); append
+    flow.with_cond(conds[i]) last to computer i's warper train."""
+    from keldy_amd import impurity_oneband as _io
+    return _io.train_flow_warper_conditioned(list(computers), [float(c) for c in conds], n_layers,
+                                             samples_per_computer, steps, batch, lr, lr_final, rng_seed_shift,
+                                             init_seed, weight_power)

@@ -17,7 +17,10 @@ Map T: l in (0,1)^n -> x in (0,1)^n.
     variance finite near the faces of the cube
     x = sigmoid(y)
     log J(l) = -sum log(l (1-l)) + sum_k sum s + sum log(x (1-x))
-Parameter vector, per layer: W1[H][nA], b1[H], W2[H][H], b2[H], W3[2nB][H], b3[2nB].
+Parameter vector, per layer: W1[H][nA], b1[H], W2[H][H], b2[H], W3[2nB][H], b3[2nB],
+and with one conditioning input c (nc = 1; the time-conditioned flow) Wc[H]:
+the input layer becomes h1 = relu(W1 a + Wc c + b1), c the same for every
+coupling layer (the flow is then a family of maps T_c, one per value of c).
 
 With bf16=True the operands the device feeds to its bf16 MFMA GEMMs (h1, W2,
 h2, W3) are rounded to bfloat16 (round to nearest even) first, as the device
@@ -50,27 +53,28 @@ def dims_b(n, k):
     return [i for i in range(n) if (i + k) % 2 == 1]
 
 
-def layer_size(n, k):
+def layer_size(n, k, nc=0):
     na, nb = n_a(n, k), n_b(n, k)
-    return H * na + H + H * H + H + 2 * nb * H + 2 * nb
+    return H * na + H + H * H + H + 2 * nb * H + 2 * nb + nc * H
 
 
-def n_params(n, layers):
-    return sum(layer_size(n, k) for k in range(layers))
+def n_params(n, layers, nc=0):
+    return sum(layer_size(n, k, nc) for k in range(layers))
 
 
 def s_scale(layers):
     return float(np.float32(0.62383246) / np.float32((layers + 1) // 2))
 
 
-def unpack(params, n, layers):
-    """-> list of dicts W1, b1, W2, b2, W3, b3 (views into params)."""
+def unpack(params, n, layers, nc=0):
+    """-> list of dicts W1, b1, W2, b2, W3, b3 (and Wc) (views into params)."""
     out, off = [], 0
     for k in range(layers):
         na, nb = n_a(n, k), n_b(n, k)
         d = {}
-        for name, shape in (("W1", (H, na)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)),
-                            ("W3", (2 * nb, H)), ("b3", (2 * nb,))):
+        shapes = [("W1", (H, na)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)),
+                  ("W3", (2 * nb, H)), ("b3", (2 * nb,))] + ([("Wc", (H,))] if nc else [])
+        for name, shape in shapes:
             size = int(np.prod(shape))
             d[name] = params[off:off + size].reshape(shape)
             off += size
@@ -87,9 +91,12 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def _net(L, a, emulate, S):
-    """a: (P, nA) -> s, t (P, nB) and the intermediates for backprop."""
+def _net(L, a, emulate, S, c=None):
+    """a: (P, nA), c: (P,) conditioning or None -> s, t (P, nB) and the
+    intermediates for backprop."""
     z1 = a @ L["W1"].T + L["b1"]
+    if "Wc" in L:
+        z1 = z1 + c[:, None] * L["Wc"][None, :]
     h1 = np.maximum(z1, 0.0)
     W2 = L["W2"]
     W3 = L["W3"]
@@ -122,52 +129,64 @@ def _sigmoid(y):
     return 1.0 / (1.0 + np.exp(-y))
 
 
-def forward(params, n, layers, l, emulate=False):
-    """T(l): returns x (P, n) and log J (P,)."""
-    Ls = unpack(np.asarray(params, dtype=np.float64), n, layers)
+def _cond(cond, P, emulate):
+    if cond is None:
+        return None
+    c = np.broadcast_to(np.asarray(cond, dtype=np.float64), (P,)).copy()
+    return c.astype(np.float32).astype(np.float64) if emulate else c
+
+
+def forward(params, n, layers, l, emulate=False, nc=0, cond=None):
+    """T(l) (T_c(l) with nc = 1, cond a scalar or one value per point): returns
+    x (P, n) and log J (P,)."""
+    Ls = unpack(np.asarray(params, dtype=np.float64), n, layers, nc)
     y, lc = _logit(np.array(l, dtype=np.float64, ndmin=2))
+    c = _cond(cond, y.shape[0], emulate)
     logj = -np.sum(np.log(lc) + np.log1p(-lc), axis=1)
     for k in range(layers):
         A, B = dims_a(n, k), dims_b(n, k)
         a = y[:, A].astype(np.float32).astype(np.float64) if emulate else y[:, A]
-        s, t, _ = _net(Ls[k], a, emulate, s_scale(layers))
+        s, t, _ = _net(Ls[k], a, emulate, s_scale(layers), c)
         y[:, B] = y[:, B] * np.exp(s) + t
         logj += s.sum(axis=1)
     logj += _log_sig_prime(y).sum(axis=1)
     return _sigmoid(y), logj
 
 
-def inverse(params, n, layers, x, emulate=False):
+def inverse(params, n, layers, x, emulate=False, nc=0, cond=None):
     """T^{-1}(x): returns l (P, n) and log q(x) = -log J(T^{-1}(x))."""
-    Ls = unpack(np.asarray(params, dtype=np.float64), n, layers)
+    Ls = unpack(np.asarray(params, dtype=np.float64), n, layers, nc)
     v, xc = _logit(np.array(x, dtype=np.float64, ndmin=2))
+    c = _cond(cond, v.shape[0], emulate)
     logq = -np.sum(np.log(xc) + np.log1p(-xc), axis=1)
     for k in reversed(range(layers)):
         A, B = dims_a(n, k), dims_b(n, k)
         a = v[:, A].astype(np.float32).astype(np.float64) if emulate else v[:, A]
-        s, t, _ = _net(Ls[k], a, emulate, s_scale(layers))
+        s, t, _ = _net(Ls[k], a, emulate, s_scale(layers), c)
         v[:, B] = (v[:, B] - t) * np.exp(-s)
         logq -= s.sum(axis=1)
     logq += _log_sig_prime(v).sum(axis=1)
     return _sigmoid(v), logq
 
 
-def loss_and_grad(params, n, layers, x, w, emulate=False):
+def loss_and_grad(params, n, layers, x, w, emulate=False, nc=0, cond=None):
     """Weighted maximum-likelihood loss mean_p w~_p (-log q(x_p)), w~ = w / mean(w),
-    and its gradient with respect to params (analytic backprop, FP64)."""
+    and its gradient with respect to params (analytic backprop, FP64); with
+    nc = 1, q = q_c at each point's conditioning value cond[p]."""
     params = np.asarray(params, dtype=np.float64)
-    Ls = unpack(params, n, layers)
+    Ls = unpack(params, n, layers, nc)
     grad = np.zeros_like(params)
-    G = unpack(grad, n, layers)
+    G = unpack(grad, n, layers, nc)
     x = np.array(x, dtype=np.float64, ndmin=2)
     P = x.shape[0]
+    cv = _cond(cond, P, emulate)
     wt = np.asarray(w, dtype=np.float64) / np.mean(w)
     v, xc = _logit(x)
     nlq = np.sum(np.log(xc) + np.log1p(-xc), axis=1)  # -log q, accumulated
     for k in reversed(range(layers)):
         A, B = dims_a(n, k), dims_b(n, k)
         a = v[:, A].astype(np.float32).astype(np.float64) if emulate else v[:, A]
-        s, t, _ = _net(Ls[k], a, emulate, s_scale(layers))
+        s, t, _ = _net(Ls[k], a, emulate, s_scale(layers), cv)
         v[:, B] = (v[:, B] - t) * np.exp(-s)
         nlq += s.sum(axis=1)
     nlq -= _log_sig_prime(v).sum(axis=1)
@@ -178,7 +197,7 @@ def loss_and_grad(params, n, layers, x, w, emulate=False):
     for k in range(layers):
         A, B = dims_a(n, k), dims_b(n, k)
         a = v[:, A].astype(np.float32).astype(np.float64) if emulate else v[:, A]
-        s, t, I = _net(Ls[k], a, emulate, s_scale(layers))
+        s, t, I = _net(Ls[k], a, emulate, s_scale(layers), cv)
         vB = v[:, B]
         gB = g[:, B]
         es = np.exp(-s)
@@ -197,6 +216,8 @@ def loss_and_grad(params, n, layers, x, w, emulate=False):
         dz1 = (dz2 @ W2) * (I["z1"] > 0)
         G[k]["W1"][...] = dz1.T @ I["a"]
         G[k]["b1"][...] = dz1.sum(axis=0)
+        if nc:
+            G[k]["Wc"][...] = dz1.T @ cv
         da = dz1 @ Ls[k]["W1"]
         g_new = g.copy()
         g_new[:, B] = gB * es
@@ -208,13 +229,15 @@ def loss_and_grad(params, n, layers, x, w, emulate=False):
     return loss, grad
 
 
-def init_params(n, layers, seed):
-    """Deterministic initialisation: He-uniform W1, W2; zero biases; W3 = b3 = 0,
-    so the untrained flow is the identity map."""
+def init_params(n, layers, seed, nc=0):
+    """Deterministic initialisation: He-uniform W1, W2 (and Wc); zero biases;
+    W3 = b3 = 0, so the untrained flow is the identity map."""
     rng = np.random.default_rng(seed)
-    p = np.zeros(n_params(n, layers))
-    for k, L in enumerate(unpack(p, n, layers)):
+    p = np.zeros(n_params(n, layers, nc))
+    for k, L in enumerate(unpack(p, n, layers, nc)):
         na = max(n_a(n, k), 1)
         L["W1"][...] = rng.uniform(-1, 1, L["W1"].shape) * np.sqrt(6.0 / na)
         L["W2"][...] = rng.uniform(-1, 1, L["W2"].shape) * np.sqrt(6.0 / H)
+        if nc:
+            L["Wc"][...] = rng.uniform(-1, 1, L["Wc"].shape) * np.sqrt(6.0 / na)
     return p

@@ -152,3 +152,79 @@ def test_training_gain_and_unbiasedness():
     assert abs(learned.mean() - plain.mean()) <= 5 * err
     ratio = sp ** 2 / sl ** 2
     assert ratio > 2.0, f"variance ratio plain/learned = {ratio:.2f}"
+
+
+# ---------------------------------------------------------------------------
+# the time-conditioned flow (one conditioning input c per layer)
+
+def _cparams(n, layers, seed):
+    p0 = flow_params(n, layers, seed)
+    rng = np.random.default_rng(seed + 7)
+    out, off = [], 0
+    for k in range(layers):
+        size = F.layer_size(n, k)
+        out.append(p0[off:off + size])
+        out.append(rng.normal(0, 0.5, F.H))
+        off += size
+    return np.concatenate(out).astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("n,layers", [(3, 4), (10, 6)])
+def test_conditioned_flow_map_vs_oracle(n, layers):
+    ctx = capi.Context(0)
+    p = _cparams(n, layers, 40 + n)
+    ctx.flow_set_conditioning(1, 0.0)
+    ctx.flow_set_params(n, layers, p)
+    l = np.random.default_rng(n).uniform(0.0, 1.0, (1000, n))
+    for cv in (0.0, 0.37, 1.0):
+        ctx.flow_set_conditioning(1, cv)   # the value only: the parameters stay
+        x, jac = ctx.flow_map(l)
+        xo, ljo = F.forward(p, n, layers, l, emulate=True, nc=1, cond=cv)
+        dx = np.max(np.abs(x - xo), axis=1)
+        dj = np.abs(np.log(jac) - ljo)
+        assert np.quantile(dx, 0.99) <= 1e-6 and np.max(dx) <= 1e-4, cv
+        assert np.quantile(dj, 0.99) <= 1e-5 and np.max(dj) <= 1e-3, cv
+    # a different value is a different map
+    xa, _ = ctx.flow_map(l)
+    ctx.flow_set_conditioning(1, 0.0)
+    xb, _ = ctx.flow_map(l)
+    assert np.max(np.abs(xa - xb)) > 1e-4
+    np.testing.assert_array_equal(ctx.flow_get_params(), p.astype(np.float32))
+
+
+@pytest.mark.parametrize("n,layers", [(3, 4), (10, 6)])
+def test_conditioned_flow_gradient_vs_oracle(n, layers):
+    ctx = capi.Context(0)
+    p = _cparams(n, layers, 7 * n)
+    ctx.flow_set_conditioning(1, 0.5)
+    ctx.flow_set_params(n, layers, p)
+    rng = np.random.default_rng(11)
+    m = 2048 + 37
+    x = rng.uniform(0.02, 0.98, (m, n))
+    w = rng.uniform(0.0, 2.0, m)
+    c = rng.uniform(0.0, 1.0, m)
+    loss, g = ctx.flow_loss_grad(x, w, cond=c)
+    lo, go = F.loss_and_grad(p, n, layers, x, w, emulate=True, nc=1, cond=c)
+    # the network's s and t are FP32 on the device: an absolute floor of 1e-7
+    # on a loss of a few 1e-4 (the sum of s over layers, each rounded in FP32)
+    assert abs(loss - lo) <= 1e-5 * abs(lo) + 1e-7
+    off = 0
+    for k in range(layers):
+        sz = F.layer_size(n, k, 1)
+        gk, gok = g[off:off + sz].astype(np.float64), go[off:off + sz]
+        assert np.linalg.norm(gk - gok) <= 3e-2 * np.linalg.norm(gok), f"layer {k}"
+        # the conditioning column alone (the last H entries of the layer)
+        gc, goc = gk[-F.H:], gok[-F.H:]
+        assert np.linalg.norm(gc - goc) <= 3e-2 * np.linalg.norm(goc), f"layer {k} Wc"
+        off += sz
+
+
+def test_conditioned_flow_checks():
+    ctx = capi.Context(0)
+    with pytest.raises(capi.KdError, match="n_cond"):
+        ctx.flow_set_conditioning(2, 0.0)
+    ctx.flow_set_conditioning(1, 0.0)
+    with pytest.raises(capi.KdError, match="parameter count"):
+        ctx.flow_set_params(3, 2, np.zeros(F.n_params(3, 2), dtype=np.float32))
+    ctx.flow_set_params(3, 2, np.zeros(F.n_params(3, 2, 1), dtype=np.float32))
+    assert capi.flow_n_params(3, 2, 1) == F.n_params(3, 2, 1)

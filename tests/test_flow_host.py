@@ -51,3 +51,31 @@ def test_flow_in_train_host_map():
     u, jt = tr.map_reverse(l)
     np.testing.assert_allclose(u, kw.WarperPlasmaUvT(10.0).map_reverse(x)[0], atol=1e-13)
     assert abs(jt - jf) < 1e-12 * abs(jf)
+
+
+def test_conditioned_host_map_vs_oracle():
+    """The time-conditioned flow on the host: T_c of the oracle at each value,
+    with_cond moves to another member of the family, the parameter count
+    carries the conditioning column."""
+    n, layers = 6, 3
+    p0 = flow_params(n, layers, 17)
+    rng = np.random.default_rng(5)
+    blocks, off = [], 0
+    for k in range(layers):
+        size = F.layer_size(n, k)
+        blocks += [p0[off:off + size], rng.normal(0, 0.5, F.H)]
+        off += size
+    p = np.concatenate(blocks).astype(np.float32)
+    with pytest.raises(RuntimeError, match="parameter count"):
+        kw.WarperFlowT(n, layers, p)  # unconditioned count expected
+    w = kw.WarperFlowT(n, layers, p, n_cond=1, cond=0.25)
+    assert w.n_cond() == 1 and w.cond() == 0.25
+    l = rng.uniform(0.01, 0.99, (20, n))
+    for cv, ww in ((0.25, w), (0.9, w.with_cond(0.9))):
+        xo, ljo = F.forward(p.astype(np.float64), n, layers, l, emulate=True, nc=1, cond=cv)
+        for i in range(len(l)):
+            x, j = ww.map_reverse(list(l[i]))
+            np.testing.assert_allclose(x, xo[i], rtol=0, atol=1e-6)
+            assert abs(np.log(j) - ljo[i]) < 1e-5
+            l2, jf = ww.map_forward(x)
+            np.testing.assert_allclose(l2, l[i], rtol=0, atol=1e-10)

@@ -101,6 +101,16 @@ def pmc_traffic(kernel, points_per_launch):
     return None
 
 
+def charge_truth(order):
+    """The committed high-precision Q_8(10) (tests/golden/q8_truth.json, made by
+    tests/golden/make_q8_truth.py: 16 replicas of 2^28 points), or None."""
+    path = os.path.join(ROOT, "tests", "golden", "q8_truth.json")
+    if order != 8 or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
 def model_params():
     return dict(beta=-1.0, bias_V=0.0, eps_d=0.0, Gamma=1.0, alpha=0.0, half_bandwidth=10.0,
                 bath_type="flatband", time_max=10.0, nr_time_points_gf=10000, ft_method="analytic")
@@ -293,7 +303,8 @@ def workload_rates():
     - the time-resolved current at order 10: 128 external times in (0, 20],
       2^20 points each, dot-lead semicircle g0 by contour integration,
       [plasma_uv(t), inverse(t, 1)]; summed over the times, then all times as
-      one hipGraph launch."""
+      one hipGraph launch;
+    - the same with the time-conditioned flow appended (current_learned)."""
     import numpy as np
 
     from keldy_amd import impurity_oneband as kio
@@ -348,13 +359,68 @@ def workload_rates():
         cur.append(c)
     kio.run_batch(cur, [m] * len(cur))
     cur_ms = kio.run_batch(cur, [m] * len(cur))
+    learned = current_learned(cur, times, m)
     return {
         "orders_1_to_10": {"per_order": per_order, "one_graph_launch_ms": graph_ms},
         "current_order10_128_times": {"points": 128 * m, "in_domain": int(dom_t), "lu_ms_sum": lu_t,
                                       "in_domain_points_per_s": dom_t / (lu_t * 1e-3),
                                       "fallback_configurations": int(fb_t), "algorithm": cur[0].get_algorithm(),
                                       "one_graph_launch_ms": cur_ms},
+        "current_order10_128_times_learned": learned,
     }
+
+
+def current_learned(cur, times, m, reps=4, seed0=300):
+    """The time-conditioned flow on the order-10 current (the 128 computers of
+    workload_rates): a 2-layer flow with c = t / 20 trained on 2^18 samples
+    per time (weights |f|^(1/4)), then `reps` replicas of m points per time
+    with and without it on the same seeds, each replica set one hipGraph
+    launch: per-time variance ratio plain/learned at equal points, replica-set
+    wall times and in-domain fractions."""
+    import numpy as np
+
+    from keldy_amd import impurity_oneband as kio
+    from keldy_amd import warpers as kw
+
+    def replicas():
+        est = np.zeros((reps, len(cur)), dtype=complex)
+        dom = np.zeros(len(cur))
+        secs = []
+        for r in range(reps):
+            prev = []
+            for c in cur:
+                c.reset_rng("sobol", 0, True, True, seed0 + r)
+                n0 = c.reduce_nr_points_run()
+                prev.append((c.reduce_result() * n0 if n0 else 0j, c.reduce_nr_points_in_domain()))
+            t0 = time.perf_counter()
+            kio.run_batch(cur, [m] * len(cur))
+            secs.append(time.perf_counter() - t0)
+            for i, c in enumerate(cur):
+                est[r, i] = (c.reduce_result() * c.reduce_nr_points_run() - prev[i][0]) / m
+                dom[i] += c.reduce_nr_points_in_domain() - prev[i][1]
+        return est, dom / (reps * m), float(np.median(secs))
+
+    conds = [float(t) / 20.0 for t in times]
+    plain, dom_p, sec_p = replicas()
+    t0 = time.perf_counter()
+    flow, losses = kw.train_flow_warper_conditioned(cur, conds, n_layers=2, samples_per_computer=1 << 18,
+                                                    steps=3000, batch=1 << 13, lr=1e-3, lr_final=1e-5,
+                                                    weight_power=0.25)
+    t_train = time.perf_counter() - t0
+    for c, cv in zip(cur, conds):
+        c.warper.emplace_back(flow.with_cond(cv))
+    learned, dom_l, sec_l = replicas()
+    vp = plain.real.var(axis=0, ddof=1) + plain.imag.var(axis=0, ddof=1)
+    vl = learned.real.var(axis=0, ddof=1) + learned.imag.var(axis=0, ddof=1)
+    ok = (vp > 0) & (vl > 0) & (dom_p * m >= 100)
+    ratio = vp[ok] / vl[ok]
+    gm = float(np.exp(np.mean(np.log(ratio)))) if ok.any() else float("nan")
+    return {"points_per_time": m, "replicas": reps, "layers": 2, "train_s": t_train,
+            "in_domain_fraction_plain": float(dom_p.mean()), "in_domain_fraction_learned": float(dom_l.mean()),
+            "times_compared": int(ok.sum()), "variance_ratio_plain_over_learned_geomean": gm,
+            "variance_ratio_quantiles_10_50_90": [float(x) for x in np.quantile(ratio, [0.1, 0.5, 0.9])] if ok.any() else None,
+            "replica_set_s_plain": sec_p, "replica_set_s_learned": sec_l,
+            "time_to_target_ratio_learned_over_plain": (sec_l / sec_p) / gm if ok.any() else None}
 
 
 def main():
@@ -541,6 +607,15 @@ def main():
 
     estimator = {"plain": summary(plain), "setup_s": setup_s,
                  "end_to_end_s": {"plain": setup_s + plain["t_needed"]}}
+    truth = charge_truth(args.order)
+    if truth:
+        qp = (-1j * (1j ** args.order) * plain["est"]).real
+        err = float(np.hypot(qp.std(ddof=1) / np.sqrt(len(qp)), truth["charge_std_error"]))
+        estimator["plain_vs_truth"] = {
+            "truth": truth["charge_mean"], "truth_std_error": truth["charge_std_error"],
+            "truth_points": truth["replicas"] * truth["points_per_replica"], "estimate": float(qp.mean()),
+            "rel_deviation": float(qp.mean() / truth["charge_mean"] - 1.0),
+            "deviation_over_sigma": float(abs(qp.mean() - truth["charge_mean"]) / err)}
     if fp32_side:
         estimator["plain_fp32_lu"] = fp32_side
     if selection:

@@ -124,6 +124,10 @@ g0_model::g0_model(g0_model_omega model_omega_, bool make_dot_lead_)
     // as the reference does (model.cpp:151-165); "fft" (a TRIQS FFT on the
     // adjoint mesh in the reference, model.cpp:179-212) by the same
     // quadrature, which is the limit that FFT approximates.
+    if (p.ft_method == "fft")
+      std::cout << "WARNING: ft_method 'fft' is computed by quadrature of the frequency integral (the limit the "
+                   "FFT approximates); the tables differ from an FFT's by its discretisation error."
+                << std::endl;
     const double half_bias = std::abs(p.bias_V / 2);
     if (p.bath_type == "semicircle") {
       if (half_bias < p.half_bandwidth)

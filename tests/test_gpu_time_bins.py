@@ -6,7 +6,8 @@ Checks (the reference's wick_direct_time_test.cpp:19-84 and binner semantics):
   integrand there (1e-16 there; here 1e-15 relative: same device evaluation);
 - the bins of a run sum to the plain run's result on the same points (1e-13
   relative: bin sums reassociate the per-point values);
-- bin contents equal the oracle's binning of the same per-point values
+- bin contents equal the oracle's binning of its own per-point values (its
+  warper chain and Keldysh sum at the same Sobol points)
   (1e-10 relative), counts of values added per bin match exactly, and points
   whose earliest time falls outside [0, t] are counted as dropped.
 """
@@ -61,10 +62,16 @@ def test_binned_run_consistency(order):
     added, dropped = b.get_nr_values_added().sum(), b.get_nr_values_dropped()
     assert added + dropped == n and dropped == n - cd.reduce_nr_points_in_domain()
 
-    # the same binning by the oracle from per-point values of the same Sobol points
+    # the same binning by the oracle: its own warper chain and Keldysh sum at the
+    # same Sobol points (wick_direct_time.cpp:42-163 bins by the earliest time)
+    from tests import gpu_util as U
     l = O.Sobol(order, 0).points(n)
-    vals, dom, jac = cd.evaluate_warped_integrand_batch(l)
-    u = np.array([cd.warper.map_reverse(list(x))[0] for x in l])
+    train, _ = U.chain([("plasma", t), ("map", "exponential", t, 1.0, 1)])
+    f = O.charge_integrand(O.G0Table(m.g0_lesser[0, 0], m.g0_greater[0, 0], m.time_max), t)
+    vals = np.array([O.eval_warped(f, train, x)[0] for x in l])
+    u = np.array([train.reverse(x)[0] for x in l])
+    dvals, _, _ = cd.evaluate_warped_integrand_batch(l)
+    assert np.max(np.abs(dvals - vals)) <= 1e-10 * np.max(np.abs(vals))
     tmin = u.min(axis=1)
     inb = (tmin >= 0.0) & (tmin <= t)
     idx = np.where(tmin == t, slices - 1, np.floor(np.where(inb, tmin, 0.0) / (t / slices)).astype(int))

@@ -211,6 +211,7 @@ kd::KdParams base_params(const kd_ctx *c) {
   p.n_orb = c->n_orb;
   p.t_min = -c->time_max;
   p.dt = 2.0 * c->time_max / (double)(c->n_pts - 1);
+  p.inv_dt = 1.0 / p.dt;
   p.alpha = c->alpha;
   p.A = kd::KdGf{c->A.time, c->A.k_idx, c->A.spin, c->A.orbital};
   p.B = kd::KdGf{c->B.time, c->B.k_idx, c->B.spin, c->B.orbital};

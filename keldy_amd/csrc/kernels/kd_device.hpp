@@ -62,7 +62,7 @@ struct KdParams {
   const double2 *lesser;
   const double2 *greater;
   int n_pts, n_orb;
-  double t_min, dt, alpha;
+  double t_min, dt, inv_dt, alpha; // inv_dt = RN(1 / dt)
   // integrand (wick_direct.hpp:34-60)
   KdGf A, B;
   double cutoff;
@@ -149,14 +149,22 @@ struct TabPos {
   bool out;  // outside the mesh by more than rounding: NaN (the reference's gf evaluation throws)
 };
 
+// The mesh coordinate a = (t - t_min) / dt, correctly rounded (so that mesh
+// points land on their index exactly), by Markstein's correction of the
+// reciprocal product: q = x (1/dt), r = x - q dt exactly (FMA),
+// a = RN(q + r (1/dt)) -- equal to the IEEE quotient when 1/dt is the
+// correctly rounded reciprocal (Markstein, IBM J. Res. Dev. 34 (1990) 111),
+// three operations instead of the scaled division sequence.  The index in
+// 32-bit arithmetic from a clamped copy (no out-of-range conversion).
 __device__ __forceinline__ TabPos table_pos(const KdParams &p, double t) {
-  const double a = (t - p.t_min) / p.dt; // a quotient, so that mesh points land on their index exactly
-  long i = (long)floor(a);
+  const double x = t - p.t_min;
+  const double q0 = x * p.inv_dt;
+  const double a = fma(fma(-q0, p.dt, x), p.inv_dt, q0);
   TabPos q;
   q.out = !(a >= -1e-9) || !(a <= (double)(p.n_pts - 1) + 1e-9);
-  i = i > p.n_pts - 2 ? p.n_pts - 2 : i;
-  i = i < 0 ? 0 : i;
-  q.i = (int)i;
+  const double ac = fmin(fmax(a, 0.0), (double)(p.n_pts - 2));
+  const int i = (int)floor(ac);
+  q.i = i;
   q.w = a - (double)i;
   return q;
 }

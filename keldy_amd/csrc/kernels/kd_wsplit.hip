@@ -56,6 +56,8 @@ __device__ __forceinline__ double lane_f64(double x, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
                           __builtin_amdgcn_readlane(__double2loint(x), l));
 }
+// a wave-uniform lane's value (the lane index uniform: v_readlane, no LDS trip)
+__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ double wave_sum(double x) {
   x += dpp_f64<0xB1>(x);  // quad_perm [1,0,3,2]
   x += dpp_f64<0x4E>(x);  // quad_perm [2,3,0,1]
@@ -261,14 +263,14 @@ integrand_wsplit_kernel(KdParams p, PrepBuf buf) {
       wb = adv % nbatch;
       if (!mask) continue;
       const int j1 = __builtin_ctzll(mask);
-      const int64_t s1 = (int64_t)__shfl(bl, j1, 64) * 64 + __shfl(gl, j1, 64);
+      const int64_t s1 = (int64_t)rdlane(bl, j1) * 64 + rdlane(gl, j1);
       u_n = buf.u[s1 * N + (lane < N ? lane : N - 1)];
       jac_n = buf.jac[s1];
     }
     const int j = __builtin_ctzll(mask);
     mask &= mask - 1;
-    const int g = __shfl(gl, j, 64);
-    const int64_t b = __shfl(bl, j, 64);
+    const int g = rdlane(gl, j);
+    const int64_t b = rdlane(bl, j);
     const int64_t slot = b * 64 + g;
     const double uc = u_n, jj = jac_n;
     // Opaque per-point copy of the lane id: keeps the lane-only index math of
@@ -311,7 +313,7 @@ integrand_wsplit_kernel(KdParams p, PrepBuf buf) {
     // would make the compiler copy the register and wait for the load at once)
     if (mask) {
       const int j2 = __builtin_ctzll(mask);
-      const int64_t s2 = (int64_t)__shfl(bl, j2, 64) * 64 + __shfl(gl, j2, 64);
+      const int64_t s2 = (int64_t)rdlane(bl, j2) * 64 + rdlane(gl, j2);
       u_n = buf.u[s2 * N + (lane < N ? lane : N - 1)];
       jac_n = buf.jac[s2];
     }

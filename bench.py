@@ -447,7 +447,7 @@ def main():
     ap.add_argument("--flow-lr", type=float, default=1e-3)
     ap.add_argument("--flow-alphas", type=float, nargs="+", default=[1.0, 0.5, 0.25],
                     help="weight powers of the flow candidates (density ~ |f|^alpha)")
-    ap.add_argument("--algorithm", default=None, choices=["config", "pair", "split", "wsplit", "tree"],
+    ap.add_argument("--algorithm", default=None, choices=["config", "pair", "split", "wsplit", "ctree", "tree"],
                     help="Keldysh-sum algorithm (default: the library's)")
     ap.add_argument("--observable", default="charge", choices=["charge", "kernel"],
                     help="kernel: only time the kernel method (compute_gf_kernel, 10 bins) on the same chain")
@@ -600,7 +600,7 @@ def main():
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     launches_per_step = max(1, -(-args.points // (1 << 21)))
     algo = comp.get_algorithm()
-    prefix = {"tree": "tree_", "split": "split_", "wsplit": "wsplit_"}.get(algo, "")
+    prefix = {"tree": "tree_", "split": "split_", "wsplit": "wsplit_", "ctree": "ctree_"}.get(algo, "")
     kname = (f"integrand_{prefix}kernel"
              f"<{args.order},{'double' if args.precision == 'fp64' else 'float'}{',pair' if algo == 'pair' else ''}>")
     traffic = pmc_traffic(kname, min(args.points, 1 << 21))

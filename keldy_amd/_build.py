@@ -29,6 +29,7 @@ HIP_SOURCES = [
     "kernels/kd_cofactor.hip",
     "kernels/kd_split.hip",
     "kernels/kd_wsplit.hip",
+    "kernels/kd_ctree.hip",
     "capi/kd_capi.cpp",
 ]
 

@@ -22,6 +22,7 @@ KD_MAP_INVERSE_POW, KD_MAP_NEAREST, KD_MAP_HYBRID = 4, 5, 6
 KD_MAP_PROJECTION = 7
 KD_WARP_IDENTITY, KD_WARP_PLASMA, KD_WARP_PRODUCT, KD_WARP_FLOW = 0, 1, 2, 3
 KD_ALGO_DEFAULT, KD_ALGO_CONFIG, KD_ALGO_TREE, KD_ALGO_PAIR, KD_ALGO_SPLIT, KD_ALGO_WSPLIT = -1, 0, 1, 2, 3, 4
+KD_ALGO_CTREE = 5
 KD_OBS_DIRECT, KD_OBS_KERNEL, KD_OBS_KERNEL_OMEGA = 0, 1, 2
 
 

@@ -161,14 +161,17 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               v = KD_ALGO_SPLIT;
             else if (a == "wsplit")
               v = KD_ALGO_WSPLIT;
+            else if (a == "ctree")
+              v = KD_ALGO_CTREE;
             else if (a == "default")
               v = KD_ALGO_DEFAULT;
             else
-              throw keldy_error("algorithm must be 'default', 'config', 'pair', 'split', 'wsplit' or 'tree'");
+              throw keldy_error("algorithm must be 'default', 'config', 'pair', 'split', 'wsplit', 'ctree' or 'tree'");
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
           py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (wsplit at orders 6..10, config otherwise), 'config', 'pair' "
-          "(orders >= 7), 'split' or 'wsplit' (orders 6..10) or 'tree' (orders >= 6)")
+          "(orders >= 7), 'split' or 'wsplit' (orders 6..10), 'ctree' (the time-ordered tree, orders 6..8) or 'tree' "
+          "(orders >= 6)")
       .def("get_algorithm",
            [](T &s) {
              int32_t v = KD_ALGO_CONFIG;
@@ -177,6 +180,7 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
                                 : v == KD_ALGO_PAIR  ? "pair"
                                 : v == KD_ALGO_SPLIT ? "split"
                                 : v == KD_ALGO_WSPLIT ? "wsplit"
+                                : v == KD_ALGO_CTREE ? "ctree"
                                                      : "config");
            })
       .def("last_timing", &T::last_timing)

@@ -161,6 +161,11 @@ bool use_wsplit(const kd_ctx *c) {
   return chosen && c->obs == KD_OBS_DIRECT && kd::wsplit_supported(c->order);
 }
 
+// the time-ordered tree of vertex-level corrections, one point per wave (orders 6..8)
+bool use_ctree(const kd_ctx *c) {
+  return c->algo == KD_ALGO_CTREE && c->obs == KD_OBS_DIRECT && kd::ctree_supported(c->order);
+}
+
 // the split tree + per-configuration LU with one point per workgroup (orders 6..10)
 bool use_split(const kd_ctx *c) {
   return c->algo == KD_ALGO_SPLIT && c->obs == KD_OBS_DIRECT && kd::split_threads(c->order) > 0;
@@ -175,7 +180,7 @@ int64_t vals_per_point(const kd_ctx *c) {
 // group), capped at lu_waves_per_cu waves per CU -- the kernel is persistent
 // over the rest.
 int grid_for(const kd_ctx *c, int64_t n_points) {
-  const bool per_point = use_tree(c) || use_split(c) || use_wsplit(c); // one workgroup item per point
+  const bool per_point = use_tree(c) || use_split(c) || use_wsplit(c) || use_ctree(c); // one workgroup item per point
   const int64_t per_batch = per_point ? 64 : kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
   int64_t nb = (n_points + 63) / 64 * per_batch;
   // waves per CU -> workgroups per CU (the split kernel's workgroups hold several waves)
@@ -390,6 +395,8 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepB
       KD_HIP(c, kd::launch_tree(N, c->precision, p, buf, grid, c->stream));
     else if (use_split(c))
       KD_HIP(c, kd::launch_split(N, c->precision, p, buf, grid, c->stream));
+    else if (use_ctree(c))
+      KD_HIP(c, kd::launch_ctree(N, c->precision, p, buf, grid, c->stream));
     else if (use_wsplit(c))
       KD_HIP(c, kd::launch_wsplit(N, c->precision, p, buf, grid, c->stream));
     else
@@ -985,6 +992,45 @@ static double wsplit_flops(int N) {
   return f + std::ldexp(16 * dc * dc + 61 * dc + 19, K);
 }
 
+// The time-ordered tree, as executed (complex multiply-add 8, multiply 6,
+// subtraction 2, reciprocal 7, FP64 path-determinant product 6):
+// levels 1..3 breadth first, each (child, row) item with its own pivot
+// reciprocal and row factor, a subtraction and a multiply-add per entry (a
+// multiply at level 1); each lane's forced levels 3..5 (pivot, reciprocal, the
+// factors u and w, every entry of levels 4..6 one multiply-add) and its
+// subtree over the later vertices; per leaf the last pivot, the border factors,
+// p (p c_ee + u_e v_e), its product with dp^2 and the signed FP64 sum.
+static double ctree_flops(int N) {
+  double f = 0.0;
+  for (int k = 0; k < 3; k++) {
+    const double rc = N - k, items = std::ldexp(rc, k + 1);
+    const double per_item = (k > 0 ? 4 : 0) + 7 + 6 + (k > 0 ? rc * (2 + 8) : rc * 6);
+    f += items * per_item + std::ldexp(6.0, k + 1);
+  }
+  // per lane: forced steps at vertices 3..min(5, N-2), then the subtree
+  double lane = 0.0;
+  const int kf = N - 1 < 6 ? N - 1 : 6; // first vertex not forced
+  for (int k = 3; k < kf; k++) {
+    const double r = N - k + 1;
+    lane += 2 + 7 + (r - 1) * (2 + 2 + 6) + 6;
+    if (k > 3) lane += 8 * r * r; // the level-k entries, one multiply-add each
+  }
+  const double leaf = 2 + 4 + 20 + 6 + 2;
+  if (kf == N - 1 && N <= 6) {
+    lane += 8 * 4 + 6 + leaf; // the level-(N-1) 2x2 entries, dp^2 and the lane's own leaf
+  } else {
+    lane += 8.0 * (N - kf + 1) * (N - kf + 1); // the level-6 entries
+    double sub = 0.0;
+    for (int k = kf; k < N - 1; k++) { // internal subtree levels: 2^(k-kf+1) children
+      const double r = N - k + 1;
+      sub += std::ldexp(2 + 7 + (r - 1) * (2 + 2 + 6) + 8 * (r - 1) * (r - 1) + 6, k - kf + 1);
+    }
+    sub += std::ldexp(6.0, N - 1 - kf) + std::ldexp(leaf, N - kf); // dp^2 per level-(N-1) node, the leaves
+    lane += sub;
+  }
+  return f + 64.0 * lane;
+}
+
 int kd_flops_per_point(const kd_ctx *c, double *flops) {
   if (!c || !flops) return KD_ERR_ARG;
   if (c->obs != KD_OBS_DIRECT) {
@@ -1001,6 +1047,10 @@ int kd_flops_per_point(const kd_ctx *c, double *flops) {
   }
   if (use_split(c)) {
     *flops = split_flops(c->order);
+    return KD_OK;
+  }
+  if (use_ctree(c)) {
+    *flops = ctree_flops(c->order);
     return KD_OK;
   }
   if (use_wsplit(c)) {
@@ -1389,18 +1439,21 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   // or 32 waves per CU
   const bool direct = c->obs == KD_OBS_DIRECT;
   const int li0 = !direct                           ? 0
+                  : kd::ctree_supported(c->order)    ? -5
                   : kd::wsplit_supported(c->order)   ? -4
                   : kd::split_threads(c->order) > 0 ? -3
                   : c->order >= 7                   ? -2
                   : c->order >= 6                   ? -1
                                                     : 0;
   for (int li = li0; li < (direct ? 3 : 1) && !rc; li++) {
+    if (li == -4 && !kd::wsplit_supported(c->order)) continue;
     if (li == -3 && kd::split_threads(c->order) == 0) continue;
     if (li == -2 && c->order < 7) continue;
     const int L = li < 0 ? 64 : lane_opts[li];
     if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
     for (int wi = 0; wi < 3 && !rc; wi++) {
-      c->algo = li == -4   ? KD_ALGO_WSPLIT
+      c->algo = li == -5   ? KD_ALGO_CTREE
+                : li == -4 ? KD_ALGO_WSPLIT
                 : li == -3 ? KD_ALGO_SPLIT
                 : li == -2 ? KD_ALGO_PAIR
                 : li == -1 ? KD_ALGO_TREE
@@ -1431,7 +1484,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   c->lu_waves_per_cu = best_w;
   c->algo = best_a;
   c->tune_ms_per_mpt = best * 1048576.f / (float)sample_points;
-  if (lanes) *lanes = use_tree(c) || use_split(c) || use_wsplit(c) ? 0 : kd::lu_lanes(c->order, best_l);
+  if (lanes) *lanes = use_tree(c) || use_split(c) || use_wsplit(c) || use_ctree(c) ? 0 : kd::lu_lanes(c->order, best_l);
   if (waves_per_cu) *waves_per_cu = best_w;
   if (ms_per_mpoint) *ms_per_mpoint = c->tune_ms_per_mpt;
   return KD_OK;
@@ -1449,7 +1502,7 @@ int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
 int kd_set_algorithm(kd_ctx *c, int32_t algo) {
   if (!c) return KD_ERR_ARG;
   if (algo != KD_ALGO_DEFAULT && algo != KD_ALGO_CONFIG && algo != KD_ALGO_TREE && algo != KD_ALGO_PAIR &&
-      algo != KD_ALGO_SPLIT && algo != KD_ALGO_WSPLIT)
+      algo != KD_ALGO_SPLIT && algo != KD_ALGO_WSPLIT && algo != KD_ALGO_CTREE)
     return fail(c, KD_ERR_ARG, "unknown Keldysh-sum algorithm");
   c->algo = algo;
   return KD_OK;
@@ -1460,6 +1513,7 @@ int kd_get_algorithm(const kd_ctx *c, int32_t *algo) {
   *algo = use_tree(c)     ? KD_ALGO_TREE
           : use_pair(c)   ? KD_ALGO_PAIR
           : use_split(c)  ? KD_ALGO_SPLIT
+          : use_ctree(c)  ? KD_ALGO_CTREE
           : use_wsplit(c) ? KD_ALGO_WSPLIT
                           : KD_ALGO_CONFIG;
   return KD_OK;

@@ -1,0 +1,494 @@
+// kd_ctree.hip -- the Keldysh sum by the time-ordered tree (orders 6..8), one
+// point per wavefront.
+//
+// In the contour order of compare_3way (common.cpp:47-57), g0 between two
+// vertices depends only on the Keldysh index of the earlier one (earlier time,
+// or the lower index at equal times): for vertex i after vertex j,
+//     g0(i a; j b) = b ? g0^<(t_i - t_j) : g0^>(t_i - t_j)     for both a,
+//     g0(j b; i a) = b ? g0^>(t_j - t_i) : g0^<(t_j - t_i)     for both a,
+// and the diagonal g0(i a; i a) = g0^<(0) - i alpha is one constant D.  With
+// the vertices sorted by (time, index), entry (i, j) of configuration sigma's
+// bordered matrix (wick_direct.cpp:92-101 picks) is W(i, j, sigma_min(i,j)),
+// the border column W(i, e, sigma_i), the border row W(e, j, sigma_j).
+// Eliminating vertex k first to last, with pivot p = D - C[k][k], subtracts
+//     u_i w_j,   u_i = W(i, k, s) - C[i][k],   w_j = (W(k, j, s) - C[k][j]) / p
+// from every remaining entry (i, j): u and w do not depend on the Keldysh
+// indices of the vertices after k, so the Schur complement after vertices
+// 0..k-1 is W - C with a correction C that depends on sigma_0..sigma_{k-1}
+// only -- one value per vertex pair instead of the four of the interleaved
+// Wick matrix.  The tree of these corrections over the 2^n configurations
+// holds one (n-k+1)^2 table per node of level k; a leaf (all n vertices
+// eliminated) gives det of the vertex block = the product of its pivots and the
+// bordered determinant = that product times 0 - C[e][e], so the configuration's
+// term (-1)^|sigma| det(A)^2 s.
+//
+// Layout per wave (one point at a time): the sorted times, the vertex-level
+// table Wv in LDS (FP64); levels 1..3 of the tree breadth first in LDS, one
+// (child, row) item per lane and level; below level 3 each lane follows its own
+// path -- sigma_0..sigma_5 are the lane's bits -- with the level-4 and level-5
+// entries formed on demand from the level-3 table and the rank-1 factors of the
+// levels between (each entry once), then the subtree over the vertices after
+// the sixth in registers, its leaves summed per lane.
+//
+// Pivot check and fallback as in the other Keldysh-sum kernels: every pivot on
+// a configuration's path against PivotTol times the largest |Re|+|Im| of the
+// point's vertex block; a flagged configuration is recomputed by
+// partial-pivoting FP64 LU of its matrix from Wv.
+#include "kd_device.hpp"
+#include "kd_kernels.hpp"
+#include "kd_lu.hpp"
+#include "kd_pivoted.hpp"
+#include "kd_split_lu.hpp"
+#include "kd_wave.hpp"
+
+namespace kd {
+
+// a + l b
+template <typename T> __device__ __forceinline__ cx<T> cfma(cx<T> a, cx<T> l, cx<T> b) {
+  return cmk<T>(a.x + l.x * b.x - l.y * b.y, a.y + l.x * b.y + l.y * b.x);
+}
+template <typename T> __device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return cmk<T>(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 dmul(double2 d, double2 p) {
+  return make_double2(d.x * p.x - d.y * p.y, d.x * p.y + d.y * p.x);
+}
+template <typename T> __device__ __forceinline__ double2 dcx(cx<T> v) { return make_double2((double)v.x, (double)v.y); }
+
+template <int N> struct CShape {
+  static constexpr int WS = N + 1;                           // Wv row stride: vertices, then the border e = N
+  static constexpr int R(int k) { return N - k + 1; }        // a level-k table: vertices k..N-1 and the border
+  static constexpr int NS(int k) { return (R(k) * R(k)) | 1; } // node stride (odd: nodes spread over the banks)
+  static constexpr int sizeA() { return 2 * NS(1) > 8 * NS(3) ? 2 * NS(1) : 8 * NS(3); } // levels 1 and 3
+  static constexpr int sizeB() { return 4 * NS(2); }                                     // level 2
+  // global index of local row/column a of a level-k table
+  static constexpr int gidx(int k, int a) { return a < R(k) - 1 ? k + a : N; }
+};
+
+template <int N> __device__ __forceinline__ const double2 &wv(const double2 *Wv, int i, int j, int s) {
+  return Wv[(i * (N + 1) + j) * 2 + s];
+}
+
+// Level K+1 of the tree from level K (K = 0: the root, whose correction is 0):
+// child m has parent g = m mod 2^K and sigma_K = m >> K; one (child, row) item
+// per lane, all columns of the row.  The item of a child's row 0 also extends
+// the path determinant and pivot flag (heap index (1 << k) + bits for level k).
+template <int N, int K, typename T>
+__device__ __forceinline__ void clevel(const cx<T> *__restrict__ par, cx<T> *__restrict__ ch,
+                                       const double2 *__restrict__ Wv, cx<T> D, T thr, double2 *__restrict__ dpath,
+                                       int *__restrict__ fpath, int lane) {
+  using S = CShape<N>;
+  constexpr int r = S::R(K), rc = r - 1;
+  constexpr int NIT = (2 << K) * rc;
+  for (int it = lane; it < NIT; it += 64) {
+    const int m = it / rc, a = it - m * rc;
+    const int g = m & ((1 << K) - 1), s = m >> K;
+    const int ig = a < rc - 1 ? K + 1 + a : N;
+    const cx<T> *P = par + g * S::NS(K);
+    cx<T> piv = D, u = ldx<T>(wv<N>(Wv, ig, K, s));
+    cx<T> wr[rc], cr[rc];
+    if constexpr (K > 0) {
+      piv = csub<T>(D, P[0]);
+      u = csub<T>(u, P[(a + 1) * r]);
+    }
+    static_for<0, rc>([&](auto B) {
+      constexpr int b = B;
+      wr[b] = ldx<T>(wv<N>(Wv, K, S::gidx(K + 1, b), s));
+      if constexpr (K > 0) {
+        wr[b] = csub<T>(wr[b], P[b + 1]);
+        cr[b] = P[(a + 1) * r + b + 1];
+      }
+    });
+    const cx<T> l = cmul<T>(u, cinv<T>(piv));
+    cx<T> *C = ch + m * S::NS(K + 1) + a * rc;
+    static_for<0, rc>([&](auto B) {
+      constexpr int b = B;
+      if constexpr (K > 0)
+        C[b] = cfma<T>(cr[b], l, wr[b]);
+      else
+        C[b] = cmul<T>(l, wr[b]);
+    });
+    if (a == 0) {
+      const int hp = (1 << K) + g, hc = (2 << K) + m;
+      dpath[hc] = dmul(dpath[hp], dcx<T>(piv));
+      fpath[hc] = fpath[hp] | (int)!(cabs1<T>(piv) >= thr);
+    }
+  }
+  wave_sync();
+}
+
+// A lane's path below its level-3 node: the entries of its level-k tables
+// (k = 3..6) on demand -- level 3 from LDS, level k > 3 as level k-1's entry
+// plus the rank-1 term of vertex k-1 -- and the rank-1 factors u, w of the
+// vertices 3..5 along the lane's Keldysh choices.
+template <int N, typename T> struct LanePath {
+  using S = CShape<N>;
+  static constexpr int R3 = S::R(3);
+  const cx<T> *P3;
+  cx<T> u[3][R3 - 1], w[3][R3 - 1];
+  template <int K, int A, int B> __device__ __forceinline__ cx<T> at() const {
+    if constexpr (K == 3)
+      return ldx<T>(P3[A * R3 + B]);
+    else
+      return cfma<T>(at<K - 1, A + 1, B + 1>(), u[K - 4][A], w[K - 4][B]);
+  }
+  // vertex K eliminated with Keldysh index s (K < N - 1): its factors; returns the pivot
+  template <int K> __device__ __forceinline__ cx<T> step(const double2 *__restrict__ Wv, cx<T> D, int s) {
+    constexpr int r = S::R(K);
+    const cx<T> piv = csub<T>(D, at<K, 0, 0>());
+    const cx<T> inv = cinv<T>(piv);
+    static_for<0, r - 1>([&](auto A) {
+      constexpr int a = A;
+      constexpr int ig = S::gidx(K + 1, a);
+      u[K - 3][a] = csub<T>(ldx<T>(wv<N>(Wv, ig, K, s)), at<K, a + 1, 0>());
+      w[K - 3][a] = cmul<T>(csub<T>(ldx<T>(wv<N>(Wv, K, ig, s)), at<K, 0, a + 1>()), inv);
+    });
+    return piv;
+  }
+};
+
+// One leaf: vertex K = N-1 eliminated with index s from its 2x2 table
+// {(K,K), (K,e), (e,K), (e,e)} = c00, c0e, ce0, cee; term
+//   (-1)^|sigma| (dp p)^2 (0 - cee - ue ve / p) = -(-1)^|sigma| dp^2 p (p cee + ue ve).
+template <int N, typename T>
+__device__ __forceinline__ void leaf(const double2 *__restrict__ Wv, cx<T> D, T thr, cx<T> c00, cx<T> c0e, cx<T> ce0,
+                                     cx<T> cee, int s, double2 D2, int flag, int cfg, int bit, double &sre,
+                                     double &sim, uint32_t &bad) {
+  constexpr int K = N - 1;
+  const cx<T> piv = csub<T>(D, c00);
+  const cx<T> ue = csub<T>(ldx<T>(wv<N>(Wv, N, K, s)), ce0);
+  const cx<T> ve = csub<T>(ldx<T>(wv<N>(Wv, K, N, s)), c0e);
+  const cx<T> x = cmul<T>(piv, cfma<T>(cmul<T>(piv, cee), ue, ve));
+  const double2 v = dmul(D2, dcx<T>(x));
+  const bool b = flag || !(cabs1<T>(piv) >= thr);
+  const double sg = (__popc(cfg) & 1) ? 1.0 : -1.0; // -(-1)^|sigma|
+  sre += b ? 0.0 : sg * v.x;
+  sim += b ? 0.0 : sg * v.y;
+  bad |= (uint32_t)b << bit;
+}
+
+// The subtree of the lane's level-K node (R = N - K + 1, table C) over all
+// choices of vertices K..N-1.  cfg holds sigma_0..sigma_{K-1}; leaf j of the
+// subtree (sigma_6.. = bits of j) sets bit j of bad when flagged.
+template <int N, int R, typename T>
+__device__ __forceinline__ void subtree(const cx<T> (&C)[R][R], const double2 *__restrict__ Wv, cx<T> D, T thr,
+                                        double2 dp, int flag, int cfg, double &sre, double &sim, uint32_t &bad) {
+  constexpr int K = N + 1 - R;
+  if constexpr (R == 2) {
+    const double2 D2 = dmul(dp, dp);
+    static_for<0, 2>([&](auto Sx) {
+      constexpr int s = Sx;
+      const int c = cfg | (s << K);
+      leaf<N, T>(Wv, D, thr, C[0][0], C[0][1], C[1][0], C[1][1], s, D2, flag, c, c >> 6, sre, sim, bad);
+    });
+  } else {
+    static_for<0, 2>([&](auto Sx) {
+      constexpr int s = Sx;
+      const cx<T> piv = csub<T>(D, C[0][0]);
+      const cx<T> inv = cinv<T>(piv);
+      cx<T> u[R - 1], w[R - 1], ch[R - 1][R - 1];
+      static_for<0, R - 1>([&](auto A) {
+        constexpr int a = A;
+        constexpr int ig = CShape<N>::gidx(K + 1, a);
+        u[a] = csub<T>(ldx<T>(wv<N>(Wv, ig, K, s)), C[a + 1][0]);
+        w[a] = cmul<T>(csub<T>(ldx<T>(wv<N>(Wv, K, ig, s)), C[0][a + 1]), inv);
+      });
+      static_for<0, R - 1>([&](auto A) {
+        static_for<0, R - 1>([&](auto B) { ch[A][B] = cfma<T>(C[A + 1][B + 1], u[A], w[B]); });
+      });
+      subtree<N, R - 1, T>(ch, Wv, D, thr, dmul(dp, dcx<T>(piv)), flag | (int)!(cabs1<T>(piv) >= thr),
+                           cfg | (s << K), sre, sim, bad);
+    });
+  }
+}
+
+#ifndef KD_CTREE_WAVES_PER_EU
+#define KD_CTREE_WAVES_PER_EU 3
+#endif
+template <int N, typename T>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KD_CTREE_WAVES_PER_EU))) void
+integrand_ctree_kernel(KdParams p, PrepBuf buf) {
+  static_assert(N >= 6 && N <= 8, "time-ordered tree: orders 6..8");
+  using S = CShape<N>;
+  constexpr int WS = S::WS;
+  __shared__ double2 Wv[WS * WS * 2];
+  __shared__ cx<T> LA[S::sizeA()], LB[S::sizeB()];
+  __shared__ double2 dpath[16];
+  __shared__ int fpath[16];
+  __shared__ double Us[N], Ts[N];
+  extern __shared__ double2 sbins[];
+
+  const int lane = threadIdx.x;
+  if (p.nbins > 0) {
+    for (int i = lane; i < p.nbins; i += 64) sbins[i] = make_double2(0.0, 0.0);
+    wave_sync();
+  }
+  // the diagonal D = g0^<(0) - i alpha and the constant part of the vertex
+  // block's scale (|D| and the same-vertex entries g0^<(0), g0^>(0))
+  const double2 gl0 = table_eval(p, p.lesser, 0, 0, 0.0), gg0 = table_eval(p, p.greater, 0, 0, 0.0);
+  const double2 Dd = make_double2(gl0.x, gl0.y - p.alpha);
+  const double wconst = fmax(fmax(cabs1<double>(gl0), cabs1<double>(gg0)), cabs1<double>(Dd));
+  const cx<T> D = cmk<T>((T)Dd.x, (T)Dd.y);
+
+  const int64_t nbatch = (p.n_points + 63) / 64;
+  double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
+  const bool lane_sums = p.mode == MODE_RUN && p.nbins == 0 && !(p.cutoff > 0.0);
+  // Work items and the window scan as in kd_wsplit.hip: one compacted point per
+  // item, numbered slot-major (item = grp * nbatch + b), every gridDim.x-th
+  // item to this wave; lane l of a window loads the batch count of the
+  // window's item l and a ballot keeps the live ones.
+  const int64_t G = gridDim.x;
+  int wgrp = nbatch > 0 ? (int)(blockIdx.x / nbatch) : 64;
+  int64_t wb = blockIdx.x - (int64_t)wgrp * nbatch;
+  uint64_t mask = 0;
+  int gl = 0, bl = 0;
+  double u_n = 0.0, jac_n = 0.0;
+  for (;;) {
+    if (!mask) {
+      if (wgrp >= 64) break;
+      const int64_t bw = wb + (int64_t)lane * G;
+      gl = wgrp + (int)(bw / nbatch);
+      bl = (int)(bw % nbatch);
+      mask = __ballot(gl < 64 && gl < buf.cnt[bl]);
+      const int64_t adv = wb + 64 * G;
+      wgrp += (int)(adv / nbatch);
+      wb = adv % nbatch;
+      if (!mask) continue;
+      const int j1 = __builtin_ctzll(mask);
+      const int64_t s1 = (int64_t)rdlane(bl, j1) * 64 + rdlane(gl, j1);
+      u_n = buf.u[s1 * N + (lane < N ? lane : N - 1)];
+      jac_n = buf.jac[s1];
+    }
+    const int j = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    const int g = rdlane(gl, j);
+    const int64_t b = rdlane(bl, j);
+    const int64_t slot = b * 64 + g;
+    const double uc = u_n, jj = jac_n;
+    int lq = lane;
+    __asm__ volatile("" : "+v"(lq));
+    if (lq < N) Us[lq] = uc;
+    if (lq == 0) {
+      dpath[1] = make_double2(1.0, 0.0);
+      fpath[1] = 0;
+    }
+    wave_sync();
+    // the next live item's record, in flight until the next point
+    if (mask) {
+      const int j2 = __builtin_ctzll(mask);
+      const int64_t s2 = (int64_t)rdlane(bl, j2) * 64 + rdlane(gl, j2);
+      u_n = buf.u[s2 * N + (lane < N ? lane : N - 1)];
+      jac_n = buf.jac[s2];
+    }
+    // sort by (time, index): the rank of vertex i among the others (NaN last,
+    // so that the ranks stay a permutation and the lookups yield NaN)
+    if (lq < N) {
+      const double kt = isnan(uc) ? __builtin_inf() : uc;
+      int rank = 0;
+#pragma unroll
+      for (int q = 0; q < N; q++) {
+        const double tq = Us[q];
+        const double kq = isnan(tq) ? __builtin_inf() : tq;
+        rank += (kq < kt) || (kq == kt && q < lq);
+      }
+      Ts[rank] = uc;
+    }
+    wave_sync();
+    // Wv: item (a, c) = a N + c; a < c: the vertex pair (a earlier, c later),
+    // g0^< and g0^> at d = t_c - t_a and at -d; a == c: vertex a's border
+    // column and row (compare_3way with the external points, no alpha: the
+    // points differ); a > c: nothing
+    double wmax = 0.0;
+    static_for<0, (N * N + 63) / 64>([&](auto RI) {
+      const int q = lq + 64 * (int)RI;
+      const int a = q / N, c = q - a * N;
+      if (q < N * N && a <= c) {
+        const double ta = Ts[a], tc = Ts[c];
+        const double2 *tab[4];
+        TabPos pos[4];
+        if (a < c) {
+          const TabPos p1 = table_pos(p, tc - ta), p2 = table_pos(p, ta - tc);
+          tab[0] = p.lesser;
+          tab[1] = p.greater;
+          tab[2] = p.lesser;
+          tab[3] = p.greater;
+          pos[0] = p1;
+          pos[1] = p1;
+          pos[2] = p2;
+          pos[3] = p2;
+        } else {
+          // (a s; B) for s = 0, 1 and (A; a s) for s = 0, 1
+#pragma unroll
+          for (int h = 0; h < 4; h++) {
+            const int s = h & 1;
+            CPt x, y;
+            if (h < 2) {
+              x = CPt{ta, s, 0, 0};
+              y = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
+            } else {
+              x = CPt{p.A.time, p.A.k_idx, N, p.A.orbital};
+              y = CPt{ta, s, 0, 0};
+            }
+            const int ord = compare_3way(x, y);
+            tab[h] = (ord > 0 ? p.greater : p.lesser) + (size_t)(x.orb * p.n_orb + y.orb) * p.n_pts;
+            pos[h] = table_pos(p, x.time - y.time);
+          }
+        }
+        double2 e0[4], e1[4];
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+          e0[h] = tab[h][pos[h].i];
+          e1[h] = tab[h][pos[h].i + 1];
+        }
+        double2 v[4];
+#pragma unroll
+        for (int h = 0; h < 4; h++) v[h] = table_mix(e0[h], e1[h], pos[h]);
+        if (a < c) {
+          // later row c, earlier column a: index of a picks greater (0) / lesser (1);
+          // earlier row a, later column c: lesser (0) / greater (1)
+          Wv[(c * WS + a) * 2 + 0] = v[1];
+          Wv[(c * WS + a) * 2 + 1] = v[0];
+          Wv[(a * WS + c) * 2 + 0] = v[2];
+          Wv[(a * WS + c) * 2 + 1] = v[3];
+#pragma unroll
+          for (int h = 0; h < 4; h++) wmax = fmax(wmax, cabs1<double>(v[h]));
+        } else {
+          Wv[(a * WS + N) * 2 + 0] = v[0];
+          Wv[(a * WS + N) * 2 + 1] = v[1];
+          Wv[(N * WS + a) * 2 + 0] = v[2];
+          Wv[(N * WS + a) * 2 + 1] = v[3];
+        }
+      }
+    });
+    wmax = fmax(wave_max(wmax), wconst);
+    const T thr = PivotTol<T>::v * fmax((T)wmax, PivotTol<T>::floor);
+    wave_sync();
+
+    // levels 1..3 breadth first (level 1 and 3 in LA, level 2 in LB)
+    clevel<N, 0, T>(nullptr, LA, Wv, D, thr, dpath, fpath, lq);
+    clevel<N, 1, T>(LA, LB, Wv, D, thr, dpath, fpath, lq);
+    clevel<N, 2, T>(LB, LA, Wv, D, thr, dpath, fpath, lq);
+
+    // the lane's path: sigma_0..sigma_5 = bits of the lane
+    double sre = 0.0, sim = 0.0;
+    uint32_t bad = 0;
+    {
+      const int g3 = lq & 7;
+      LanePath<N, T> lp;
+      lp.P3 = LA + g3 * S::NS(3);
+      double2 dp = dpath[8 + g3];
+      int flag = fpath[8 + g3];
+      static_for<3, 6>([&](auto KI) {
+        constexpr int k = KI;
+        if constexpr (k < N - 1) {
+          const cx<T> piv = lp.template step<k>(Wv, D, (lq >> k) & 1);
+          dp = dmul(dp, dcx<T>(piv));
+          flag |= (int)!(cabs1<T>(piv) >= thr);
+        }
+      });
+      if constexpr (N == 6) {
+        // vertex 5 is the last: the lane's own configuration
+        leaf<N, T>(Wv, D, thr, lp.template at<5, 0, 0>(), lp.template at<5, 0, 1>(), lp.template at<5, 1, 0>(),
+                   lp.template at<5, 1, 1>(), (lq >> 5) & 1, dmul(dp, dp), flag, lq, 0, sre, sim, bad);
+      } else {
+        constexpr int R6 = S::R(6);
+        cx<T> C6[R6][R6];
+        static_for<0, R6>([&](auto A) {
+          static_for<0, R6>([&](auto B) { C6[A][B] = lp.template at<6, A, B>(); });
+        });
+        subtree<N, R6, T>(C6, Wv, D, thr, dp, flag, lq, sre, sim, bad);
+      }
+    }
+    // flagged configurations: partial-pivoting FP64 LU of their matrices
+    int nfb = 0;
+    while (bad) {
+      const int jb = __builtin_ctz(bad);
+      bad &= bad - 1;
+      const int cfg = lq | (jb << 6);
+      const double2 v = config_value_pivoted_sorted(Wv, cfg, N, Dd);
+      const double sg = (__popc(cfg) & 1) ? -1.0 : 1.0;
+      sre += sg * v.x;
+      sim += sg * v.y;
+      nfb++;
+    }
+    acc_fb += (double)nfb;
+    if (lane_sums) {
+      acc_re += sre * jj;
+      acc_im += sim * jj;
+      acc_dom += lane == 0 ? 1.0 : 0.0;
+      wave_sync(); // Us, Ts, Wv and the tables are rewritten by the next point
+      continue;
+    }
+    sre = wave_sum(sre);
+    sim = wave_sum(sim);
+    if (lane == 0) {
+      if (sqrt(sre * sre + sim * sim) < p.cutoff) {
+        sre = 0.0;
+        sim = 0.0;
+      }
+      sre *= jj;
+      sim *= jj;
+      if (p.mode == MODE_RUN) {
+        acc_re += sre;
+        acc_im += sim;
+        acc_dom += 1.0;
+        if (p.nbins > 0) {
+          const int bi = bin_index(p, Ts[0]);
+          if (bi >= 0) {
+            sbins[bi].x += sre;
+            sbins[bi].y += sim;
+          }
+        }
+      } else {
+        const int64_t idx = b * 64 + buf.idx[slot];
+        p.out_vals[2 * idx] = sre;
+        p.out_vals[2 * idx + 1] = sim;
+        if (p.out_dom) p.out_dom[idx] = 1;
+        if (p.out_jac) p.out_jac[idx] = jj;
+      }
+    }
+    wave_sync();
+  }
+
+  if (p.mode == MODE_RUN) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc_fb += __shfl_xor(acc_fb, off, 64);
+    acc_re = wave_sum(acc_re);
+    acc_im = wave_sum(acc_im);
+    if (lane == 0) {
+      double *o = p.partials + 4 * (int64_t)blockIdx.x;
+      o[0] = acc_re;
+      o[1] = acc_im;
+      o[2] = acc_dom;
+      o[3] = acc_fb;
+    }
+    if (p.nbins > 0) {
+      wave_sync();
+      for (int i = lane; i < p.nbins; i += 64) p.bin_part[(int64_t)blockIdx.x * p.nbins + i] = sbins[i];
+    }
+  }
+}
+
+template <int N, typename T>
+static hipError_t launch_ctree_n(const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
+  const size_t shm = p.nbins > 0 ? sizeof(double2) * (size_t)p.nbins : 0;
+  hipLaunchKernelGGL((integrand_ctree_kernel<N, T>), dim3(grid), dim3(64), shm, st, p, buf);
+  return hipGetLastError();
+}
+
+bool ctree_supported(int order) { return order >= 6 && order <= 8; }
+
+hipError_t launch_ctree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
+  const bool f32 = lu_is_fp32(order, precision);
+#define KD_CTREE_CASE(n)                                                                                          \
+  case n:                                                                                                         \
+    return f32 ? launch_ctree_n<n, float>(p, buf, grid, st) : launch_ctree_n<n, double>(p, buf, grid, st);
+  switch (order) {
+    KD_CTREE_CASE(6)
+    KD_CTREE_CASE(7)
+    KD_CTREE_CASE(8)
+  default:
+    return hipErrorInvalidValue;
+  }
+#undef KD_CTREE_CASE
+}
+
+} // namespace kd

@@ -56,7 +56,8 @@ PILOT_REPLICAS = 128
 SELECT_MARGIN = 0.8  # a flow is selected when its pilot score is below this fraction of the plain one
 # committed rocprofv3 --pmc summaries holding "pmc-traffic <kernel> ..." lines
 PMC_SUMMARIES = [os.path.join(ROOT, "profiles", f) for f in
-                 ("integrand_wsplit_kernel_n8_fp64_pmc.md", "integrand_split_kernel_n8_fp64_pmc.md",
+                 ("integrand_ctree_kernel_n8_fp64_pmc.md", "integrand_wsplit_kernel_n8_fp64_pmc.md",
+                  "integrand_split_kernel_n8_fp64_pmc.md",
                   "integrand_kernel_n8_fp64_pmc.md")]
 
 

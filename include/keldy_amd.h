@@ -50,8 +50,8 @@ extern "C" {
 #define KD_PREC_FP32_LU 1
 #define KD_PREC_FP32_LU_ALL 2
 
-/* Keldysh-sum algorithm.  KD_ALGO_DEFAULT (a new ctx's setting): KD_ALGO_WSPLIT
- * at orders 6..10, KD_ALGO_CONFIG otherwise -- the fastest on MI355X for
+/* Keldysh-sum algorithm.  KD_ALGO_DEFAULT (a new ctx's setting): KD_ALGO_CTREE
+ * at orders 6..8, KD_ALGO_WSPLIT at 9 and 10, KD_ALGO_CONFIG otherwise -- the fastest on MI355X for
  * full-size launches; kd_autotune measures every candidate instead.
  * KD_ALGO_CONFIG: one bordered LU per
  * configuration sigma (2^n of them, sharing the staged Wick matrix).

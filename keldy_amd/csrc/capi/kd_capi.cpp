@@ -81,7 +81,7 @@ struct kd_ctx {
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
   // point and waves per CU of the persistent grid
   int lu_lanes_max = 64, lu_waves_per_cu = 16;
-  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: wave-per-point split at orders 6..10)
+  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: time-ordered tree at orders 6..8, wave-per-point split at 9..10)
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
   // time binning (kd_set_time_bins)
@@ -155,15 +155,18 @@ bool use_pair(const kd_ctx *c) {
 }
 
 // the split tree + per-configuration LU with one point per wave (orders 6..10;
-// the default there)
+// the default at orders 9 and 10)
 bool use_wsplit(const kd_ctx *c) {
-  const bool chosen = c->algo == KD_ALGO_WSPLIT || c->algo == KD_ALGO_DEFAULT;
+  const bool chosen =
+      c->algo == KD_ALGO_WSPLIT || (c->algo == KD_ALGO_DEFAULT && !kd::ctree_supported(c->order));
   return chosen && c->obs == KD_OBS_DIRECT && kd::wsplit_supported(c->order);
 }
 
-// the time-ordered tree of vertex-level corrections, one point per wave (orders 6..8)
+// the time-ordered tree of vertex-level corrections, one point per wave (orders
+// 6..8; the default there)
 bool use_ctree(const kd_ctx *c) {
-  return c->algo == KD_ALGO_CTREE && c->obs == KD_OBS_DIRECT && kd::ctree_supported(c->order);
+  const bool chosen = c->algo == KD_ALGO_CTREE || c->algo == KD_ALGO_DEFAULT;
+  return chosen && c->obs == KD_OBS_DIRECT && kd::ctree_supported(c->order);
 }
 
 // the split tree + per-configuration LU with one point per workgroup (orders 6..10)

@@ -59,13 +59,18 @@ struct kd_ctx {
   size_t fpart_cap = 0;
   double *d_floss = nullptr; // [loss_part blocks | per-step losses]
   size_t floss_cap = 0;
-  // Sobol
+  // Sobol: one device buffer for the largest dimension (d_shfr points into it),
+  // filled from the pinned staging buffer h_sobol by copies on the ctx stream
   uint32_t *d_Csr = nullptr, *d_shfr = nullptr;
+  uint32_t *h_sobol = nullptr;
+  hipEvent_t ev_sobol = nullptr; // recorded after the last staging copy
+  bool sobol_pending = false;
   int sobol_dim = 0;
   uint32_t k = 0;
   bool has_sobol = false;
   // accumulators
   double *d_acc = nullptr; // re, im, n_dom, n_fallback
+  double *h_acc = nullptr; // pinned copy of d_acc
   double *d_part = nullptr;
   int part_cap = 0;
   uint64_t n_points = 0;
@@ -79,8 +84,8 @@ struct kd_ctx {
   float last_ms = 0.f, last_lu_ms = 0.f, last_prep_ms = 0.f;
   int last_launches = 0, last_chunks = 0;
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
-  // point and waves per CU of the persistent grid
-  int lu_lanes_max = 64, lu_waves_per_cu = 16;
+  // point and waves per CU of the persistent grid (0: the algorithm's default)
+  int lu_lanes_max = 64, lu_waves_per_cu = 0;
   int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: time-ordered tree at orders 6..8, wave-per-point split at 9..10)
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
@@ -179,6 +184,19 @@ int64_t vals_per_point(const kd_ctx *c) {
   return c->obs == KD_OBS_KERNEL ? (int64_t(1) << c->order) * c->order : 1;
 }
 
+// Waves per CU of the persistent Keldysh-sum grid: the tuned or set value, else
+// the algorithm's default.  The one-point-per-wave kernels give each wave a
+// fixed share of the points, so a grid of several times the waves that fit at
+// once lets the dispatcher balance the shares across the CUs (order 8, the
+// time-ordered tree: 5.50 ms per 2^21 points at 64 waves per CU, 6.24 ms at
+// 16; the wave-per-point split: 9.7 ms at 24-32, 9.9 ms at 16).
+int grid_waves_per_cu(const kd_ctx *c) {
+  if (c->lu_waves_per_cu > 0) return c->lu_waves_per_cu;
+  if (use_ctree(c)) return 64;
+  if (use_wsplit(c)) return 32;
+  return 16;
+}
+
 // LU-kernel grid: one workgroup per group of points (64 / lanes points per
 // group), capped at lu_waves_per_cu waves per CU -- the kernel is persistent
 // over the rest.
@@ -188,7 +206,7 @@ int grid_for(const kd_ctx *c, int64_t n_points) {
   int64_t nb = (n_points + 63) / 64 * per_batch;
   // waves per CU -> workgroups per CU (the split kernel's workgroups hold several waves)
   const int wg_waves = use_split(c) ? std::max(1, kd::split_threads(c->order) / 64) : 1;
-  int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * std::max(1, c->lu_waves_per_cu / wg_waves);
+  int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * std::max(1, grid_waves_per_cu(c) / wg_waves);
   if (nb > cap) nb = cap;
   return (int)(nb < 1 ? 1 : nb);
 }
@@ -495,6 +513,7 @@ int kd_create(kd_ctx **out, int device) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc(&c->d_acc, 4 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&c->h_acc, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipMemset(c->d_acc, 0, 4 * sizeof(double)) != hipSuccess) {
     kd_destroy(c);
     return KD_ERR_HIP;
@@ -527,7 +546,13 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_bin_part);
   dfree(c->d_bin_counts);
   dfree(c->d_Csr);
-  dfree(c->d_shfr);
+  c->d_shfr = nullptr;
+  if (c->h_sobol) (void)hipHostFree(c->h_sobol);
+  c->h_sobol = nullptr;
+  if (c->h_acc) (void)hipHostFree(c->h_acc);
+  c->h_acc = nullptr;
+  if (c->ev_sobol) (void)hipEventDestroy(c->ev_sobol);
+  c->ev_sobol = nullptr;
   dfree(c->d_acc);
   dfree(c->d_part);
   dfree(c->d_stage);
@@ -778,19 +803,32 @@ int kd_eval_warped_batch(kd_ctx *c, const double *l, int64_t n_points, int32_t k
   return finish_timing(c);
 }
 
+constexpr int kSobolMaxDim = 32;
+constexpr size_t kSobolWords = 31 * kSobolMaxDim + kSobolMaxDim;
+
 int kd_set_sobol(kd_ctx *c, int32_t dim, const uint32_t *Csr, const uint32_t *shfr, uint32_t k) {
   if (!c) return KD_ERR_ARG;
   if (dim <= 0) return fail(c, KD_ERR_ARG, "Dimension of the integration space must be > 0.");
-  if (dim > 32) return fail(c, KD_ERR_ARG, "Sobol dimension must be <= 32");
+  if (dim > kSobolMaxDim) return fail(c, KD_ERR_ARG, "Sobol dimension must be <= 32");
   if (!Csr || !shfr) return fail(c, KD_ERR_ARG, "null Sobol table");
   KD_HIP(c, hipSetDevice(c->device));
-  KD_HIP(c, hipStreamSynchronize(c->stream));
-  dfree(c->d_Csr);
-  dfree(c->d_shfr);
-  KD_HIP(c, hipMalloc(&c->d_Csr, sizeof(uint32_t) * 31 * dim));
-  KD_HIP(c, hipMalloc(&c->d_shfr, sizeof(uint32_t) * dim));
-  KD_HIP(c, hipMemcpy(c->d_Csr, Csr, sizeof(uint32_t) * 31 * dim, hipMemcpyHostToDevice));
-  KD_HIP(c, hipMemcpy(c->d_shfr, shfr, sizeof(uint32_t) * dim, hipMemcpyHostToDevice));
+  // No host wait for the device: the new tables go in by copies on the ctx
+  // stream, behind every launch that reads the old ones.  Only the staging
+  // buffer's previous copy must have been consumed before it is refilled.
+  if (!c->d_Csr) {
+    KD_HIP(c, hipMalloc(&c->d_Csr, sizeof(uint32_t) * kSobolWords));
+    c->d_shfr = c->d_Csr + 31 * kSobolMaxDim;
+  }
+  if (!c->h_sobol) KD_HIP(c, hipHostMalloc(&c->h_sobol, sizeof(uint32_t) * kSobolWords, hipHostMallocDefault));
+  if (!c->ev_sobol) KD_HIP(c, hipEventCreateWithFlags(&c->ev_sobol, hipEventDisableTiming));
+  if (c->sobol_pending) KD_HIP(c, hipEventSynchronize(c->ev_sobol));
+  std::memcpy(c->h_sobol, Csr, sizeof(uint32_t) * 31 * dim);
+  std::memcpy(c->h_sobol + 31 * kSobolMaxDim, shfr, sizeof(uint32_t) * dim);
+  KD_HIP(c, hipMemcpyAsync(c->d_Csr, c->h_sobol, sizeof(uint32_t) * 31 * dim, hipMemcpyHostToDevice, c->stream));
+  KD_HIP(c, hipMemcpyAsync(c->d_shfr, c->h_sobol + 31 * kSobolMaxDim, sizeof(uint32_t) * dim,
+                           hipMemcpyHostToDevice, c->stream));
+  KD_HIP(c, hipEventRecord(c->ev_sobol, c->stream));
+  c->sobol_pending = true;
   c->sobol_dim = dim;
   c->k = k;
   c->has_sobol = true;
@@ -846,8 +884,16 @@ int kd_run(kd_ctx *c, uint64_t n_steps, kd_accum *out) {
   if (!c) return KD_ERR_ARG;
   int rc = run_common(c, n_steps, kd::MODE_RUN, nullptr, nullptr);
   if (rc) return rc;
+  if (!out) return n_steps ? finish_timing(c) : KD_OK;
+  // the accumulator's copy queued behind the launches: one host wait for both
+  KD_HIP(c, hipMemcpyAsync(c->h_acc, c->d_acc, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  KD_HIP(c, hipStreamSynchronize(c->stream));
   if (n_steps && (rc = finish_timing(c))) return rc;
-  if (out) return kd_get_accum(c, out);
+  out->re = c->h_acc[0];
+  out->im = c->h_acc[1];
+  out->n_points = c->n_points;
+  out->n_in_domain = (uint64_t)c->h_acc[2];
+  out->n_fallback = (uint64_t)c->h_acc[3];
   return KD_OK;
 }
 
@@ -1427,7 +1473,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   if ((uint64_t)c->k + (uint64_t)sample_points > (1ull << 31))
     return fail(c, KD_ERR_RANGE, "More sample points requested then generator can hold.");
   KD_HIP(c, hipSetDevice(c->device));
-  const int lane_opts[3] = {64, 32, 16}, wave_opts[3] = {8, 16, 32};
+  const int lane_opts[3] = {64, 32, 16}, wave_opts[4] = {8, 16, 32, 64};
   const int old_l = c->lu_lanes_max, old_w = c->lu_waves_per_cu, old_a = c->algo;
   int best_l = old_l, best_w = old_w, best_a = old_a;
   float best = 1e30f;
@@ -1438,8 +1484,8 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   c->no_reduce = true;
   // candidates: the split tree + per-configuration LU (orders 6..10), the
   // paired LU (orders >= 7), the tree (orders >= 6; these at 64 lanes) and the
-  // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16
-  // or 32 waves per CU
+  // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16,
+  // 32 or 64 waves per CU
   const bool direct = c->obs == KD_OBS_DIRECT;
   const int li0 = !direct                           ? 0
                   : kd::ctree_supported(c->order)    ? -5
@@ -1454,7 +1500,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
     if (li == -2 && c->order < 7) continue;
     const int L = li < 0 ? 64 : lane_opts[li];
     if (li > 0 && kd::lu_lanes(c->order, L) == kd::lu_lanes(c->order, lane_opts[li - 1])) continue;
-    for (int wi = 0; wi < 3 && !rc; wi++) {
+    for (int wi = 0; wi < 4 && !rc; wi++) {
       c->algo = li == -5   ? KD_ALGO_CTREE
                 : li == -4 ? KD_ALGO_WSPLIT
                 : li == -3 ? KD_ALGO_SPLIT
@@ -1496,7 +1542,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
 int kd_set_launch_shape(kd_ctx *c, int32_t lanes, int32_t waves_per_cu) {
   if (!c) return KD_ERR_ARG;
   if (lanes != 16 && lanes != 32 && lanes != 64) return fail(c, KD_ERR_ARG, "lanes must be 16, 32 or 64");
-  if (waves_per_cu < 1 || waves_per_cu > 64) return fail(c, KD_ERR_ARG, "waves_per_cu must be in [1, 64]");
+  if (waves_per_cu < 1 || waves_per_cu > 128) return fail(c, KD_ERR_ARG, "waves_per_cu must be in [1, 128]");
   c->lu_lanes_max = lanes;
   c->lu_waves_per_cu = waves_per_cu;
   return KD_OK;
@@ -1525,7 +1571,7 @@ int kd_get_algorithm(const kd_ctx *c, int32_t *algo) {
 int kd_get_launch_shape(const kd_ctx *c, int32_t *lanes, int32_t *waves_per_cu) {
   if (!c) return KD_ERR_ARG;
   if (lanes) *lanes = kd::lu_lanes(c->order > 0 ? c->order : 0, c->lu_lanes_max);
-  if (waves_per_cu) *waves_per_cu = c->lu_waves_per_cu;
+  if (waves_per_cu) *waves_per_cu = grid_waves_per_cu(c);
   return KD_OK;
 }
 
@@ -1698,8 +1744,6 @@ int kd_pilot_histograms(kd_ctx *c, const uint32_t *Csr, const uint32_t *shfr, in
     const int r2 = kd_set_sobol(c, sdim, sCsr.data(), sShfr.data(), sk);
     if (!rc) rc = r2;
   } else {
-    dfree(c->d_Csr);
-    dfree(c->d_shfr);
     c->has_sobol = false;
     c->k = 0;
   }

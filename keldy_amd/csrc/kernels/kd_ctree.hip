@@ -73,8 +73,8 @@ template <int N> __device__ __forceinline__ const double2 &wv(const double2 *Wv,
 // the path determinant and pivot flag (heap index (1 << k) + bits for level k).
 template <int N, int K, typename T>
 __device__ __forceinline__ void clevel(const cx<T> *__restrict__ par, cx<T> *__restrict__ ch,
-                                       const double2 *__restrict__ Wv, cx<T> D, T thr, double2 *__restrict__ dpath,
-                                       int *__restrict__ fpath, int lane) {
+                                       const double2 *__restrict__ Wv, cx<T> D, cx<T> invD, T thr,
+                                       double2 *__restrict__ dpath, int *__restrict__ fpath, int lane) {
   using S = CShape<N>;
   constexpr int r = S::R(K), rc = r - 1;
   constexpr int NIT = (2 << K) * rc;
@@ -97,7 +97,7 @@ __device__ __forceinline__ void clevel(const cx<T> *__restrict__ par, cx<T> *__r
         cr[b] = P[(a + 1) * r + b + 1];
       }
     });
-    const cx<T> l = cmul<T>(u, cinv<T>(piv));
+    const cx<T> l = cmul<T>(u, K > 0 ? cinv<T>(piv) : invD);
     cx<T> *C = ch + m * S::NS(K + 1) + a * rc;
     static_for<0, rc>([&](auto B) {
       constexpr int b = B;
@@ -146,23 +146,18 @@ template <int N, typename T> struct LanePath {
 };
 
 // One leaf: vertex K = N-1 eliminated with index s from its 2x2 table
-// {(K,K), (K,e), (e,K), (e,e)} = c00, c0e, ce0, cee; term
-//   (-1)^|sigma| (dp p)^2 (0 - cee - ue ve / p) = -(-1)^|sigma| dp^2 p (p cee + ue ve).
+// {(K,K), (K,e), (e,K), (e,e)} = c00, c0e, ce0, cee.  Its term is
+//   (-1)^|sigma| (dp p)^2 (0 - cee - ue ve / p) = -(-1)^|sigma| dp^2 x,  x = p (p cee + ue ve);
+// returns x, and bad when p fails the pivot check.
 template <int N, typename T>
-__device__ __forceinline__ void leaf(const double2 *__restrict__ Wv, cx<T> D, T thr, cx<T> c00, cx<T> c0e, cx<T> ce0,
-                                     cx<T> cee, int s, double2 D2, int flag, int cfg, int bit, double &sre,
-                                     double &sim, uint32_t &bad) {
+__device__ __forceinline__ cx<T> leafx(const double2 *__restrict__ Wv, cx<T> D, T thr, cx<T> c00, cx<T> c0e,
+                                      cx<T> ce0, cx<T> cee, int s, bool &bad) {
   constexpr int K = N - 1;
   const cx<T> piv = csub<T>(D, c00);
   const cx<T> ue = csub<T>(ldx<T>(wv<N>(Wv, N, K, s)), ce0);
   const cx<T> ve = csub<T>(ldx<T>(wv<N>(Wv, K, N, s)), c0e);
-  const cx<T> x = cmul<T>(piv, cfma<T>(cmul<T>(piv, cee), ue, ve));
-  const double2 v = dmul(D2, dcx<T>(x));
-  const bool b = flag || !(cabs1<T>(piv) >= thr);
-  const double sg = (__popc(cfg) & 1) ? 1.0 : -1.0; // -(-1)^|sigma|
-  sre += b ? 0.0 : sg * v.x;
-  sim += b ? 0.0 : sg * v.y;
-  bad |= (uint32_t)b << bit;
+  bad = !(cabs1<T>(piv) >= thr);
+  return cmul<T>(piv, cfma<T>(cmul<T>(piv, cee), ue, ve));
 }
 
 // The subtree of the lane's level-K node (R = N - K + 1, table C) over all
@@ -173,12 +168,26 @@ __device__ __forceinline__ void subtree(const cx<T> (&C)[R][R], const double2 *_
                                         double2 dp, int flag, int cfg, double &sre, double &sim, uint32_t &bad) {
   constexpr int K = N + 1 - R;
   if constexpr (R == 2) {
-    const double2 D2 = dmul(dp, dp);
+    // the two leaves' x, signed, summed in FP64, then one product with dp^2
+    // (a flagged path can make dp non-finite: the product is taken only when a
+    // leaf passed, whose path pivots all did)
+    double2 xs = make_double2(0.0, 0.0);
+    const double sg = (__popc(cfg) & 1) ? 1.0 : -1.0; // -(-1)^|sigma| of the index-0 leaf
+    bool good = false;
     static_for<0, 2>([&](auto Sx) {
       constexpr int s = Sx;
-      const int c = cfg | (s << K);
-      leaf<N, T>(Wv, D, thr, C[0][0], C[0][1], C[1][0], C[1][1], s, D2, flag, c, c >> 6, sre, sim, bad);
+      bool b;
+      const double2 x = dcx<T>(leafx<N, T>(Wv, D, thr, C[0][0], C[0][1], C[1][0], C[1][1], s, b));
+      b = b || flag;
+      const double g = s ? -sg : sg;
+      xs.x += b ? 0.0 : g * x.x;
+      xs.y += b ? 0.0 : g * x.y;
+      good = good || !b;
+      bad |= (uint32_t)b << ((cfg | (s << K)) >> 6);
     });
+    const double2 v = dmul(dmul(dp, dp), xs);
+    sre += good ? v.x : 0.0;
+    sim += good ? v.y : 0.0;
   } else {
     static_for<0, 2>([&](auto Sx) {
       constexpr int s = Sx;
@@ -227,6 +236,7 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
   const double2 Dd = make_double2(gl0.x, gl0.y - p.alpha);
   const double wconst = fmax(fmax(cabs1<double>(gl0), cabs1<double>(gg0)), cabs1<double>(Dd));
   const cx<T> D = cmk<T>((T)Dd.x, (T)Dd.y);
+  const cx<T> invD = cinv<T>(D);
 
   const int64_t nbatch = (p.n_points + 63) / 64;
   double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
@@ -278,84 +288,68 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
       u_n = buf.u[s2 * N + (lane < N ? lane : N - 1)];
       jac_n = buf.jac[s2];
     }
-    // sort by (time, index): the rank of vertex i among the others (NaN last,
-    // so that the ranks stay a permutation and the lookups yield NaN)
-    if (lq < N) {
-      const double kt = isnan(uc) ? __builtin_inf() : uc;
-      int rank = 0;
-#pragma unroll
-      for (int q = 0; q < N; q++) {
-        const double tq = Us[q];
-        const double kq = isnan(tq) ? __builtin_inf() : tq;
-        rank += (kq < kt) || (kq == kt && q < lq);
-      }
-      Ts[rank] = uc;
+    // sort by (time, index): lane a N + c asks whether vertex c comes before
+    // vertex a, and a ballot gives each vertex its rank (NaN last, so that the
+    // ranks stay a permutation and the lookups yield NaN)
+    {
+      const int a = lq / N, c = lq - a * N;
+      const double ta = Us[a < N ? a : 0], tc = Us[c];
+      const double ka = isnan(ta) ? __builtin_inf() : ta, kc = isnan(tc) ? __builtin_inf() : tc;
+      const uint64_t before = __ballot(lq < N * N && (kc < ka || (kc == ka && c < a)));
+      if (lq < N) Ts[__popcll((before >> (lq * N)) & ((1ull << N) - 1))] = uc;
     }
     wave_sync();
-    // Wv: item (a, c) = a N + c; a < c: the vertex pair (a earlier, c later),
-    // g0^< and g0^> at d = t_c - t_a and at -d; a == c: vertex a's border
-    // column and row (compare_3way with the external points, no alpha: the
-    // points differ); a > c: nothing
+    // Wv: item (a, c) = a N + c with a <= c, four table lookups at two time
+    // differences.  a < c, the vertex pair (a earlier): g0^> and g0^< at
+    // d = t_c - t_a (row c, column a: Keldysh index 0 / 1 of a), g0^< and g0^>
+    // at -d (row a, column c).  a == c, vertex a's border: column (a s; B) at
+    // t_a - t_B and row (A; a s) at t_A - t_a, the table of each s by
+    // compare_3way (the vertex's split is below the external points' N, no
+    // alpha: the points differ).  One instruction stream for both kinds.
     double wmax = 0.0;
     static_for<0, (N * N + 63) / 64>([&](auto RI) {
       const int q = lq + 64 * (int)RI;
       const int a = q / N, c = q - a * N;
       if (q < N * N && a <= c) {
+        const bool pair = a < c;
         const double ta = Ts[a], tc = Ts[c];
+        const double d1 = pair ? tc - ta : ta - p.B.time;
+        const double d2 = pair ? ta - tc : p.A.time - ta;
+        const TabPos p1 = table_pos(p, d1), p2 = table_pos(p, d2);
+        // the border's contour order per Keldysh index s of the vertex
+        const int kA = p.A.k_idx, kB = p.B.k_idx;
+        const int sgC = d1 != 0.0 ? (d1 < 0.0 ? -1 : 1) : -1; // vertex vs B at equal k: B's split is larger
+        const int sgR = d2 != 0.0 ? (d2 < 0.0 ? -1 : 1) : 1;  // A vs vertex at equal k
         const double2 *tab[4];
-        TabPos pos[4];
-        if (a < c) {
-          const TabPos p1 = table_pos(p, tc - ta), p2 = table_pos(p, ta - tc);
-          tab[0] = p.lesser;
-          tab[1] = p.greater;
-          tab[2] = p.lesser;
-          tab[3] = p.greater;
-          pos[0] = p1;
-          pos[1] = p1;
-          pos[2] = p2;
-          pos[3] = p2;
-        } else {
-          // (a s; B) for s = 0, 1 and (A; a s) for s = 0, 1
 #pragma unroll
-          for (int h = 0; h < 4; h++) {
-            const int s = h & 1;
-            CPt x, y;
-            if (h < 2) {
-              x = CPt{ta, s, 0, 0};
-              y = CPt{p.B.time, p.B.k_idx, N, p.B.orbital};
-            } else {
-              x = CPt{p.A.time, p.A.k_idx, N, p.A.orbital};
-              y = CPt{ta, s, 0, 0};
-            }
-            const int ord = compare_3way(x, y);
-            tab[h] = (ord > 0 ? p.greater : p.lesser) + (size_t)(x.orb * p.n_orb + y.orb) * p.n_pts;
-            pos[h] = table_pos(p, x.time - y.time);
-          }
+        for (int h = 0; h < 2; h++) {
+          const int ordC = h != kB ? h - kB : (1 - 2 * h) * sgC;
+          const int ordR = kA != h ? kA - h : (1 - 2 * kA) * sgR;
+          const double2 *bc = (ordC > 0 ? p.greater : p.lesser) + (size_t)p.B.orbital * p.n_pts;
+          const double2 *br = (ordR > 0 ? p.greater : p.lesser) + (size_t)(p.A.orbital * p.n_orb) * p.n_pts;
+          tab[h] = pair ? (h == 0 ? p.greater : p.lesser) : bc;
+          tab[2 + h] = pair ? (h == 0 ? p.lesser : p.greater) : br;
         }
         double2 e0[4], e1[4];
 #pragma unroll
         for (int h = 0; h < 4; h++) {
-          e0[h] = tab[h][pos[h].i];
-          e1[h] = tab[h][pos[h].i + 1];
+          const int i = h < 2 ? p1.i : p2.i;
+          e0[h] = tab[h][i];
+          e1[h] = tab[h][i + 1];
         }
         double2 v[4];
 #pragma unroll
-        for (int h = 0; h < 4; h++) v[h] = table_mix(e0[h], e1[h], pos[h]);
-        if (a < c) {
-          // later row c, earlier column a: index of a picks greater (0) / lesser (1);
-          // earlier row a, later column c: lesser (0) / greater (1)
-          Wv[(c * WS + a) * 2 + 0] = v[1];
-          Wv[(c * WS + a) * 2 + 1] = v[0];
-          Wv[(a * WS + c) * 2 + 0] = v[2];
-          Wv[(a * WS + c) * 2 + 1] = v[3];
+        for (int h = 0; h < 4; h++) v[h] = table_mix(e0[h], e1[h], h < 2 ? p1 : p2);
+        // pair: (c, a, s) and (a, c, s); border: (a, e, s) and (e, a, s)
+        const int i1 = pair ? c * WS + a : a * WS + N, i2 = pair ? a * WS + c : N * WS + a;
+        Wv[i1 * 2 + 0] = v[0];
+        Wv[i1 * 2 + 1] = v[1];
+        Wv[i2 * 2 + 0] = v[2];
+        Wv[i2 * 2 + 1] = v[3];
+        double m = 0.0;
 #pragma unroll
-          for (int h = 0; h < 4; h++) wmax = fmax(wmax, cabs1<double>(v[h]));
-        } else {
-          Wv[(a * WS + N) * 2 + 0] = v[0];
-          Wv[(a * WS + N) * 2 + 1] = v[1];
-          Wv[(N * WS + a) * 2 + 0] = v[2];
-          Wv[(N * WS + a) * 2 + 1] = v[3];
-        }
+        for (int h = 0; h < 4; h++) m = fmax(m, cabs1<double>(v[h]));
+        wmax = pair ? fmax(wmax, m) : wmax;
       }
     });
     wmax = fmax(wave_max(wmax), wconst);
@@ -363,9 +357,9 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
     wave_sync();
 
     // levels 1..3 breadth first (level 1 and 3 in LA, level 2 in LB)
-    clevel<N, 0, T>(nullptr, LA, Wv, D, thr, dpath, fpath, lq);
-    clevel<N, 1, T>(LA, LB, Wv, D, thr, dpath, fpath, lq);
-    clevel<N, 2, T>(LB, LA, Wv, D, thr, dpath, fpath, lq);
+    clevel<N, 0, T>(nullptr, LA, Wv, D, invD, thr, dpath, fpath, lq);
+    clevel<N, 1, T>(LA, LB, Wv, D, invD, thr, dpath, fpath, lq);
+    clevel<N, 2, T>(LB, LA, Wv, D, invD, thr, dpath, fpath, lq);
 
     // the lane's path: sigma_0..sigma_5 = bits of the lane
     double sre = 0.0, sim = 0.0;
@@ -386,8 +380,15 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
       });
       if constexpr (N == 6) {
         // vertex 5 is the last: the lane's own configuration
-        leaf<N, T>(Wv, D, thr, lp.template at<5, 0, 0>(), lp.template at<5, 0, 1>(), lp.template at<5, 1, 0>(),
-                   lp.template at<5, 1, 1>(), (lq >> 5) & 1, dmul(dp, dp), flag, lq, 0, sre, sim, bad);
+        bool b;
+        const double2 x = dcx<T>(leafx<N, T>(Wv, D, thr, lp.template at<5, 0, 0>(), lp.template at<5, 0, 1>(),
+                                             lp.template at<5, 1, 0>(), lp.template at<5, 1, 1>(), (lq >> 5) & 1, b));
+        b = b || flag;
+        const double2 v = dmul(dmul(dp, dp), x);
+        const double sg = (__popc(lq) & 1) ? 1.0 : -1.0;
+        sre += b ? 0.0 : sg * v.x;
+        sim += b ? 0.0 : sg * v.y;
+        bad |= (uint32_t)b;
       } else {
         constexpr int R6 = S::R(6);
         cx<T> C6[R6][R6];

@@ -51,7 +51,7 @@ extern "C" {
 #define KD_PREC_FP32_LU_ALL 2
 
 /* Keldysh-sum algorithm.  KD_ALGO_DEFAULT (a new ctx's setting): KD_ALGO_CTREE
- * at orders 6..8, KD_ALGO_WSPLIT at 9 and 10, KD_ALGO_CONFIG otherwise -- the fastest on MI355X for
+ * at orders 6..10, KD_ALGO_CONFIG otherwise -- the fastest on MI355X for
  * full-size launches; kd_autotune measures every candidate instead.
  * KD_ALGO_CONFIG: one bordered LU per
  * configuration sigma (2^n of them, sharing the staged Wick matrix).
@@ -84,14 +84,14 @@ extern "C" {
  * paired.  Same arithmetic up to reassociation (FP64 results agree to
  * ~1e-13). */
 #define KD_ALGO_WSPLIT 4
-/* KD_ALGO_CTREE (orders 6..8; otherwise KD_ALGO_CONFIG): the time-ordered
+/* KD_ALGO_CTREE (orders 6..10; otherwise KD_ALGO_CONFIG): the time-ordered
  * tree.  With the vertices sorted by time, compare_3way (common.cpp:47-57)
  * makes each Wick entry between two vertices depend only on the Keldysh index
  * of the earlier one, so eliminating the vertices in time order subtracts from
  * every remaining entry a correction that does not depend on the later
  * vertices' indices: one value per vertex pair and tree node instead of four.
- * Order 8: ~3x fewer flops than KD_ALGO_WSPLIT; same determinants up to
- * reassociation (FP64 results agree to ~1e-13). */
+ * Orders 8-10: 2.2-2.7x fewer flops than KD_ALGO_WSPLIT; same determinants up
+ * to reassociation (FP64 results agree to ~1e-13 at order 8, 1e-12 at 10). */
 #define KD_ALGO_CTREE 5
 
 typedef struct kd_ctx kd_ctx;

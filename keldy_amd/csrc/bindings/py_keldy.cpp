@@ -169,8 +169,8 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
               throw keldy_error("algorithm must be 'default', 'config', 'pair', 'split', 'wsplit', 'ctree' or 'tree'");
             s.ctx().check(kd_set_algorithm(s.ctx().get(), v));
           },
-          py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (ctree at orders 6..8, wsplit at 9 and 10, config otherwise), 'config', 'pair' "
-          "(orders >= 7), 'split' or 'wsplit' (orders 6..10), 'ctree' (the time-ordered tree, orders 6..8) or 'tree' "
+          py::arg("algorithm"), "Keldysh-sum algorithm: 'default' (ctree at orders 6..10, config otherwise), 'config', 'pair' "
+          "(orders >= 7), 'split' or 'wsplit' (orders 6..10), 'ctree' (the time-ordered tree, orders 6..10) or 'tree' "
           "(orders >= 6)")
       .def("get_algorithm",
            [](T &s) {

@@ -86,7 +86,7 @@ struct kd_ctx {
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
   // point and waves per CU of the persistent grid (0: the algorithm's default)
   int lu_lanes_max = 64, lu_waves_per_cu = 0;
-  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: time-ordered tree at orders 6..8, wave-per-point split at 9..10)
+  int algo = KD_ALGO_DEFAULT; // Keldysh-sum algorithm (KD_ALGO_DEFAULT: time-ordered tree at orders 6..10)
   float tune_ms_per_mpt = 0.f;
   bool no_reduce = false; // autotuning launches leave the accumulator alone
   // time binning (kd_set_time_bins)
@@ -159,18 +159,21 @@ bool use_pair(const kd_ctx *c) {
   return c->algo == KD_ALGO_PAIR && c->order >= 7 && c->obs == KD_OBS_DIRECT && c->lu_lanes_max == 64;
 }
 
-// the split tree + per-configuration LU with one point per wave (orders 6..10;
-// the default at orders 9 and 10)
+// orders where the time-ordered tree is the default algorithm (all it supports:
+// the fastest there on MI355X)
+bool ctree_default(int order) { return kd::ctree_supported(order); }
+
+// the split tree + per-configuration LU with one point per wave (orders 6..10)
 bool use_wsplit(const kd_ctx *c) {
   const bool chosen =
-      c->algo == KD_ALGO_WSPLIT || (c->algo == KD_ALGO_DEFAULT && !kd::ctree_supported(c->order));
+      c->algo == KD_ALGO_WSPLIT || (c->algo == KD_ALGO_DEFAULT && !ctree_default(c->order));
   return chosen && c->obs == KD_OBS_DIRECT && kd::wsplit_supported(c->order);
 }
 
 // the time-ordered tree of vertex-level corrections, one point per wave (orders
-// 6..8; the default there)
+// 6..10; the default there)
 bool use_ctree(const kd_ctx *c) {
-  const bool chosen = c->algo == KD_ALGO_CTREE || c->algo == KD_ALGO_DEFAULT;
+  const bool chosen = c->algo == KD_ALGO_CTREE || (c->algo == KD_ALGO_DEFAULT && ctree_default(c->order));
   return chosen && c->obs == KD_OBS_DIRECT && kd::ctree_supported(c->order);
 }
 
@@ -1042,40 +1045,40 @@ static double wsplit_flops(int N) {
 }
 
 // The time-ordered tree, as executed (complex multiply-add 8, multiply 6,
-// subtraction 2, reciprocal 7, FP64 path-determinant product 6):
-// levels 1..3 breadth first, each (child, row) item with its own pivot
-// reciprocal and row factor, a subtraction and a multiply-add per entry (a
-// multiply at level 1); each lane's forced levels 3..5 (pivot, reciprocal, the
-// factors u and w, every entry of levels 4..6 one multiply-add) and its
-// subtree over the later vertices; per leaf the last pivot, the border factors,
-// p (p c_ee + u_e v_e), its product with dp^2 and the signed FP64 sum.
+// subtraction 2, reciprocal 7, FP64 complex product 6): levels 1..3 breadth
+// first, each (child, row) item with its own pivot reciprocal and row factor
+// (the root's pivot reciprocal is a constant), a subtraction and a
+// multiply-add per entry (a multiply at level 1); per lane, its forced levels
+// from 3 (pivot, reciprocal, the factors u and w, the path determinant; every
+// entry of the levels after 3 one multiply-add) down to its own node -- level
+// 6, or at order 10 level 7 in two passes over vertex 6's index -- then the
+// subtree below it; per leaf the last pivot, the border factors,
+// x = p (p c_ee + u_e v_e) and its signed FP64 sum, per pair of leaves dp^2 and
+// its product with that sum.
 static double ctree_flops(int N) {
   double f = 0.0;
   for (int k = 0; k < 3; k++) {
     const double rc = N - k, items = std::ldexp(rc, k + 1);
-    const double per_item = (k > 0 ? 4 : 0) + 7 + 6 + (k > 0 ? rc * (2 + 8) : rc * 6);
-    f += items * per_item + std::ldexp(6.0, k + 1);
+    const double per_item = (k > 0 ? 2 + 2 + 7 : 0) + 6 + (k > 0 ? rc * (2 + 8) : rc * 6) + 6;
+    f += items * per_item;
   }
-  // per lane: forced steps at vertices 3..min(5, N-2), then the subtree
+  const double leaf = 2 + 4 + 20 + 4, leaf_pair = 6 + 6 + 2;
+  auto step = [&](int k) { const double r = N - k + 1; return 2 + 7 + (r - 1) * (2 + 2 + 6) + 6; };
   double lane = 0.0;
-  const int kf = N - 1 < 6 ? N - 1 : 6; // first vertex not forced
-  for (int k = 3; k < kf; k++) {
-    const double r = N - k + 1;
-    lane += 2 + 7 + (r - 1) * (2 + 2 + 6) + 6;
-    if (k > 3) lane += 8 * r * r; // the level-k entries, one multiply-add each
-  }
-  const double leaf = 2 + 4 + 20 + 6 + 2;
-  if (kf == N - 1 && N <= 6) {
-    lane += 8 * 4 + 6 + leaf; // the level-(N-1) 2x2 entries, dp^2 and the lane's own leaf
+  if (N == 6) {
+    lane = step(3) + step(4) + 8.0 * (9 + 4) + leaf + leaf_pair; // levels 4 and 5 entries, the lane's one leaf
   } else {
-    lane += 8.0 * (N - kf + 1) * (N - kf + 1); // the level-6 entries
+    const int kf = N <= 9 ? 6 : 7, passes = N <= 9 ? 1 : 2; // the lane's own node level
+    double path = 0.0;
+    for (int k = 3; k < kf; k++) path += step(k);
+    for (int k = 4; k <= kf; k++) path += 8.0 * (N - k + 1) * (N - k + 1); // entries of levels 4..kf
     double sub = 0.0;
     for (int k = kf; k < N - 1; k++) { // internal subtree levels: 2^(k-kf+1) children
       const double r = N - k + 1;
-      sub += std::ldexp(2 + 7 + (r - 1) * (2 + 2 + 6) + 8 * (r - 1) * (r - 1) + 6, k - kf + 1);
+      sub += std::ldexp(step(k) + 8 * (r - 1) * (r - 1), k - kf + 1);
     }
-    sub += std::ldexp(6.0, N - 1 - kf) + std::ldexp(leaf, N - kf); // dp^2 per level-(N-1) node, the leaves
-    lane += sub;
+    sub += std::ldexp(leaf_pair, N - 1 - kf) + std::ldexp(leaf, N - kf);
+    lane = passes * (path + sub);
   }
   return f + 64.0 * lane;
 }

@@ -1,4 +1,4 @@
-// kd_ctree.hip -- the Keldysh sum by the time-ordered tree (orders 6..8), one
+// kd_ctree.hip -- the Keldysh sum by the time-ordered tree (orders 6..10), one
 // point per wavefront.
 //
 // In the contour order of compare_3way (common.cpp:47-57), g0 between two
@@ -116,14 +116,14 @@ __device__ __forceinline__ void clevel(const cx<T> *__restrict__ par, cx<T> *__r
 }
 
 // A lane's path below its level-3 node: the entries of its level-k tables
-// (k = 3..6) on demand -- level 3 from LDS, level k > 3 as level k-1's entry
-// plus the rank-1 term of vertex k-1 -- and the rank-1 factors u, w of the
-// vertices 3..5 along the lane's Keldysh choices.
-template <int N, typename T> struct LanePath {
+// (k = 3..3+NL) on demand -- level 3 from LDS, level k > 3 as level k-1's
+// entry plus the rank-1 term of vertex k-1 -- and the rank-1 factors u, w of
+// the vertices 3..2+NL along the lane's Keldysh choices.
+template <int N, typename T, int NL> struct LanePath {
   using S = CShape<N>;
   static constexpr int R3 = S::R(3);
   const cx<T> *P3;
-  cx<T> u[3][R3 - 1], w[3][R3 - 1];
+  cx<T> u[NL][R3 - 1], w[NL][R3 - 1];
   template <int K, int A, int B> __device__ __forceinline__ cx<T> at() const {
     if constexpr (K == 3)
       return ldx<T>(P3[A * R3 + B]);
@@ -209,13 +209,16 @@ __device__ __forceinline__ void subtree(const cx<T> (&C)[R][R], const double2 *_
   }
 }
 
-#ifndef KD_CTREE_WAVES_PER_EU
-#define KD_CTREE_WAVES_PER_EU 3
-#endif
+// waves per SIMD the register allocation targets: three up to order 8 (at most
+// 168 VGPRs), two at order 9 (256), one at order 10, whose lane path and
+// subtree then keep their tables in the 512 VGPRs and AGPRs of a lane instead
+// of spilling to memory (order 10, all points in the domain: 18.6 ms per 2^21
+// points against 29.5 ms at two waves per SIMD)
+template <int N> constexpr int ctree_waves_per_eu() { return N <= 8 ? 3 : N == 9 ? 2 : 1; }
 template <int N, typename T>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KD_CTREE_WAVES_PER_EU))) void
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ctree_waves_per_eu<N>()))) void
 integrand_ctree_kernel(KdParams p, PrepBuf buf) {
-  static_assert(N >= 6 && N <= 8, "time-ordered tree: orders 6..8");
+  static_assert(N >= 6 && N <= 10, "time-ordered tree: orders 6..10");
   using S = CShape<N>;
   constexpr int WS = S::WS;
   __shared__ double2 Wv[WS * WS * 2];
@@ -291,12 +294,22 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
     // sort by (time, index): lane a N + c asks whether vertex c comes before
     // vertex a, and a ballot gives each vertex its rank (NaN last, so that the
     // ranks stay a permutation and the lookups yield NaN)
-    {
+    if constexpr (N * N <= 64) {
       const int a = lq / N, c = lq - a * N;
       const double ta = Us[a < N ? a : 0], tc = Us[c];
       const double ka = isnan(ta) ? __builtin_inf() : ta, kc = isnan(tc) ? __builtin_inf() : tc;
       const uint64_t before = __ballot(lq < N * N && (kc < ka || (kc == ka && c < a)));
       if (lq < N) Ts[__popcll((before >> (lq * N)) & ((1ull << N) - 1))] = uc;
+    } else if (lq < N) { // above order 8: each vertex counts the ones before it
+      const double kt = isnan(uc) ? __builtin_inf() : uc;
+      int rank = 0;
+#pragma unroll
+      for (int q = 0; q < N; q++) {
+        const double tq = Us[q];
+        const double kq = isnan(tq) ? __builtin_inf() : tq;
+        rank += (kq < kt) || (kq == kt && q < lq);
+      }
+      Ts[rank] = uc;
     }
     wave_sync();
     // Wv: item (a, c) = a N + c with a <= c, four table lookups at two time
@@ -366,36 +379,62 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
     uint32_t bad = 0;
     {
       const int g3 = lq & 7;
-      LanePath<N, T> lp;
-      lp.P3 = LA + g3 * S::NS(3);
-      double2 dp = dpath[8 + g3];
-      int flag = fpath[8 + g3];
-      static_for<3, 6>([&](auto KI) {
-        constexpr int k = KI;
-        if constexpr (k < N - 1) {
-          const cx<T> piv = lp.template step<k>(Wv, D, (lq >> k) & 1);
-          dp = dmul(dp, dcx<T>(piv));
-          flag |= (int)!(cabs1<T>(piv) >= thr);
-        }
-      });
-      if constexpr (N == 6) {
-        // vertex 5 is the last: the lane's own configuration
-        bool b;
-        const double2 x = dcx<T>(leafx<N, T>(Wv, D, thr, lp.template at<5, 0, 0>(), lp.template at<5, 0, 1>(),
-                                             lp.template at<5, 1, 0>(), lp.template at<5, 1, 1>(), (lq >> 5) & 1, b));
-        b = b || flag;
-        const double2 v = dmul(dmul(dp, dp), x);
-        const double sg = (__popc(lq) & 1) ? 1.0 : -1.0;
-        sre += b ? 0.0 : sg * v.x;
-        sim += b ? 0.0 : sg * v.y;
-        bad |= (uint32_t)b;
-      } else {
-        constexpr int R6 = S::R(6);
-        cx<T> C6[R6][R6];
-        static_for<0, R6>([&](auto A) {
-          static_for<0, R6>([&](auto B) { C6[A][B] = lp.template at<6, A, B>(); });
+      if constexpr (N <= 9) {
+        LanePath<N, T, 3> lp;
+        lp.P3 = LA + g3 * S::NS(3);
+        double2 dp = dpath[8 + g3];
+        int flag = fpath[8 + g3];
+        static_for<3, 6>([&](auto KI) {
+          constexpr int k = KI;
+          if constexpr (k < N - 1) {
+            const cx<T> piv = lp.template step<k>(Wv, D, (lq >> k) & 1);
+            dp = dmul(dp, dcx<T>(piv));
+            flag |= (int)!(cabs1<T>(piv) >= thr);
+          }
         });
-        subtree<N, R6, T>(C6, Wv, D, thr, dp, flag, lq, sre, sim, bad);
+        if constexpr (N == 6) {
+          // vertex 5 is the last: the lane's own configuration
+          bool b;
+          const double2 x = dcx<T>(leafx<N, T>(Wv, D, thr, lp.template at<5, 0, 0>(), lp.template at<5, 0, 1>(),
+                                               lp.template at<5, 1, 0>(), lp.template at<5, 1, 1>(), (lq >> 5) & 1,
+                                               b));
+          b = b || flag;
+          const double2 v = dmul(dmul(dp, dp), x);
+          const double sg = (__popc(lq) & 1) ? 1.0 : -1.0;
+          sre += b ? 0.0 : sg * v.x;
+          sim += b ? 0.0 : sg * v.y;
+          bad |= (uint32_t)b;
+        } else {
+          constexpr int R6 = S::R(6);
+          cx<T> C6[R6][R6];
+          static_for<0, R6>([&](auto A) {
+            static_for<0, R6>([&](auto B) { C6[A][B] = lp.template at<6, A, B>(); });
+          });
+          subtree<N, R6, T>(C6, Wv, D, thr, dp, flag, lq, sre, sim, bad);
+        }
+      } else {
+        // order 10: two passes over sigma_6, each recomputing the lane's path to
+        // its level-7 node (measured faster than keeping the 5x5 level-6 table
+        // through both passes: 18.6 against 20.4 ms per 2^21 points)
+#pragma unroll 1
+        for (int s6 = 0; s6 < 2; s6++) {
+          LanePath<N, T, 4> lp;
+          lp.P3 = LA + g3 * S::NS(3);
+          double2 dp = dpath[8 + g3];
+          int flag = fpath[8 + g3];
+          static_for<3, 7>([&](auto KI) {
+            constexpr int k = KI;
+            const cx<T> piv = lp.template step<k>(Wv, D, k < 6 ? (lq >> k) & 1 : s6);
+            dp = dmul(dp, dcx<T>(piv));
+            flag |= (int)!(cabs1<T>(piv) >= thr);
+          });
+          constexpr int R7 = S::R(7);
+          cx<T> C7[R7][R7];
+          static_for<0, R7>([&](auto A) {
+            static_for<0, R7>([&](auto B) { C7[A][B] = lp.template at<7, A, B>(); });
+          });
+          subtree<N, R7, T>(C7, Wv, D, thr, dp, flag, lq | (s6 << 6), sre, sim, bad);
+        }
       }
     }
     // flagged configurations: partial-pivoting FP64 LU of their matrices
@@ -475,7 +514,7 @@ static hipError_t launch_ctree_n(const KdParams &p, const PrepBuf &buf, int grid
   return hipGetLastError();
 }
 
-bool ctree_supported(int order) { return order >= 6 && order <= 8; }
+bool ctree_supported(int order) { return order >= 6 && order <= 10; }
 
 hipError_t launch_ctree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
   const bool f32 = lu_is_fp32(order, precision);
@@ -486,6 +525,8 @@ hipError_t launch_ctree(int order, int precision, const KdParams &p, const PrepB
     KD_CTREE_CASE(6)
     KD_CTREE_CASE(7)
     KD_CTREE_CASE(8)
+    KD_CTREE_CASE(9)
+    KD_CTREE_CASE(10)
   default:
     return hipErrorInvalidValue;
   }

@@ -40,7 +40,7 @@ hipError_t launch_split(int order, int precision, const KdParams &p, const PrepB
 // one point per 64-thread workgroup (one wave), no workgroup barriers.
 bool wsplit_supported(int order);
 hipError_t launch_wsplit(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
-// integrand_ctree_kernel<order, T> (orders 6..8): the time-ordered tree of
+// integrand_ctree_kernel<order, T> (orders 6..10): the time-ordered tree of
 // vertex-level corrections, one point per 64-thread workgroup (one wave).
 bool ctree_supported(int order);
 hipError_t launch_ctree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);

@@ -1,4 +1,4 @@
-"""The time-ordered tree (KD_ALGO_CTREE, orders 6..8) against the oracle and
+"""The time-ordered tree (KD_ALGO_CTREE, orders 6..10) against the oracle and
 the other Keldysh-sum kernels.
 
 The kernel sorts each point's vertex times and eliminates the vertices in
@@ -21,7 +21,7 @@ from tests import gpu_util as U
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-10
-ORDERS = [6, 7, 8]
+ORDERS = [6, 7, 8, 9, 10]
 
 
 def _normwise(a, b):
@@ -106,7 +106,7 @@ def test_ctree_near_coincident_pairs_vs_oracle(flat_g0, order):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("order", [6, 8])
+@pytest.mark.parametrize("order", [6, 8, 10])
 def test_ctree_alpha_half_all_fallback(flat_g0, order):
     """alpha = 1/2 at half filling: the diagonal D = g^<(0) - i/2 is exactly 0,
     so every configuration's first pivot fails the check and every

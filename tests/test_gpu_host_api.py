@@ -202,15 +202,14 @@ def test_pilot_histograms_on_device_match_host_binning(model30):
 
 
 def test_default_algorithm_per_order(model30):
-    """A new computer uses the time-ordered tree at orders 6-8, the
-    wave-per-point split algorithm at orders 9-10 and one LU per configuration
-    otherwise; an explicit choice holds until 'default' restores the
+    """A new computer uses the time-ordered tree at orders 6-10 and one LU
+    per configuration otherwise; an explicit choice holds until 'default' restores the
     per-order rule; unknown names raise; both give the same run on the same
     points (1e-12 relative)."""
     from keldy_amd import impurity_oneband as kio
     from keldy_amd import warpers as kw
     t = 10.0
-    for order, want in ((5, "config"), (6, "ctree"), (7, "ctree"), (8, "ctree"), (9, "wsplit"), (10, "wsplit"),
+    for order, want in ((5, "config"), (6, "ctree"), (7, "ctree"), (8, "ctree"), (9, "ctree"), (10, "ctree"),
                         (11, "config")):
         assert kio.ComputeChargeQDirect(model30, t, order, 0.0).get_algorithm() == want, order
     c = kio.ComputeChargeQDirect(model30, t, 8, 0.0)

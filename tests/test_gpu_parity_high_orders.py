@@ -48,6 +48,7 @@ def _shapes(order):
     if order <= 10:
         out.append(("split", capi.KD_ALGO_SPLIT, 64))
         out.append(("wsplit", capi.KD_ALGO_WSPLIT, 64))
+        out.append(("ctree", capi.KD_ALGO_CTREE, 64))
     return out
 
 
@@ -160,7 +161,7 @@ def test_half_filling_alpha_half_all_fallback(flat_g0, order):
         assert np.array_equal(dom, rdom), label
         assert np.all(np.isfinite(got)), label
         assert np.max(np.abs(got - ref)) <= TOL * scale, label
-        if label in ("split", "wsplit", "config64", "tree"):
+        if label in ("split", "wsplit", "ctree", "config64", "tree"):
             # the uniform map on [0, t]^n: every point in the domain
             _, (descs, maps, tabs) = U.chain([("map", "uniform", t, 1.0, 1)])
             ctx.set_warper_chain(descs, maps, tabs)

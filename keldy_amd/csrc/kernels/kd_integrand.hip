@@ -564,18 +564,37 @@ __global__ __launch_bounds__(256) void reduce_bins_kernel(const double2 *__restr
 }
 
 // Deterministic sum of per-block partials into acc[0..3] (+=).
-__global__ __launch_bounds__(256) void reduce_partials_kernel(const double *__restrict__ part, int nblk,
-                                                              double *__restrict__ acc) {
-  __shared__ double sh[4][256];
+// One workgroup of 1024 threads: thread t sums partials t, t + 1024, ... in
+// that order, four of them loaded at once, then a fixed tree -- the same sum for
+// the same partials on every run.
+__global__ __launch_bounds__(1024) void reduce_partials_kernel(const double *__restrict__ part, int nblk,
+                                                               double *__restrict__ acc) {
+  __shared__ double sh[4][1024];
   const int t = threadIdx.x;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int i = t; i < nblk; i += 256)
+  int i = t;
+  for (; i + 3 * 1024 < nblk; i += 4 * 1024) {
+    double2 v[4][2];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      v[u][0] = reinterpret_cast<const double2 *>(part)[2 * (i + u * 1024)];
+      v[u][1] = reinterpret_cast<const double2 *>(part)[2 * (i + u * 1024) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      s[0] += v[u][0].x;
+      s[1] += v[u][0].y;
+      s[2] += v[u][1].x;
+      s[3] += v[u][1].y;
+    }
+  }
+  for (; i < nblk; i += 1024)
 #pragma unroll
     for (int c = 0; c < 4; c++) s[c] += part[4 * i + c];
 #pragma unroll
   for (int c = 0; c < 4; c++) sh[c][t] = s[c];
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
+  for (int w = 512; w > 0; w >>= 1) {
     if (t < w)
 #pragma unroll
       for (int c = 0; c < 4; c++) sh[c][t] += sh[c][t + w];
@@ -644,7 +663,7 @@ hipError_t launch_lu(int order, int precision, int lanes, bool pair, const KdPar
 }
 
 hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, part, nblk, acc);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, st, part, nblk, acc);
   return hipGetLastError();
 }
 

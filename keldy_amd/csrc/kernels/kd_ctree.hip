@@ -56,7 +56,8 @@ template <typename T> __device__ __forceinline__ double2 dcx(cx<T> v) { return m
 template <int N> struct CShape {
   static constexpr int WS = N + 1;                           // Wv row stride: vertices, then the border e = N
   static constexpr int R(int k) { return N - k + 1; }        // a level-k table: vertices k..N-1 and the border
-  static constexpr int NS(int k) { return (R(k) * R(k)) | 1; } // node stride (odd: nodes spread over the banks)
+  static constexpr int RS(int k) { return R(k); }                // row stride of a level-k table
+  static constexpr int NS(int k) { return (R(k) * RS(k)) | 1; }  // node stride (odd: nodes spread over the banks)
   static constexpr int sizeA() { return 2 * NS(1) > 8 * NS(3) ? 2 * NS(1) : 8 * NS(3); } // levels 1 and 3
   static constexpr int sizeB() { return 4 * NS(2); }                                     // level 2
   // global index of local row/column a of a level-k table
@@ -76,7 +77,7 @@ __device__ __forceinline__ void clevel(const cx<T> *__restrict__ par, cx<T> *__r
                                        const double2 *__restrict__ Wv, cx<T> D, cx<T> invD, T thr,
                                        double2 *__restrict__ dpath, int *__restrict__ fpath, int lane) {
   using S = CShape<N>;
-  constexpr int r = S::R(K), rc = r - 1;
+  constexpr int r = S::R(K), rc = r - 1, rsp = S::RS(K), rsc = S::RS(K + 1);
   constexpr int NIT = (2 << K) * rc;
   for (int it = lane; it < NIT; it += 64) {
     const int m = it / rc, a = it - m * rc;
@@ -87,18 +88,18 @@ __device__ __forceinline__ void clevel(const cx<T> *__restrict__ par, cx<T> *__r
     cx<T> wr[rc], cr[rc];
     if constexpr (K > 0) {
       piv = csub<T>(D, P[0]);
-      u = csub<T>(u, P[(a + 1) * r]);
+      u = csub<T>(u, P[(a + 1) * rsp]);
     }
     static_for<0, rc>([&](auto B) {
       constexpr int b = B;
       wr[b] = ldx<T>(wv<N>(Wv, K, S::gidx(K + 1, b), s));
       if constexpr (K > 0) {
         wr[b] = csub<T>(wr[b], P[b + 1]);
-        cr[b] = P[(a + 1) * r + b + 1];
+        cr[b] = P[(a + 1) * rsp + b + 1];
       }
     });
     const cx<T> l = cmul<T>(u, K > 0 ? cinv<T>(piv) : invD);
-    cx<T> *C = ch + m * S::NS(K + 1) + a * rc;
+    cx<T> *C = ch + m * S::NS(K + 1) + a * rsc;
     static_for<0, rc>([&](auto B) {
       constexpr int b = B;
       if constexpr (K > 0)
@@ -126,7 +127,7 @@ template <int N, typename T, int NL> struct LanePath {
   cx<T> u[NL][R3 - 1], w[NL][R3 - 1];
   template <int K, int A, int B> __device__ __forceinline__ cx<T> at() const {
     if constexpr (K == 3)
-      return ldx<T>(P3[A * R3 + B]);
+      return ldx<T>(P3[A * S::RS(3) + B]);
     else
       return cfma<T>(at<K - 1, A + 1, B + 1>(), u[K - 4][A], w[K - 4][B]);
   }
@@ -241,40 +242,42 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
   const cx<T> D = cmk<T>((T)Dd.x, (T)Dd.y);
   const cx<T> invD = cinv<T>(D);
 
-  const int64_t nbatch = (p.n_points + 63) / 64;
+  // a launch holds at most 2^21 points (the C-ABI launches chunks of that
+  // size), so item, batch and slot indices fit 32 bits
+  const int nbatch = (int)((p.n_points + 63) / 64);
   double acc_re = 0.0, acc_im = 0.0, acc_dom = 0.0, acc_fb = 0.0;
   const bool lane_sums = p.mode == MODE_RUN && p.nbins == 0 && !(p.cutoff > 0.0);
   // Work items and the window scan as in kd_wsplit.hip: one compacted point per
   // item, numbered slot-major (item = grp * nbatch + b), every gridDim.x-th
   // item to this wave; lane l of a window loads the batch count of the
   // window's item l and a ballot keeps the live ones.
-  const int64_t G = gridDim.x;
-  int wgrp = nbatch > 0 ? (int)(blockIdx.x / nbatch) : 64;
-  int64_t wb = blockIdx.x - (int64_t)wgrp * nbatch;
+  const int G = (int)gridDim.x;
+  int wgrp = nbatch > 0 ? (int)(blockIdx.x / (unsigned)nbatch) : 64;
+  int wb = (int)blockIdx.x - wgrp * nbatch;
   uint64_t mask = 0;
   int gl = 0, bl = 0;
   double u_n = 0.0, jac_n = 0.0;
   for (;;) {
     if (!mask) {
       if (wgrp >= 64) break;
-      const int64_t bw = wb + (int64_t)lane * G;
-      gl = wgrp + (int)(bw / nbatch);
-      bl = (int)(bw % nbatch);
+      const unsigned bw = (unsigned)(wb + lane * G);
+      gl = wgrp + (int)(bw / (unsigned)nbatch);
+      bl = (int)(bw % (unsigned)nbatch);
       mask = __ballot(gl < 64 && gl < buf.cnt[bl]);
-      const int64_t adv = wb + 64 * G;
-      wgrp += (int)(adv / nbatch);
-      wb = adv % nbatch;
+      const unsigned adv = (unsigned)(wb + 64 * G);
+      wgrp += (int)(adv / (unsigned)nbatch);
+      wb = (int)(adv % (unsigned)nbatch);
       if (!mask) continue;
       const int j1 = __builtin_ctzll(mask);
-      const int64_t s1 = (int64_t)rdlane(bl, j1) * 64 + rdlane(gl, j1);
+      const int s1 = rdlane(bl, j1) * 64 + rdlane(gl, j1);
       u_n = buf.u[s1 * N + (lane < N ? lane : N - 1)];
       jac_n = buf.jac[s1];
     }
     const int j = __builtin_ctzll(mask);
     mask &= mask - 1;
     const int g = rdlane(gl, j);
-    const int64_t b = rdlane(bl, j);
-    const int64_t slot = b * 64 + g;
+    const int b = rdlane(bl, j);
+    const int slot = b * 64 + g;
     const double uc = u_n, jj = jac_n;
     int lq = lane;
     __asm__ volatile("" : "+v"(lq));
@@ -287,7 +290,7 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
     // the next live item's record, in flight until the next point
     if (mask) {
       const int j2 = __builtin_ctzll(mask);
-      const int64_t s2 = (int64_t)rdlane(bl, j2) * 64 + rdlane(gl, j2);
+      const int s2 = rdlane(bl, j2) * 64 + rdlane(gl, j2);
       u_n = buf.u[s2 * N + (lane < N ? lane : N - 1)];
       jac_n = buf.jac[s2];
     }
@@ -478,7 +481,7 @@ integrand_ctree_kernel(KdParams p, PrepBuf buf) {
           }
         }
       } else {
-        const int64_t idx = b * 64 + buf.idx[slot];
+        const int idx = b * 64 + buf.idx[slot];
         p.out_vals[2 * idx] = sre;
         p.out_vals[2 * idx + 1] = sim;
         if (p.out_dom) p.out_dom[idx] = 1;

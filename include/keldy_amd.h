@@ -189,7 +189,11 @@ int kd_eval_batch_device_noalloc(kd_ctx *ctx, const double *d_u, int64_t n_point
                                  double *d_out, int32_t *d_in_domain);
 
 /* Warper chain, in the order warpers were added (applied last-to-first).
- * maps: all kd_map1d of PRODUCT warpers; tables: the doubles they index. */
+ * maps: all kd_map1d of PRODUCT warpers; tables: the doubles they index.
+ * At most KD_MAX_WARPERS warpers and KD_MAX_MAPS maps (KD_ERR_ARG beyond):
+ * the device keeps a chain's descriptors in on-chip memory. */
+#define KD_MAX_WARPERS 16
+#define KD_MAX_MAPS 64
 int kd_set_warper_chain(kd_ctx *ctx, const kd_warper_desc *warpers, int32_t n_warpers,
                         const kd_map1d *maps, int32_t n_maps, const double *tables,
                         int64_t n_tables);

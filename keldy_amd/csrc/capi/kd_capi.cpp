@@ -36,10 +36,11 @@ struct kd_ctx {
   int obs = KD_OBS_DIRECT; // observable (kd_set_integrand / kd_set_kernel_integrand)
   double omega = 0.0;
   // warper chain
-  kd_warper_desc *d_warpers = nullptr;
-  kd_map1d *d_maps = nullptr;
+  kd::WarperDev *d_warpers = nullptr;
+  kd::MapDev *d_maps = nullptr;
   double *d_tables = nullptr;
-  int n_warpers = 0;
+  int2 *d_guides = nullptr; // knot-search guides of the maps
+  int n_warpers = 0, n_maps = 0;
   std::vector<kd_warper_desc> h_warpers;
   // learned flow warper (KD_WARP_FLOW): FP32 parameters, Adam moments,
   // gradient, bf16 operand copies
@@ -249,7 +250,9 @@ kd::KdParams base_params(const kd_ctx *c) {
   p.warpers = c->d_warpers;
   p.maps = c->d_maps;
   p.tables = c->d_tables;
+  p.guides = c->d_guides;
   p.n_warpers = c->n_warpers;
+  p.n_maps = c->n_maps;
   p.Csr = c->d_Csr;
   p.shfr = c->d_shfr;
   p.k0 = c->k;
@@ -534,6 +537,7 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_warpers);
   dfree(c->d_maps);
   dfree(c->d_tables);
+  dfree(c->d_guides);
   dfree(c->d_fw);
   dfree(c->d_fm);
   dfree(c->d_fv);
@@ -712,10 +716,54 @@ int kd_eval_batch(kd_ctx *c, const double *u, int64_t n_points, int32_t keep_u_h
   return finish_timing(c);
 }
 
+// The interval bsearch_knots (the device's GSL bisection over [0, n-1]) finds
+// for x: max{i : xa[i] <= x} clamped to [0, n-2] when xa is non-decreasing.
+static int knot_interval(const double *xa, int n, double x) {
+  int ilo = 0, ihi = n - 1;
+  while (ihi > ilo + 1) {
+    const int i = (ihi + ilo) >> 1;
+    if (xa[i] > x)
+      ihi = i;
+    else
+      ilo = i;
+  }
+  return ilo;
+}
+
+// Guide of the knot search over xa[0..n-1] (kd::KnotGuide): 2n uniform buckets
+// over [xa[0], xa[n-1]], each holding the intervals of its edges widened by
+// 1e-6 of a bucket, far beyond the rounding of the device's bucket index, so
+// that the interval of every x of the bucket lies in [lo, hi).  Arrays that are
+// not finite and non-decreasing, or whose span is lost in the rounding of x - a,
+// get no guide (G = 0: plain bisection).
+static kd::KnotGuide knot_guide(const double *xa, int n, std::vector<int2> &out) {
+  kd::KnotGuide g{0.0, 0.0, 0, 0};
+  if (n < 3) return g;
+  for (int i = 0; i < n; i++)
+    if (!std::isfinite(xa[i]) || (i > 0 && xa[i] < xa[i - 1])) return g;
+  const double a = xa[0], span = xa[n - 1] - a;
+  const int G = 2 * n;
+  const double width = span / G, margin = 1e-6 * width;
+  if (!(span > 0.0) || !(margin > 1e-12 * (std::fabs(a) + std::fabs(xa[n - 1])))) return g;
+  g.a = a;
+  g.s = G / span;
+  g.off = (int)out.size();
+  g.G = G;
+  for (int b = 0; b < G; b++) {
+    const int lo = b == 0 ? 0 : knot_interval(xa, n, a + b * width - margin);
+    const int hi = b == G - 1 ? n - 1 : knot_interval(xa, n, a + (b + 1) * width + margin) + 1;
+    out.push_back(int2{lo, hi});
+  }
+  return g;
+}
+
 int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const kd_map1d *maps,
                         int32_t n_maps, const double *tables, int64_t n_tables) {
   if (!c) return KD_ERR_ARG;
   if (n_w < 0 || n_maps < 0 || n_tables < 0) return fail(c, KD_ERR_ARG, "negative size");
+  if (n_w > KD_MAX_WARPERS || n_maps > KD_MAX_MAPS)
+    return fail(c, KD_ERR_ARG, "a warper chain holds at most " + std::to_string(KD_MAX_WARPERS) + " warpers and " +
+                                   std::to_string(KD_MAX_MAPS) + " maps");
   if ((n_w && !w) || (n_maps && !maps) || (n_tables && !tables)) return fail(c, KD_ERR_ARG, "null buffer");
   for (int i = 0; i < n_w; i++) {
     const kd_warper_desc &d = w[i];
@@ -748,24 +796,51 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
         return fail(c, KD_ERR_ARG, "map table out of bounds");
     }
   }
+  std::vector<kd::WarperDev> wdev(n_w);
+  for (int i = 0; i < n_w; i++) {
+    wdev[i].w = w[i];
+    wdev[i].map_kind = -1;
+    wdev[i].pad = 0;
+    if (w[i].kind == KD_WARP_PRODUCT) {
+      const kd_map1d *m = maps + w[i].map_offset;
+      wdev[i].map_kind = m[0].kind;
+      for (int k = 1; k < w[i].n_maps; k++)
+        if (m[k].kind != m[0].kind) wdev[i].map_kind = -1;
+    }
+  }
+  std::vector<kd::MapDev> dev(n_maps);
+  std::vector<int2> guides(1, int2{0, 1}); // entry 0: what maps without a guide read (and ignore)
+  for (int i = 0; i < n_maps; i++) {
+    const kd_map1d &m = maps[i];
+    dev[i].m = m;
+    dev[i].gF = dev[i].gt = kd::KnotGuide{0.0, 0.0, 0, 0};
+    if (m.kind == KD_MAP_NEAREST || m.kind == KD_MAP_HYBRID) dev[i].gF = knot_guide(tables + m.F_off, m.n_inv, guides);
+    if (m.kind == KD_MAP_NEAREST) dev[i].gt = knot_guide(tables + m.t_off, m.n_pts, guides);
+  }
   KD_HIP(c, hipSetDevice(c->device));
   KD_HIP(c, hipStreamSynchronize(c->stream));
   dfree(c->d_warpers);
   dfree(c->d_maps);
   dfree(c->d_tables);
+  dfree(c->d_guides);
   if (n_w) {
-    KD_HIP(c, hipMalloc(&c->d_warpers, sizeof(kd_warper_desc) * n_w));
-    KD_HIP(c, hipMemcpy(c->d_warpers, w, sizeof(kd_warper_desc) * n_w, hipMemcpyHostToDevice));
+    KD_HIP(c, hipMalloc(&c->d_warpers, sizeof(kd::WarperDev) * n_w));
+    KD_HIP(c, hipMemcpy(c->d_warpers, wdev.data(), sizeof(kd::WarperDev) * n_w, hipMemcpyHostToDevice));
   }
   if (n_maps) {
-    KD_HIP(c, hipMalloc(&c->d_maps, sizeof(kd_map1d) * n_maps));
-    KD_HIP(c, hipMemcpy(c->d_maps, maps, sizeof(kd_map1d) * n_maps, hipMemcpyHostToDevice));
+    KD_HIP(c, hipMalloc(&c->d_maps, sizeof(kd::MapDev) * n_maps));
+    KD_HIP(c, hipMemcpy(c->d_maps, dev.data(), sizeof(kd::MapDev) * n_maps, hipMemcpyHostToDevice));
+  }
+  if (n_maps) {
+    KD_HIP(c, hipMalloc(&c->d_guides, sizeof(int2) * guides.size()));
+    KD_HIP(c, hipMemcpy(c->d_guides, guides.data(), sizeof(int2) * guides.size(), hipMemcpyHostToDevice));
   }
   if (n_tables) {
     KD_HIP(c, hipMalloc(&c->d_tables, sizeof(double) * n_tables));
     KD_HIP(c, hipMemcpy(c->d_tables, tables, sizeof(double) * n_tables, hipMemcpyHostToDevice));
   }
   c->n_warpers = n_w;
+  c->n_maps = n_maps;
   c->h_warpers.assign(w, w + n_w);
   c->flow_in_chain = n_w > 0 && w[n_w - 1].kind == KD_WARP_FLOW;
   return KD_OK;

@@ -57,6 +57,29 @@ struct KdGf {
   int k_idx, spin, orbital;
 };
 
+// Guide of a knot search over a non-decreasing array xa[0..n-1]: x falls in
+// bucket b = clamp(floor((x - a) * s), 0, G - 1) of a uniform partition of
+// [xa[0], xa[n-1]], and entry off + b of the guide buffer holds bounds
+// (lo, hi) with lo <= ans(x) < hi for every x of that bucket, where ans is the
+// bisection's answer over the whole array.  G = 0: no guide (plain bisection).
+struct KnotGuide {
+  double a, s;
+  int off, G;
+};
+// A one-dimensional map of a product warper with the guides of its knot
+// searches (the inverse CDF's knots F, the forward spline's knots t).
+struct MapDev {
+  kd_map1d m;
+  KnotGuide gF, gt;
+};
+// A warper of the chain; for a product warper, map_kind is the kind every one
+// of its maps has (-1: kinds differ), so that the axes run as one straight
+// sequence per kind instead of one branch per axis.
+struct WarperDev {
+  kd_warper_desc w;
+  int map_kind, pad;
+};
+
 struct KdParams {
   // g0 table (model.hpp:238-268): [n_orb][n_orb][n_pts] complex
   const double2 *lesser;
@@ -68,10 +91,12 @@ struct KdParams {
   double cutoff;
   int keep_u;
   // warper chain (warpers.hpp:41-137)
-  const kd_warper_desc *warpers;
-  const kd_map1d *maps;
+  const WarperDev *warpers;
+  const MapDev *maps;
   const double *tables;
-  int n_warpers;
+  const int2 *guides; // bisection bounds of the knot searches (KnotGuide)
+  int n_warpers;       // warpers applied (<= KD_MAX_WARPERS)
+  int n_maps;          // maps of the chain (<= KD_MAX_MAPS)
   // Sobol (qrng.hpp:49-114): bit-reversed matrices [dim][31], shifts [dim]
   const uint32_t *Csr;
   const uint32_t *shfr;
@@ -341,9 +366,11 @@ __device__ __forceinline__ double sobol_coord(const uint32_t *__restrict__ Csr, 
 }
 
 // ---------------------------------------------------------------------------
-// GSL-compatible interval search (gsl_interp_bsearch over [0, n-1]).
-__device__ __forceinline__ int bsearch_knots(const double *xa, int n, double x) {
-  int ilo = 0, ihi = n - 1;
+// GSL-compatible interval search (gsl_interp_bsearch over [0, n-1]), between
+// bounds lo < hi that hold the answer: the largest i with xa[i] <= x, clamped to
+// [0, n-2] (n-2 for NaN x, which no comparison stops).
+__device__ __forceinline__ int bisect_knots(const double *xa, int2 r, double x) {
+  int ilo = r.x, ihi = r.y;
   while (ihi > ilo + 1) {
     int i = (ihi + ilo) >> 1;
     if (xa[i] > x)
@@ -352,6 +379,25 @@ __device__ __forceinline__ int bsearch_knots(const double *xa, int n, double x) 
       ilo = i;
   }
   return ilo;
+}
+__device__ __forceinline__ int bsearch_knots(const double *xa, int n, double x) {
+  return bisect_knots(xa, make_int2(0, n - 1), x);
+}
+
+// Bounds of the search of x over xa[0..n-1] from the array's guide: usually
+// hi = lo + 1, so that no bisection step remains and a search costs one guide
+// load instead of log2(n) dependent knot loads.  Without a guide (G = 0, off =
+// 0: the buffer's first entry is a placeholder), the whole array; for NaN x,
+// the interval the full bisection ends in.  Branch-free, so that the loads of
+// several searches issue together.
+__device__ __forceinline__ int2 guide_bounds(int n, const int2 *__restrict__ gd, const KnotGuide &g, double x) {
+  const double tb = (x - g.a) * g.s;
+  int b = tb >= 0.0 ? (tb < (double)g.G ? (int)tb : g.G - 1) : 0; // NaN: 0
+  b = g.G == 0 ? 0 : b;
+  int2 r = gd[g.off + b];
+  if (g.G == 0) r = make_int2(0, n - 1);
+  if (x != x) r = make_int2(n - 2, n - 1);
+  return r;
 }
 
 // v [n] on the uniform mesh of [0, 1], at t: linear interpolation between the
@@ -367,52 +413,45 @@ __device__ __forceinline__ double mesh_lin(const double *__restrict__ v, int n, 
   return (1.0 - w) * v[i] + w * v[i + 1];
 }
 
-// x = F^{-1}(l), returns 1/f(x)   (product_1d_simple.hpp:109-117 and makers 560-630,
-// interp nearest 198-217, interp hybrid 403-415)
-
-__device__ inline double map1d_reverse(const kd_map1d &m, const double *__restrict__ tab, double l, double *u) {
+// x = F^{-1}(l) of one map kind; returns 1/f(x)   (product_1d_simple.hpp:109-117 and
+// makers 560-630, interp nearest 198-217, interp hybrid 403-415)
+template <int K>
+__device__ __forceinline__ double map_reverse_k(const MapDev &md, const double *__restrict__ tab,
+                                                const int2 *__restrict__ gd, double l, double *u) {
+  const kd_map1d &m = md.m;
   double x, f;
   const double w = m.w, um = m.u_max;
-  switch (m.kind) {
-  case KD_MAP_UNIFORM:
+  if constexpr (K == KD_MAP_UNIFORM) {
     x = um * l;
     f = 1.0 / um;
-    break;
-  case KD_MAP_EXPONENTIAL:
+  } else if constexpr (K == KD_MAP_EXPONENTIAL) {
     x = um - w * log1p(expm1(um / w) * l);
     f = exp(-x / w) / (w * m.norm);
-    break;
-  case KD_MAP_INVERSE:
+  } else if constexpr (K == KD_MAP_INVERSE) {
     x = w * expm1(l * m.norm);
     f = 1.0 / (m.norm * (w + x));
-    break;
-  case KD_MAP_INVERSE_SQ:
+  } else if constexpr (K == KD_MAP_INVERSE_SQ) {
     x = m.norm * l * w / (1.0 - m.norm * l);
     f = w / ((w + x) * (w + x) * m.norm);
-    break;
-  case KD_MAP_INVERSE_POW: {
+  } else if constexpr (K == KD_MAP_INVERSE_POW) {
     const int pw = m.power;
     double base = pow(w / (um + w), (double)(pw - 1));
     double denom = pow(1.0 + l * (base - 1.0), 1.0 / (pw - 1.0));
     x = w * (1.0 - denom) / denom;
     f = m.norm / pow(w + x, (double)pw);
-    break;
-  }
-  case KD_MAP_NEAREST: {
+  } else if constexpr (K == KD_MAP_NEAREST) {
     const double *F = tab + m.F_off, *t = tab + m.t_off, *fk = tab + m.f_off;
     double y = -l;
-    int i = bsearch_knots(F, m.n_inv, y);
+    int i = bisect_knots(F, guide_bounds(m.n_inv, gd, md.gF, y), y);
     double dx = F[i + 1] - F[i];
     x = dx > 0.0 ? t[i] + (y - F[i]) / dx * (t[i + 1] - t[i]) : 0.0;
     if (y < F[0] || y > F[m.n_inv - 1]) x = __builtin_nan("");
-    int j = bsearch_knots(t, m.n_pts, x);
+    int j = bisect_knots(t, guide_bounds(m.n_pts, gd, md.gt, x), x);
     f = 0.5 * (fk[j] + fk[j + 1]);
-    break;
-  }
-  case KD_MAP_HYBRID: {
+  } else if constexpr (K == KD_MAP_HYBRID) {
     const double *F = tab + m.F_off, *c = tab + m.c_off;
     double y = -l;
-    int i = bsearch_knots(F, m.n_inv, y);
+    int i = bisect_knots(F, guide_bounds(m.n_inv, gd, md.gF, y), y);
     double d = y - F[i];
     x = c[4 * i + 3] + d * (c[4 * i + 2] + d * (c[4 * i + 1] + d * c[4 * i + 0]));
     if (y < F[0] || y > F[m.n_inv - 1]) x = __builtin_nan("");
@@ -420,22 +459,78 @@ __device__ inline double map1d_reverse(const kd_map1d &m, const double *__restri
     for (int q = 1; q < m.power; q++) num *= w;
     for (int q = 1; q < m.power; q++) dd *= den;
     f = (num / dd) / m.norm;
-    break;
-  }
-  case KD_MAP_PROJECTION: {
+  } else {
+    static_assert(K == KD_MAP_PROJECTION, "unknown map kind");
     // tables on the uniform meshes of [0, 1], evaluated by linear interpolation
     // (the TRIQS gf<retime> evaluation of projection.hpp:331-357)
     x = mesh_lin(tab + m.t_off, m.n_inv, l);
     f = mesh_lin(tab + m.f_off, m.n_pts, x) / m.norm;
     if (!(l >= 0.0 && l <= 1.0)) x = __builtin_nan("");
-    break;
-  }
-  default:
-    x = __builtin_nan("");
-    f = __builtin_nan("");
   }
   *u = x;
   return 1.0 / f;
+}
+
+__device__ inline double map1d_reverse(const MapDev &md, const double *__restrict__ tab, const int2 *__restrict__ gd,
+                                       double l, double *u) {
+  switch (md.m.kind) {
+  case KD_MAP_UNIFORM: return map_reverse_k<KD_MAP_UNIFORM>(md, tab, gd, l, u);
+  case KD_MAP_EXPONENTIAL: return map_reverse_k<KD_MAP_EXPONENTIAL>(md, tab, gd, l, u);
+  case KD_MAP_INVERSE: return map_reverse_k<KD_MAP_INVERSE>(md, tab, gd, l, u);
+  case KD_MAP_INVERSE_SQ: return map_reverse_k<KD_MAP_INVERSE_SQ>(md, tab, gd, l, u);
+  case KD_MAP_INVERSE_POW: return map_reverse_k<KD_MAP_INVERSE_POW>(md, tab, gd, l, u);
+  case KD_MAP_NEAREST: return map_reverse_k<KD_MAP_NEAREST>(md, tab, gd, l, u);
+  case KD_MAP_HYBRID: return map_reverse_k<KD_MAP_HYBRID>(md, tab, gd, l, u);
+  case KD_MAP_PROJECTION: return map_reverse_k<KD_MAP_PROJECTION>(md, tab, gd, l, u);
+  default:
+    *u = __builtin_nan("");
+    return __builtin_nan("");
+  }
+}
+
+// N axes of a product of nearest-interpolation maps (map_reverse_k<KD_MAP_NEAREST>
+// on every axis, the same operations): each step runs over all axes before the
+// next starts, so the axes' dependent loads (guide, knot pair, guide, f pair)
+// are in flight together -- four memory round trips per point instead of four
+// per axis.  inv_f[a] = 1/f of axis a.
+template <int N>
+__device__ __forceinline__ void nearest_axes(const MapDev *__restrict__ ms, bool one, const double *__restrict__ tab,
+                                             const int2 *__restrict__ gd, double (&x)[N], double (&inv_f)[N]) {
+  double y[N];
+  int2 r[N];
+  int k[N];
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    const MapDev &md = ms[one ? 0 : a];
+    y[a] = -x[a];
+    r[a] = guide_bounds(md.m.n_inv, gd, md.gF, y[a]);
+  }
+#pragma unroll
+  for (int a = 0; a < N; a++) k[a] = bisect_knots(tab + ms[one ? 0 : a].m.F_off, r[a], y[a]);
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    const kd_map1d &m = ms[one ? 0 : a].m;
+    const double *F = tab + m.F_off, *t = tab + m.t_off;
+    const int i = k[a];
+    const double F0 = F[i], F1 = F[i + 1], t0 = t[i], t1 = t[i + 1];
+    const double dx = F1 - F0;
+    const double q = (y[a] - F0) / dx; // unused (inf or NaN) unless dx > 0
+    double xa = dx > 0.0 ? t0 + q * (t1 - t0) : 0.0;
+    if (y[a] < F[0] || y[a] > F[m.n_inv - 1]) xa = __builtin_nan("");
+    x[a] = xa;
+  }
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    const MapDev &md = ms[one ? 0 : a];
+    r[a] = guide_bounds(md.m.n_pts, gd, md.gt, x[a]);
+  }
+#pragma unroll
+  for (int a = 0; a < N; a++) k[a] = bisect_knots(tab + ms[one ? 0 : a].m.t_off, r[a], x[a]);
+#pragma unroll
+  for (int a = 0; a < N; a++) {
+    const double *fk = tab + ms[one ? 0 : a].m.f_off;
+    inv_f[a] = 1.0 / (0.5 * (fk[k[a]] + fk[k[a] + 1]));
+  }
 }
 
 } // namespace kd

@@ -199,23 +199,33 @@ __device__ __forceinline__ void config_value_pair(const double2 *__restrict__ Wq
 
 // Warper chain, applied last-added first (warpers.hpp:72-85).
 template <int N>
-__device__ __forceinline__ double apply_warpers(const KdParams &p, double (&x)[N]) {
+__device__ __forceinline__ double apply_warpers(const KdParams &p, const WarperDev *W, const MapDev *M,
+                                                double (&x)[N]) {
   double jac = 1.0;
   for (int q = p.n_warpers - 1; q >= 0; q--) {
-    const kd_warper_desc wd = p.warpers[q];
+    const kd_warper_desc wd = W[q].w;
     if (wd.kind == KD_WARP_PLASMA) {
       // ui_from_vi (plasma_uv.hpp:46-51)
       x[0] = wd.t_max - x[0];
 #pragma unroll
       for (int i = 1; i < N; i++) x[i] = x[i - 1] - x[i];
     } else if (wd.kind == KD_WARP_PRODUCT) {
+      // axes in order 0..N-1, the Jacobian multiplied in that order whatever the path
+      const MapDev *ms = M + wd.map_offset;
+      const bool one = wd.n_maps == 1;
+      const int kind = W[q].map_kind;
+      double inv_f[N];
+      if (kind == KD_MAP_NEAREST) {
+        nearest_axes<N>(ms, one, p.tables, p.guides, x, inv_f);
+      } else if (kind == KD_MAP_INVERSE) {
 #pragma unroll
-      for (int i = 0; i < N; i++) {
-        const kd_map1d m = p.maps[wd.map_offset + (wd.n_maps == 1 ? 0 : i)];
-        double xi;
-        jac *= map1d_reverse(m, p.tables, x[i], &xi);
-        x[i] = xi;
+        for (int i = 0; i < N; i++) inv_f[i] = map_reverse_k<KD_MAP_INVERSE>(ms[one ? 0 : i], p.tables, p.guides, x[i], &x[i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) inv_f[i] = map1d_reverse(ms[one ? 0 : i], p.tables, p.guides, x[i], &x[i]);
       }
+#pragma unroll
+      for (int i = 0; i < N; i++) jac *= inv_f[i];
     }
   }
   return jac;
@@ -233,11 +243,22 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
   // bin counts of this workgroup (nbins + 1: the last one counts the dropped
   // values), flushed to the global counters once at the end
   extern __shared__ unsigned long long scnt[];
+  // the chain's descriptors: read from LDS, their loads do not queue behind
+  // the knot-table loads of the axes before them
+  __shared__ WarperDev sW[KD_MAX_WARPERS];
+  __shared__ MapDev sM[KD_MAX_MAPS];
   // Sobol points unless the caller (or the flow stage) supplies the points
   const bool sob = (p.mode == MODE_RUN || p.mode == MODE_SAMPLE) && p.in_pts == nullptr;
   if (sob) {
     for (int i = threadIdx.x; i < N * 31; i += blockDim.x) sC[i] = p.Csr[i];
     if (threadIdx.x < N) sS[threadIdx.x] = p.shfr[threadIdx.x];
+  }
+  if (p.mode != MODE_EVAL_U) {
+    const int nw = p.n_warpers * (int)(sizeof(WarperDev) / 4), nm = p.n_maps * (int)(sizeof(MapDev) / 4);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x)
+      reinterpret_cast<uint32_t *>(sW)[i] = reinterpret_cast<const uint32_t *>(p.warpers)[i];
+    for (int i = threadIdx.x; i < nm; i += blockDim.x)
+      reinterpret_cast<uint32_t *>(sM)[i] = reinterpret_cast<const uint32_t *>(p.maps)[i];
   }
   for (int i = threadIdx.x; i <= p.nbins && p.nbins > 0; i += blockDim.x) scnt[i] = 0ull;
   __syncthreads();
@@ -246,7 +267,7 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const bool spin_ok = p.A.spin == p.B.spin;
   for (int64_t pi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pi - lane < n; pi += stride) {
-    // keep the warper descriptors' loads inside the loop (no hoisting into registers)
+    // keep the descriptors' LDS reads inside the loop (no hoisting into registers)
     __asm__ volatile("" ::: "memory");
     const bool valid = pi < n;
     double x[NU];
@@ -267,7 +288,7 @@ __global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
         for (int i = 0; i < N; i++) x[i] = p.in_pts[pi * N + i];
       }
       if constexpr (N > 0) {
-        if (p.mode != MODE_EVAL_U) jac *= apply_warpers<N>(p, x);
+        if (p.mode != MODE_EVAL_U) jac *= apply_warpers<N>(p, sW, sM, x);
       }
       if (p.keep_u) {
 #pragma unroll

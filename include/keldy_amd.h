@@ -255,10 +255,10 @@ int kd_get_time_bins(kd_ctx *ctx, double *values, uint64_t *counts, uint64_t *dr
  * the ctx's chain (from the current generator state, which does not advance;
  * the accumulator is untouched) for every candidate -- the split, paired and
  * tree algorithms and the per-configuration kernel at 64, 32 or 16 lanes per
- * point, each at 8, 16, 32 or 64 waves per CU -- and keeps the fastest for this
- * ctx (lanes = 0 when the tree or the split algorithm wins).  Without tuning
- * or kd_set_launch_shape (waves_per_cu in [1, 128]) the grid holds 64 waves per
- * CU for KD_ALGO_CTREE, 32 for KD_ALGO_WSPLIT, 16 otherwise.  kd_run_batch runs several
+ * point, each at 16, 32, 64 or 128 waves per CU -- and keeps the fastest for
+ * this ctx (lanes = 0 when the tree or the split algorithm wins).  Without
+ * tuning or kd_set_launch_shape (waves_per_cu in [1, 128]) the grid holds 128
+ * waves per CU for KD_ALGO_CTREE, 32 for KD_ALGO_WSPLIT, 16 otherwise.  kd_run_batch runs several
  * ctxs (e.g. orders 1..10, uneven point counts) as ONE hipGraph launch: one
  * graph branch per ctx (its persistent kernels on its own stream), forked from
  * and joined to ctxs[0]'s stream; each ctx advances and accumulates as by

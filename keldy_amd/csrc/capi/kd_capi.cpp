@@ -190,13 +190,13 @@ int64_t vals_per_point(const kd_ctx *c) {
 
 // Waves per CU of the persistent Keldysh-sum grid: the tuned or set value, else
 // the algorithm's default.  The one-point-per-wave kernels give each wave a
-// fixed share of the points, so a grid of several times the waves that fit at
+// fixed share of the points, so a grid of many times the waves that fit at
 // once lets the dispatcher balance the shares across the CUs (order 8, the
-// time-ordered tree: 5.50 ms per 2^21 points at 64 waves per CU, 6.24 ms at
-// 16; the wave-per-point split: 9.7 ms at 24-32, 9.9 ms at 16).
+// time-ordered tree, 2^21 points: 4.07 ms at 128 waves per CU, 4.18 at 64,
+// 4.36 at 32; the wave-per-point split: 9.7 ms at 24-32, 9.9 ms at 16).
 int grid_waves_per_cu(const kd_ctx *c) {
   if (c->lu_waves_per_cu > 0) return c->lu_waves_per_cu;
-  if (use_ctree(c)) return 64;
+  if (use_ctree(c)) return 128;
   if (use_wsplit(c)) return 32;
   return 16;
 }
@@ -211,6 +211,11 @@ int grid_for(const kd_ctx *c, int64_t n_points) {
   // waves per CU -> workgroups per CU (the split kernel's workgroups hold several waves)
   const int wg_waves = use_split(c) ? std::max(1, kd::split_threads(c->order) / 64) : 1;
   int64_t cap = (int64_t)(c->n_cu > 0 ? c->n_cu : 256) * std::max(1, grid_waves_per_cu(c) / wg_waves);
+  // the time-ordered tree below the cap: at least three candidate points per
+  // wave (order 8, 54,875 points: 121 us at 3.3 per wave, 124 at 1.7), six with
+  // two points per wave, so that each wave has a few pairs (that kernel at
+  // order 8: 146 us at 8 points per wave, 180 at 3)
+  if (use_ctree(c)) nb = std::min(nb, kd::ctree_pairs(c->order) ? (n_points + 5) / 6 : (n_points + 2) / 3);
   if (nb > cap) nb = cap;
   return (int)(nb < 1 ? 1 : nb);
 }
@@ -1551,7 +1556,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   if ((uint64_t)c->k + (uint64_t)sample_points > (1ull << 31))
     return fail(c, KD_ERR_RANGE, "More sample points requested then generator can hold.");
   KD_HIP(c, hipSetDevice(c->device));
-  const int lane_opts[3] = {64, 32, 16}, wave_opts[4] = {8, 16, 32, 64};
+  const int lane_opts[3] = {64, 32, 16}, wave_opts[4] = {16, 32, 64, 128};
   const int old_l = c->lu_lanes_max, old_w = c->lu_waves_per_cu, old_a = c->algo;
   int best_l = old_l, best_w = old_w, best_a = old_a;
   float best = 1e30f;
@@ -1562,8 +1567,8 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
   c->no_reduce = true;
   // candidates: the split tree + per-configuration LU (orders 6..10), the
   // paired LU (orders >= 7), the tree (orders >= 6; these at 64 lanes) and the
-  // per-configuration kernel with 64, 32 or 16 lanes per point, each at 8, 16,
-  // 32 or 64 waves per CU
+  // per-configuration kernel with 64, 32 or 16 lanes per point, each at 16, 32,
+  // 64 or 128 waves per CU
   const bool direct = c->obs == KD_OBS_DIRECT;
   const int li0 = !direct                           ? 0
                   : kd::ctree_supported(c->order)    ? -5

@@ -41,8 +41,11 @@ hipError_t launch_split(int order, int precision, const KdParams &p, const PrepB
 bool wsplit_supported(int order);
 hipError_t launch_wsplit(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // integrand_ctree_kernel<order, T> (orders 6..10): the time-ordered tree of
-// vertex-level corrections, one point per 64-thread workgroup (one wave).
+// vertex-level corrections, 64-thread workgroups (one wave) of one point each,
+// or of two points (one per half) where ctree_pairs(order).
 bool ctree_supported(int order);
+// whether the time-ordered tree runs two points per wave (one per half)
+bool ctree_pairs(int order);
 hipError_t launch_ctree(int order, int precision, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
 // cofactor_kernel<order> (kernel method, orders 1..12, FP64): grid blocks of
 // 64 threads, same work items as the Keldysh-sum kernel.

@@ -45,4 +45,23 @@ __device__ __forceinline__ double wave_max(double x) {
   return fmax(fmax(lane_f64(x, 0), lane_f64(x, 16)), fmax(lane_f64(x, 32), lane_f64(x, 48)));
 }
 
+// The same reductions over each half (32 lanes) of the wave: every lane gets
+// the value of its half (upper: lanes 32..63).
+__device__ __forceinline__ double half_sum(double x, bool upper) {
+  x += dpp_f64<0xB1>(x);
+  x += dpp_f64<0x4E>(x);
+  x += dpp_f64<0x141>(x);
+  x += dpp_f64<0x140>(x);
+  const double lo = lane_f64(x, 0) + lane_f64(x, 16), hi = lane_f64(x, 32) + lane_f64(x, 48);
+  return upper ? hi : lo;
+}
+__device__ __forceinline__ double half_max(double x, bool upper) {
+  x = fmax(x, dpp_f64<0xB1>(x));
+  x = fmax(x, dpp_f64<0x4E>(x));
+  x = fmax(x, dpp_f64<0x141>(x));
+  x = fmax(x, dpp_f64<0x140>(x));
+  const double lo = fmax(lane_f64(x, 0), lane_f64(x, 16)), hi = fmax(lane_f64(x, 32), lane_f64(x, 48));
+  return upper ? hi : lo;
+}
+
 } // namespace kd

@@ -66,7 +66,7 @@ def test_autotune(flat_g0, order):
     lanes, wpc, ms = ctx.autotune(1 << 16)
     # lanes == 0: the Schur-tree algorithm won (orders >= 6)
     assert lanes in (16, 32, 64) or lanes == 1 << order or (lanes == 0 and order >= 6)
-    assert wpc in (8, 16, 32) and ms > 0
+    assert wpc in (16, 32, 64, 128) and ms > 0
     assert ctx.sobol_state() == 0 and ctx.accum().n_points == 0 and ctx.accum().re == 0.0
     a = ctx.run(1 << 16)
     ref = _ctx(flat_g0, order, 7)

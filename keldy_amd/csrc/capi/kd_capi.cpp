@@ -39,7 +39,8 @@ struct kd_ctx {
   kd::WarperDev *d_warpers = nullptr;
   kd::MapDev *d_maps = nullptr;
   double *d_tables = nullptr;
-  int2 *d_guides = nullptr; // knot-search guides of the maps
+  int2 *d_guides = nullptr;   // knot-search guides of the maps
+  double *d_knots = nullptr;  // interval records of the nearest maps
   int n_warpers = 0, n_maps = 0;
   std::vector<kd_warper_desc> h_warpers;
   // learned flow warper (KD_WARP_FLOW): FP32 parameters, Adam moments,
@@ -256,6 +257,7 @@ kd::KdParams base_params(const kd_ctx *c) {
   p.maps = c->d_maps;
   p.tables = c->d_tables;
   p.guides = c->d_guides;
+  p.knots = c->d_knots;
   p.n_warpers = c->n_warpers;
   p.n_maps = c->n_maps;
   p.Csr = c->d_Csr;
@@ -543,6 +545,7 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_maps);
   dfree(c->d_tables);
   dfree(c->d_guides);
+  dfree(c->d_knots);
   dfree(c->d_fw);
   dfree(c->d_fm);
   dfree(c->d_fv);
@@ -815,12 +818,27 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
   }
   std::vector<kd::MapDev> dev(n_maps);
   std::vector<int2> guides(1, int2{0, 1}); // entry 0: what maps without a guide read (and ignore)
+  std::vector<double> knots(4, 0.0); // records start 32-byte aligned (the buffer is)
   for (int i = 0; i < n_maps; i++) {
     const kd_map1d &m = maps[i];
     dev[i].m = m;
     dev[i].gF = dev[i].gt = kd::KnotGuide{0.0, 0.0, 0, 0};
+    dev[i].rec = dev[i].fpair = 0;
+    dev[i].F0 = dev[i].Fl = 0.0;
     if (m.kind == KD_MAP_NEAREST || m.kind == KD_MAP_HYBRID) dev[i].gF = knot_guide(tables + m.F_off, m.n_inv, guides);
-    if (m.kind == KD_MAP_NEAREST) dev[i].gt = knot_guide(tables + m.t_off, m.n_pts, guides);
+    if (m.kind == KD_MAP_NEAREST) {
+      dev[i].gt = knot_guide(tables + m.t_off, m.n_pts, guides);
+      // interval records: i in [0, n_inv - 1) for the inverse CDF, j in
+      // [0, n_pts - 1) for f (the searches return intervals in those ranges)
+      const double *F = tables + m.F_off, *t = tables + m.t_off, *f = tables + m.f_off;
+      dev[i].F0 = F[0];
+      dev[i].Fl = F[m.n_inv - 1];
+      dev[i].rec = (int64_t)knots.size();
+      for (int k = 0; k + 1 < m.n_inv; k++) knots.insert(knots.end(), {F[k], F[k + 1], t[k], t[k + 1]});
+      dev[i].fpair = (int64_t)knots.size();
+      for (int k = 0; k + 1 < m.n_pts; k++) knots.insert(knots.end(), {f[k], f[k + 1]});
+      if (knots.size() & 3) knots.insert(knots.end(), 4 - (knots.size() & 3), 0.0);
+    }
   }
   KD_HIP(c, hipSetDevice(c->device));
   KD_HIP(c, hipStreamSynchronize(c->stream));
@@ -828,6 +846,7 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
   dfree(c->d_maps);
   dfree(c->d_tables);
   dfree(c->d_guides);
+  dfree(c->d_knots);
   if (n_w) {
     KD_HIP(c, hipMalloc(&c->d_warpers, sizeof(kd::WarperDev) * n_w));
     KD_HIP(c, hipMemcpy(c->d_warpers, wdev.data(), sizeof(kd::WarperDev) * n_w, hipMemcpyHostToDevice));
@@ -837,6 +856,8 @@ int kd_set_warper_chain(kd_ctx *c, const kd_warper_desc *w, int32_t n_w, const k
     KD_HIP(c, hipMemcpy(c->d_maps, dev.data(), sizeof(kd::MapDev) * n_maps, hipMemcpyHostToDevice));
   }
   if (n_maps) {
+    KD_HIP(c, hipMalloc(&c->d_knots, sizeof(double) * knots.size()));
+    KD_HIP(c, hipMemcpy(c->d_knots, knots.data(), sizeof(double) * knots.size(), hipMemcpyHostToDevice));
     KD_HIP(c, hipMalloc(&c->d_guides, sizeof(int2) * guides.size()));
     KD_HIP(c, hipMemcpy(c->d_guides, guides.data(), sizeof(int2) * guides.size(), hipMemcpyHostToDevice));
   }

@@ -71,6 +71,12 @@ struct KnotGuide {
 struct MapDev {
   kd_map1d m;
   KnotGuide gF, gt;
+  // nearest maps: interval records in the knot buffer -- {F[i], F[i+1], t[i],
+  // t[i+1]} at rec + 4 i and {f[j], f[j+1]} at fpair + 2 j (in doubles) -- so
+  // that an interval's knots come in one aligned load; F[0] and F[n_inv - 1]
+  // (the range of the inverse CDF)
+  int64_t rec, fpair;
+  double F0, Fl;
 };
 // A warper of the chain; for a product warper, map_kind is the kind every one
 // of its maps has (-1: kinds differ), so that the axes run as one straight
@@ -94,7 +100,8 @@ struct KdParams {
   const WarperDev *warpers;
   const MapDev *maps;
   const double *tables;
-  const int2 *guides; // bisection bounds of the knot searches (KnotGuide)
+  const int2 *guides;   // bisection bounds of the knot searches (KnotGuide)
+  const double *knots;  // interval records of the nearest maps (MapDev::rec, fpair)
   int n_warpers;       // warpers applied (<= KD_MAX_WARPERS)
   int n_maps;          // maps of the chain (<= KD_MAX_MAPS)
   // Sobol (qrng.hpp:49-114): bit-reversed matrices [dim][31], shifts [dim]
@@ -495,7 +502,8 @@ __device__ inline double map1d_reverse(const MapDev &md, const double *__restric
 // per axis.  inv_f[a] = 1/f of axis a.
 template <int N>
 __device__ __forceinline__ void nearest_axes(const MapDev *__restrict__ ms, bool one, const double *__restrict__ tab,
-                                             const int2 *__restrict__ gd, double (&x)[N], double (&inv_f)[N]) {
+                                             const int2 *__restrict__ gd, const double *__restrict__ kn,
+                                             double (&x)[N], double (&inv_f)[N]) {
   double y[N];
   int2 r[N];
   int k[N];
@@ -507,17 +515,28 @@ __device__ __forceinline__ void nearest_axes(const MapDev *__restrict__ ms, bool
   }
 #pragma unroll
   for (int a = 0; a < N; a++) k[a] = bisect_knots(tab + ms[one ? 0 : a].m.F_off, r[a], y[a]);
+  // the axes' interval records, in groups of four axes whose loads are all
+  // issued before the first use (registers for the records of one group)
+  constexpr int GA = 4;
 #pragma unroll
-  for (int a = 0; a < N; a++) {
-    const kd_map1d &m = ms[one ? 0 : a].m;
-    const double *F = tab + m.F_off, *t = tab + m.t_off;
-    const int i = k[a];
-    const double F0 = F[i], F1 = F[i + 1], t0 = t[i], t1 = t[i + 1];
-    const double dx = F1 - F0;
-    const double q = (y[a] - F0) / dx; // unused (inf or NaN) unless dx > 0
-    double xa = dx > 0.0 ? t0 + q * (t1 - t0) : 0.0;
-    if (y[a] < F[0] || y[a] > F[m.n_inv - 1]) xa = __builtin_nan("");
-    x[a] = xa;
+  for (int a0 = 0; a0 < N; a0 += GA) {
+    double4 rec[GA];
+#pragma unroll
+    for (int a = a0; a < a0 + GA && a < N; a++)
+      rec[a - a0] = *reinterpret_cast<const double4 *>(kn + ms[one ? 0 : a].rec + 4 * (int64_t)k[a]); // F[i], F[i+1], t[i], t[i+1]
+#pragma unroll
+    for (int a = a0; a < a0 + GA && a < N; a++) {
+      const MapDev &md = ms[one ? 0 : a];
+      const double4 r = rec[a - a0];
+      const double dx = r.y - r.x;
+      // on every lane (inf or NaN, and unused, unless dx > 0): kept out of a
+      // branch, which would hold the loads of the axes after it back
+      double xa = r.z + (y[a] - r.x) / dx * (r.w - r.z);
+      __asm__ volatile("" : "+v"(xa));
+      xa = dx > 0.0 ? xa : 0.0;
+      const bool out = (y[a] < md.F0) | (y[a] > md.Fl);
+      x[a] = out ? __builtin_nan("") : xa;
+    }
   }
 #pragma unroll
   for (int a = 0; a < N; a++) {
@@ -528,8 +547,8 @@ __device__ __forceinline__ void nearest_axes(const MapDev *__restrict__ ms, bool
   for (int a = 0; a < N; a++) k[a] = bisect_knots(tab + ms[one ? 0 : a].m.t_off, r[a], x[a]);
 #pragma unroll
   for (int a = 0; a < N; a++) {
-    const double *fk = tab + ms[one ? 0 : a].m.f_off;
-    inv_f[a] = 1.0 / (0.5 * (fk[k[a]] + fk[k[a] + 1]));
+    const double2 f = *reinterpret_cast<const double2 *>(kn + ms[one ? 0 : a].fpair + 2 * (int64_t)k[a]); // f[j], f[j+1]
+    inv_f[a] = 1.0 / (0.5 * (f.x + f.y));
   }
 }
 

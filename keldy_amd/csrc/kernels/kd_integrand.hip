@@ -216,7 +216,7 @@ __device__ __forceinline__ double apply_warpers(const KdParams &p, const WarperD
       const int kind = W[q].map_kind;
       double inv_f[N];
       if (kind == KD_MAP_NEAREST) {
-        nearest_axes<N>(ms, one, p.tables, p.guides, x, inv_f);
+        nearest_axes<N>(ms, one, p.tables, p.guides, p.knots, x, inv_f);
       } else if (kind == KD_MAP_INVERSE) {
 #pragma unroll
         for (int i = 0; i < N; i++) inv_f[i] = map_reverse_k<KD_MAP_INVERSE>(ms[one ? 0 : i], p.tables, p.guides, x[i], &x[i]);
@@ -235,8 +235,12 @@ __device__ __forceinline__ double apply_warpers(const KdParams &p, const WarperD
 // one wavefront.  l (Sobol point or caller input) -> warper chain -> u, J and
 // the domain check (wick_direct.cpp:53-57).  In-domain points of batch b are
 // compacted, in lane order, to slots [64 b, 64 b + cnt[b]).
+// three waves per SIMD up to order 8 (the point chains are latency-bound: more
+// waves in flight), two above (whose coordinates would spill at three)
+template <int N> constexpr int prep_waves_per_eu() { return N <= 8 ? 3 : 2; }
 template <int N>
-__global__ __launch_bounds__(256) void prep_kernel(KdParams p, PrepBuf buf) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(prep_waves_per_eu<N>()))) void
+prep_kernel(KdParams p, PrepBuf buf) {
   constexpr int NU = N > 0 ? N : 1;
   __shared__ uint32_t sC[NU * 31];
   __shared__ uint32_t sS[NU];

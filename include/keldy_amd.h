@@ -221,7 +221,16 @@ int kd_sample_device(kd_ctx *ctx, uint64_t n_steps, double *d_l, double *d_value
 
 /* Timing of the last kd_run / kd_eval_* / kd_sample_device call, from HIP
  * events on the ctx stream: whole call, the Keldysh-sum (LU) kernels and the
- * point-preparation kernels, and the number of kernel launches made. */
+ * point-preparation kernels, and the number of kernel launches made.  What is
+ * timed is set by kd_set_timing (no counterpart in the reference):
+ * KD_TIMING_KERNELS (the default) all three; KD_TIMING_TOTAL the whole call
+ * only (lu_ms = prep_ms = 0); KD_TIMING_NONE nothing (all 0): each timing
+ * event is a queue packet, and on a run of ~5e4 points the events of
+ * KD_TIMING_KERNELS cost ~20 us of wall time. */
+#define KD_TIMING_NONE 0
+#define KD_TIMING_TOTAL 1
+#define KD_TIMING_KERNELS 2
+int kd_set_timing(kd_ctx *ctx, int32_t level);
 int kd_last_timing(const kd_ctx *ctx, double *total_ms, double *lu_ms, double *prep_ms, int32_t *launches);
 /* Executed FP operations per point of the last configured order/precision
  * (the algorithm's own count, used for roofline figures). */

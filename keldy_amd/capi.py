@@ -23,6 +23,7 @@ KD_MAP_PROJECTION = 7
 KD_WARP_IDENTITY, KD_WARP_PLASMA, KD_WARP_PRODUCT, KD_WARP_FLOW = 0, 1, 2, 3
 KD_ALGO_DEFAULT, KD_ALGO_CONFIG, KD_ALGO_TREE, KD_ALGO_PAIR, KD_ALGO_SPLIT, KD_ALGO_WSPLIT = -1, 0, 1, 2, 3, 4
 KD_ALGO_CTREE = 5
+KD_TIMING_NONE, KD_TIMING_TOTAL, KD_TIMING_KERNELS = 0, 1, 2
 KD_OBS_DIRECT, KD_OBS_KERNEL, KD_OBS_KERNEL_OMEGA = 0, 1, 2
 
 
@@ -52,7 +53,7 @@ class Accum(C.Structure):
 SYMBOLS = ["kd_create", "kd_destroy", "kd_last_error", "kd_device_count", "kd_set_g0_table",
            "kd_set_integrand", "kd_set_precision", "kd_eval_batch", "kd_eval_batch_device",
            "kd_set_warper_chain", "kd_eval_warped_batch", "kd_set_sobol", "kd_get_sobol_state",
-           "kd_run", "kd_get_accum", "kd_reset_accum", "kd_sample_device", "kd_last_timing",
+           "kd_run", "kd_get_accum", "kd_reset_accum", "kd_sample_device", "kd_set_timing", "kd_last_timing",
            "kd_flops_per_point", "kd_flow_n_params", "kd_flow_init", "kd_flow_set_params",
            "kd_flow_get_params", "kd_flow_map", "kd_flow_loss_grad", "kd_flow_collect", "kd_flow_train",
            "kd_autotune", "kd_set_launch_shape", "kd_get_launch_shape", "kd_run_batch",
@@ -94,6 +95,7 @@ def lib():
         L.kd_get_accum.argtypes = [vp, C.POINTER(Accum)]
         L.kd_reset_accum.argtypes = [vp]
         L.kd_sample_device.argtypes = [vp, C.c_uint64, vp, vp]
+        L.kd_set_timing.argtypes = [vp, C.c_int32]
         L.kd_last_timing.argtypes = [vp, dp, dp, dp, ip]
         L.kd_flops_per_point.argtypes = [vp, dp]
         fp = C.POINTER(C.c_float)
@@ -234,6 +236,10 @@ class Context:
         k = C.c_uint32()
         self._check(lib().kd_get_sobol_state(self.h, C.byref(k)))
         return k.value
+
+    def set_timing(self, level):
+        """KD_TIMING_NONE, KD_TIMING_TOTAL or KD_TIMING_KERNELS (the default)."""
+        self._check(lib().kd_set_timing(self.h, int(level)))
 
     def last_timing(self):
         """(total_ms, lu_ms, prep_ms, launches) of the last device call."""

@@ -183,6 +183,9 @@ template <typename T, typename C> void bind_integrator_methods(C &c) {
                                 : v == KD_ALGO_CTREE ? "ctree"
                                                      : "config");
            })
+      .def("set_timing", &T::set_timing, py::arg("level"),
+           "what last_timing measures: 0 nothing, 1 the whole call, 2 also the kernels (default); each "
+           "timing event costs wall time on small runs")
       .def("last_timing", &T::last_timing)
       .def("flops_per_point", &T::flops_per_point)
       .def("get_integrand_dimension", &T::get_dimension)

@@ -85,6 +85,8 @@ struct kd_ctx {
   int prep_order = -1;
   float last_ms = 0.f, last_lu_ms = 0.f, last_prep_ms = 0.f;
   int last_launches = 0, last_chunks = 0;
+  int timing = KD_TIMING_KERNELS; // what kd_run / kd_eval_* / kd_sample_device time (kd_set_timing)
+  bool last_kernels_timed = false;
   // launch shape of the Keldysh-sum kernel (kd_autotune): maximum lanes per
   // point and waves per CU of the persistent grid (0: the algorithm's default)
   int lu_lanes_max = 64, lu_waves_per_cu = 0;
@@ -339,8 +341,10 @@ int flow_grid(const kd_ctx *c, int64_t n) {
   return (int)std::max<int64_t>(1, std::min(nb, cap));
 }
 
-int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepBuf *ext = nullptr);
-int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) { return launch_chunks(c, p0, true); }
+// timing: KD_TIMING_NONE, _TOTAL (events around the launches) or _KERNELS
+// (also around each chunk's prep and Keldysh-sum kernels)
+int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepBuf *ext = nullptr);
+int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) { return launch_chunks(c, p0, c->timing); }
 
 // Allocate every buffer a launch of p0 needs (before a graph capture).
 int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
@@ -359,7 +363,9 @@ int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
 // each kernel is bracketed by HIP events on the ctx stream (kd_last_timing).
 // `ext`: a compacted-point buffer of the caller's for up to kChunk points (no
 // allocation of the ctx's own); only for evaluation modes without the flow stage.
-int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepBuf *ext) {
+int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepBuf *ext) {
+  const bool timed = timing != KD_TIMING_NONE;
+  const bool kernels = timing == KD_TIMING_KERNELS;
   kd::PrepBuf buf;
   int rc = KD_OK;
   if (ext)
@@ -375,7 +381,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepB
     if ((rc = ensure_flow_stage(c, N))) return rc;
   }
   const int64_t nchunks = (p0.n_points + kChunk - 1) / kChunk;
-  if (timed && (rc = ensure_events(c, size_t(4 * nchunks)))) return rc;
+  if (kernels && (rc = ensure_events(c, size_t(4 * nchunks)))) return rc;
   int launches = 0;
   if (timed) KD_HIP(c, hipEventRecord(c->ev0, c->stream));
   int64_t ci = 0;
@@ -400,7 +406,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepB
     } else {
       p.nbins = 0;
     }
-    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], c->stream));
+    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], c->stream));
     if (flow) {
       kd::FlowMapIO io{};
       io.Csr = p.Csr;
@@ -421,8 +427,8 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepB
       p.n_warpers = c->n_warpers - 1; // flow suppressed: the chain without it
     }
     KD_HIP(c, kd::launch_prep(N, p, buf, grid_prep, c->stream));
-    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
-    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
+    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
+    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
     if (c->obs != KD_OBS_DIRECT)
       KD_HIP(c, kd::launch_cofactor(N, p, buf, grid, c->stream));
     else if (use_tree(c))
@@ -435,7 +441,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepB
       KD_HIP(c, kd::launch_wsplit(N, c->precision, p, buf, grid, c->stream));
     else
       KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, use_pair(c), p, buf, grid, c->stream));
-    if (timed) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
+    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
     launches += 2;
     if (p.mode == kd::MODE_RUN && !c->no_reduce) {
       KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->d_acc, c->stream));
@@ -449,6 +455,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, bool timed, const kd::PrepB
   if (timed) KD_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->last_launches = launches;
   c->last_chunks = timed ? (int)nchunks : 0;
+  c->last_kernels_timed = kernels;
   return KD_OK;
 }
 
@@ -482,10 +489,14 @@ int set_bins(kd_ctx *c, double xmin, double xmax, int nbins, int k) {
 int rc_bins_clear(kd_ctx *c) { return set_bins(c, 0.0, 1.0, 0, 1); }
 
 int finish_timing(kd_ctx *c) {
+  if (c->last_chunks == 0) { // untimed (KD_TIMING_NONE) or nothing launched
+    c->last_ms = c->last_lu_ms = c->last_prep_ms = 0.f;
+    return KD_OK;
+  }
   KD_HIP(c, hipEventSynchronize(c->ev1));
   float ms = 0.f, lu = 0.f, prep = 0.f, x = 0.f;
   KD_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  for (int i = 0; i < c->last_chunks; i++) {
+  for (int i = 0; i < (c->last_kernels_timed ? c->last_chunks : 0); i++) {
     KD_HIP(c, hipEventElapsedTime(&x, c->ev[4 * i + 0], c->ev[4 * i + 1]));
     prep += x;
     KD_HIP(c, hipEventElapsedTime(&x, c->ev[4 * i + 2], c->ev[4 * i + 3]));
@@ -697,7 +708,7 @@ int kd_eval_batch_device_noalloc(kd_ctx *c, const double *d_u, int64_t n_points,
   p.n_points = n_points;
   p.out_vals = d_out;
   p.out_dom = d_in_domain;
-  return launch_chunks(c, p, false, &buf);
+  return launch_chunks(c, p, KD_TIMING_NONE, &buf);
 }
 
 int kd_eval_batch(kd_ctx *c, const double *u, int64_t n_points, int32_t keep_u_hypercube, double *out,
@@ -1020,6 +1031,14 @@ int kd_sample_device(kd_ctx *c, uint64_t n_steps, double *d_l, double *d_value) 
   int rc = run_common(c, n_steps, kd::MODE_SAMPLE, d_l, d_value);
   if (rc) return rc;
   if (n_steps) return finish_timing(c);
+  return KD_OK;
+}
+
+int kd_set_timing(kd_ctx *c, int32_t level) {
+  if (!c) return KD_ERR_ARG;
+  if (level != KD_TIMING_NONE && level != KD_TIMING_TOTAL && level != KD_TIMING_KERNELS)
+    return fail(c, KD_ERR_ARG, "timing level must be KD_TIMING_NONE, KD_TIMING_TOTAL or KD_TIMING_KERNELS");
+  c->timing = level;
   return KD_OK;
 }
 
@@ -1615,7 +1634,7 @@ int kd_autotune(kd_ctx *c, int64_t sample_points, int32_t *lanes, int32_t *waves
       c->lu_waves_per_cu = wave_opts[wi];
       float t = 1e30f;
       for (int rep = 0; rep < 3 && !rc; rep++) { // first repetition warms up
-        if ((rc = launch_chunks(c, p, true)) || (rc = finish_timing(c))) break;
+        if ((rc = launch_chunks(c, p, KD_TIMING_KERNELS)) || (rc = finish_timing(c))) break;
         if (rep > 0) t = std::min(t, c->last_lu_ms);
       }
       if (!rc && t < best) {
@@ -1726,7 +1745,7 @@ int kd_run_batch(kd_ctx *const *ctxs, const uint64_t *n_steps, int32_t count, kd
         rc = fail(c0, KD_ERR_HIP, "hipStreamWaitEvent failed");
         break;
       }
-      if ((rc = launch_chunks(c, ps[i], false))) {
+      if ((rc = launch_chunks(c, ps[i], KD_TIMING_NONE))) {
         rc = fail(c0, rc, c->err);
         break;
       }

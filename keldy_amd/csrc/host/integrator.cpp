@@ -457,6 +457,8 @@ uint64_t integrator_direct::reduce_nr_fallback() const {
   return acc.n_fallback;
 }
 
+void integrator_direct::set_timing(int level) { ctx_.check(kd_set_timing(ctx_.get(), level)); }
+
 std::vector<double> integrator_direct::last_timing() const {
   double t = 0, lu = 0, pr = 0;
   int32_t n = 0;

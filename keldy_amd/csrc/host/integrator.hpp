@@ -185,6 +185,8 @@ class integrator_direct {
   void set_precision(int precision);
   int get_precision() const { return precision_; }
   /// timing of the last device call (total, LU kernels, prep kernels; ms)
+  // KD_TIMING_NONE / _TOTAL / _KERNELS (the default): what last_timing measures
+  void set_timing(int level);
   std::vector<double> last_timing() const;
   double flops_per_point() const;
   device_ctx const &ctx() const { return ctx_; }

@@ -1176,19 +1176,28 @@ static double wsplit_flops(int N) {
 // x = p (p c_ee + u_e v_e) and its signed FP64 sum, per pair of leaves dp^2 and
 // its product with that sum.
 static double ctree_flops(int N) {
+  // levels 2 and 3 breadth first: per item a pivot (sub, reciprocal), the row
+  // factor, a subtraction and a multiply-add per entry and the path product;
+  // the level-2 items also form their parent's level-1 entries from Wv
+  // (2 rc + 4 complex products: level 1 has no pass of its own)
   double f = 0.0;
-  for (int k = 0; k < 3; k++) {
+  for (int k = 1; k < 3; k++) {
     const double rc = N - k, items = std::ldexp(rc, k + 1);
-    const double per_item = (k > 0 ? 2 + 2 + 7 : 0) + 6 + (k > 0 ? rc * (2 + 8) : rc * 6) + 6;
+    double per_item = 2 + 2 + 7 + 6 + rc * (2 + 8) + 6;
+    if (k == 1) per_item += (2 * rc + 4) * 6;
     f += items * per_item;
   }
   const double leaf = 2 + 4 + 20 + 4, leaf_pair = 6 + 6 + 2;
   auto step = [&](int k) { const double r = N - k + 1; return 2 + 7 + (r - 1) * (2 + 2 + 6) + 6; };
+  // each lane's path from its level-3 node to its own node at level kf (the
+  // entries of levels 4..kf), then its subtree; lanes per point: 64, or 32
+  // when two points share a wave (the lane's node at level 5)
+  const bool pairs = kd::ctree_pairs(N);
+  const int kf = pairs ? 5 : N <= 9 ? 6 : 7, passes = N == 10 ? 2 : 1;
   double lane = 0.0;
-  if (N == 6) {
+  if (!pairs && N == 6) {
     lane = step(3) + step(4) + 8.0 * (9 + 4) + leaf + leaf_pair; // levels 4 and 5 entries, the lane's one leaf
   } else {
-    const int kf = N <= 9 ? 6 : 7, passes = N <= 9 ? 1 : 2; // the lane's own node level
     double path = 0.0;
     for (int k = 3; k < kf; k++) path += step(k);
     for (int k = 4; k <= kf; k++) path += 8.0 * (N - k + 1) * (N - k + 1); // entries of levels 4..kf
@@ -1200,7 +1209,7 @@ static double ctree_flops(int N) {
     sub += std::ldexp(leaf_pair, N - 1 - kf) + std::ldexp(leaf, N - kf);
     lane = passes * (path + sub);
   }
-  return f + 64.0 * lane;
+  return f + (pairs ? 32.0 : 64.0) * lane;
 }
 
 int kd_flops_per_point(const kd_ctx *c, double *flops) {

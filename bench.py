@@ -532,7 +532,9 @@ def main():
         comp.set_precision("fp32")
         run_replicas(comp, 1000, 1, args.points)
         est32, _, lu32, walls32 = run_replicas(comp, 0, args.steps, args.points)
+        comp.set_timing(0)  # as for the FP64 time to target: no timing events
         walls32_at = run_replicas(comp, 30_000, 32, int(np.ceil(plain["n_needed"])))[3]
+        comp.set_timing(2)
         comp.set_precision("fp64")
         pps32 = args.steps * args.points / float(walls32.sum())
         dev = np.abs(est32 - plain["est"]) / abs(plain["mean"])

@@ -82,6 +82,15 @@ struct kd_ctx {
   // compacted points between the two kernel stages
   void *d_prep = nullptr;
   size_t prep_cap = 0, prep_pts = 0;
+  // runs of several chunks: a second record buffer and stream, so that the
+  // next chunk's points are prepared while the Keldysh sum of this one runs
+  void *d_prep2 = nullptr;
+  size_t prep2_pts = 0;
+  int prep2_order = -1;
+  hipStream_t stream2 = nullptr;
+  // [2 c]: chunk c prepared (on stream2; chunk 0 on the ctx stream), [2 c + 1]:
+  // its records read by the Keldysh sum (ctx stream)
+  std::vector<hipEvent_t> ev_sync;
   int prep_order = -1;
   float last_ms = 0.f, last_lu_ms = 0.f, last_prep_ms = 0.f;
   int last_launches = 0, last_chunks = 0;
@@ -312,6 +321,37 @@ int ensure_prep(kd_ctx *c, int order, int64_t n_points, kd::PrepBuf *buf) {
   return KD_OK;
 }
 
+// The second record buffer, stream and synchronisation events of a run of
+// n_points > kChunk (nothing otherwise).
+int ensure_pipeline(kd_ctx *c, int order, int64_t n_points, kd::PrepBuf *buf2) {
+  if (n_points <= kChunk) return KD_OK;
+  const size_t n = (size_t)kChunk;
+  const size_t per_pt = sizeof(double) * (size_t)(order > 0 ? order : 1) + sizeof(double) + sizeof(int);
+  if (n > c->prep2_pts || c->prep2_order != order) {
+    dfree(c->d_prep2);
+    c->prep2_pts = 0;
+    KD_HIP(c, hipMalloc(&c->d_prep2, n * per_pt + (n / 64) * sizeof(int) + 256));
+    c->prep2_pts = n;
+    c->prep2_order = order;
+  }
+  if (!c->stream2) KD_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+  const size_t nev = 2 * (size_t)((n_points + kChunk - 1) / kChunk);
+  while (c->ev_sync.size() < nev) {
+    hipEvent_t e;
+    KD_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->ev_sync.push_back(e);
+  }
+  char *base = (char *)c->d_prep2;
+  buf2->u = (double *)base;
+  base += n * sizeof(double) * (size_t)(order > 0 ? order : 1);
+  buf2->jac = (double *)base;
+  base += n * sizeof(double);
+  buf2->idx = (int *)base;
+  base += n * sizeof(int);
+  buf2->cnt = (int *)base;
+  return KD_OK;
+}
+
 int ensure_events(kd_ctx *c, size_t n) {
   while (c->ev.size() < n) {
     hipEvent_t e;
@@ -348,9 +388,10 @@ int launch_timed(kd_ctx *c, const kd::KdParams &p0, int /*unused*/) { return lau
 
 // Allocate every buffer a launch of p0 needs (before a graph capture).
 int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
-  kd::PrepBuf buf;
+  kd::PrepBuf buf, buf2;
   int rc = ensure_prep(c, c->order, p0.n_points, &buf);
   if (rc) return rc;
+  if ((rc = ensure_pipeline(c, c->order, p0.n_points, &buf2))) return rc;
   if (c->flow_in_chain && p0.mode != kd::MODE_EVAL_U && c->order > 0 && (rc = ensure_flow_stage(c, c->order)))
     return rc;
   const int grid = grid_for(c, std::min<int64_t>(kChunk, p0.n_points));
@@ -366,12 +407,16 @@ int prepare_launch(kd_ctx *c, const kd::KdParams &p0) {
 int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepBuf *ext) {
   const bool timed = timing != KD_TIMING_NONE;
   const bool kernels = timing == KD_TIMING_KERNELS;
-  kd::PrepBuf buf;
+  kd::PrepBuf bufs[2];
   int rc = KD_OK;
   if (ext)
-    buf = *ext;
-  else if ((rc = ensure_prep(c, c->order, p0.n_points, &buf)))
+    bufs[0] = *ext;
+  else if ((rc = ensure_prep(c, c->order, p0.n_points, &bufs[0])))
     return rc;
+  // several chunks: chunk c's points prepared on stream2 into bufs[c % 2]
+  // while the Keldysh sum of chunk c - 1 runs on the ctx stream
+  const bool pipe = !ext && p0.n_points > kChunk;
+  if (pipe && (rc = ensure_pipeline(c, c->order, p0.n_points, &bufs[1]))) return rc;
   const int N = c->order;
   // The flow warper, when last in the chain, is applied first: its own stage
   // maps l -> x (and the Jacobian) before the rest of the chain.
@@ -386,6 +431,16 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepB
   if (timed) KD_HIP(c, hipEventRecord(c->ev0, c->stream));
   int64_t ci = 0;
   for (int64_t off = 0; off < p0.n_points; off += kChunk, ci++) {
+    // the stream that prepares this chunk's points, and their buffer
+    hipStream_t sp = pipe && ci > 0 ? c->stream2 : c->stream;
+    const kd::PrepBuf &buf = bufs[pipe ? ci & 1 : 0];
+    if (pipe && ci > 0) {
+      // stream2 starts after chunk 0's prep (the flow stage's buffer and any
+      // earlier work on the ctx stream), and reuses a buffer only after the
+      // Keldysh sum two chunks back has read it
+      if (ci == 1) KD_HIP(c, hipStreamWaitEvent(sp, c->ev_sync[0], 0));
+      if (ci >= 2) KD_HIP(c, hipStreamWaitEvent(sp, c->ev_sync[2 * (ci - 2) + 1], 0));
+    }
     kd::KdParams p = p0;
     const int64_t m = std::min<int64_t>(kChunk, p0.n_points - off);
     p.n_points = m;
@@ -406,7 +461,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepB
     } else {
       p.nbins = 0;
     }
-    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], c->stream));
+    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 0], sp));
     if (flow) {
       kd::FlowMapIO io{};
       io.Csr = p.Csr;
@@ -417,7 +472,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepB
       io.out_x = c->d_fx;
       io.out_jac = c->d_fx + (size_t)kChunk * N;
       io.out_l = p.mode == kd::MODE_SAMPLE ? p.out_l : nullptr;
-      KD_HIP(c, kd::launch_flow_map(N, flow_params(c), io, flow_grid(c, m), c->stream));
+      KD_HIP(c, kd::launch_flow_map(N, flow_params(c), io, flow_grid(c, m), sp));
       launches += 1;
       p.in_pts = io.out_x;
       p.in_jac = io.out_jac;
@@ -426,8 +481,16 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepB
     } else if (c->flow_in_chain) {
       p.n_warpers = c->n_warpers - 1; // flow suppressed: the chain without it
     }
-    KD_HIP(c, kd::launch_prep(N, p, buf, grid_prep, c->stream));
-    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], c->stream));
+    KD_HIP(c, kd::launch_prep(N, p, buf, grid_prep, sp));
+    if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 1], sp));
+    if (pipe) {
+      if (ci == 0) {
+        KD_HIP(c, hipEventRecord(c->ev_sync[0], c->stream)); // chunk 0 prepared: stream2 may start
+      } else {
+        KD_HIP(c, hipEventRecord(c->ev_sync[2 * ci], sp));
+        KD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_sync[2 * ci], 0));
+      }
+    }
     if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 2], c->stream));
     if (c->obs != KD_OBS_DIRECT)
       KD_HIP(c, kd::launch_cofactor(N, p, buf, grid, c->stream));
@@ -442,6 +505,8 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepB
     else
       KD_HIP(c, kd::launch_lu(N, c->precision, c->lu_lanes_max, use_pair(c), p, buf, grid, c->stream));
     if (kernels) KD_HIP(c, hipEventRecord(c->ev[4 * ci + 3], c->stream));
+    // this chunk's records read: its buffer may be refilled (chunk ci + 2)
+    if (pipe) KD_HIP(c, hipEventRecord(c->ev_sync[2 * ci + 1], c->stream));
     launches += 2;
     if (p.mode == kd::MODE_RUN && !c->no_reduce) {
       KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->d_acc, c->stream));
@@ -583,7 +648,10 @@ int kd_destroy(kd_ctx *c) {
   dfree(c->d_part);
   dfree(c->d_stage);
   dfree(c->d_prep);
+  dfree(c->d_prep2);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->ev_sync) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -168,9 +168,12 @@ def test_ctree_run_matches_wsplit(flat_g0, order, prec):
     assert np.max(np.abs(va - vb)) <= tol * np.max(np.abs(vb))
 
 
-def test_ctree_cutoff_matches_wsplit(flat_g0):
-    t, order = 10.0, 8
-    u = _points(order, 64, t, 1240)
+@pytest.mark.parametrize("order", [6, 7, 8])
+def test_ctree_cutoff_matches_wsplit(flat_g0, order):
+    """The cutoff on the point's value (two points per wave at orders 6-7:
+    each half sums and cuts its own point)."""
+    t = 10.0
+    u = _points(order, 64, t, 1240 + order)
     ref, _ = O.charge_integrand(U.oracle_charge(flat_g0), t).batch(u)
     a = np.sort(np.abs(ref[np.abs(ref) > 0]))
     cut = float(np.sqrt(a[len(a) // 2] * a[len(a) // 2 + 1]))  # between two values, not on one

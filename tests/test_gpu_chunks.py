@@ -5,7 +5,8 @@ chunk's Keldysh sum runs, against the same points run one chunk per call
 bit-identical, each chunk's partial sums being reduced in the same fixed order
 either way.  Four chunks (the last one partial) reuse both buffers; the cases
 cover the one-point-per-wave tree (order 8), two points per wave with time
-bins (order 6) and the learned-flow stage on the second stream (order 5)."""
+bins (order 6) and the learned-flow stage on the second stream (order 5), as
+separate runs and as one batch captured into a hipGraph."""
 import numpy as np
 import pytest
 
@@ -45,3 +46,25 @@ def test_multi_chunk_run_equals_chunk_by_chunk(flat_g0, case, order):
     if case == "bins":
         (va, ca, xa), (vb, cb, xb) = a.time_bins(), b.time_bins()
         assert np.array_equal(va, vb) and np.array_equal(ca, cb) and xa == xb
+
+
+def test_multi_chunk_batch_equals_separate_runs(flat_g0):
+    """kd_run_batch captures the runs of several contexts into one hipGraph: a
+    context of more than one chunk brings its second stream into the capture
+    (the next chunk's preparation) and joins it back.  Each context's sums,
+    counts and time bins equal those of a separate kd_run bit for bit."""
+    t = 10.0
+    cases = [("plain", 8, 2 * CH + 777), ("bins", 6, 2 * CH + 4096), ("flow", 5, CH + 1)]
+    batch = [_ctx(flat_g0, case, order, t) for case, order, _ in cases]
+    accs, ms = capi.run_batch(batch, [n for *_, n in cases])
+    assert ms > 0
+    for (case, order, n), ctx, a in zip(cases, batch, accs):
+        ref = _ctx(flat_g0, case, order, t)
+        r = ref.run(n)
+        print(f"batch {case}, order {order}: {n} points, in domain {a.n_in_domain}, sum {a.re:.12e} {a.im:.12e}")
+        assert (a.re, a.im, a.n_points, a.n_in_domain, a.n_fallback) == \
+            (r.re, r.im, r.n_points, r.n_in_domain, r.n_fallback)
+        assert a.n_points == n and a.n_in_domain > 0 and ctx.sobol_state() == n
+        if case == "bins":
+            (va, ca, xa), (vb, cb, xb) = ctx.time_bins(), ref.time_bins()
+            assert np.array_equal(va, vb) and np.array_equal(ca, cb) and xa == xb

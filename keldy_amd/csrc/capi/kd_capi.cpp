@@ -241,11 +241,15 @@ int ensure_bin_partials(kd_ctx *c, int grid) {
   return KD_OK;
 }
 
+// per-workgroup partial sums (4 per workgroup), then the reduction's block
+// sums and its ticket, which starts at zero
 int ensure_partials(kd_ctx *c, int grid) {
   if (grid <= c->part_cap) return KD_OK;
   dfree(c->d_part);
   c->part_cap = 0;
-  KD_HIP(c, hipMalloc(&c->d_part, sizeof(double) * 4 * (size_t)grid));
+  const size_t n = kd::partials_doubles(grid);
+  KD_HIP(c, hipMalloc(&c->d_part, sizeof(double) * n));
+  KD_HIP(c, hipMemset(c->d_part + n - 1, 0, sizeof(double)));
   c->part_cap = grid;
   return KD_OK;
 }
@@ -509,7 +513,7 @@ int launch_chunks(kd_ctx *c, const kd::KdParams &p0, int timing, const kd::PrepB
     if (pipe) KD_HIP(c, hipEventRecord(c->ev_sync[2 * ci + 1], c->stream));
     launches += 2;
     if (p.mode == kd::MODE_RUN && !c->no_reduce) {
-      KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->d_acc, c->stream));
+      KD_HIP(c, kd::launch_reduce(c->d_part, grid, c->part_cap, c->d_acc, c->stream));
       launches += 1;
       if (p.nbins > 0) {
         KD_HIP(c, kd::launch_reduce_bins(c->d_bin_part, grid, p.nbins * c->bin_k, c->d_bins, c->stream));

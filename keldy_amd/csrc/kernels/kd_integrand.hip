@@ -588,46 +588,77 @@ __global__ __launch_bounds__(256) void reduce_bins_kernel(const double2 *__restr
   }
 }
 
-// Deterministic sum of per-block partials into acc[0..3] (+=).
-// One workgroup of 1024 threads: thread t sums partials t, t + 1024, ... in
-// that order, four of them loaded at once, then a fixed tree -- the same sum for
-// the same partials on every run.
-__global__ __launch_bounds__(1024) void reduce_partials_kernel(const double *__restrict__ part, int nblk,
-                                                               double *__restrict__ acc) {
-  __shared__ double sh[4][1024];
+// Deterministic sum of per-block partials into acc[0..3] (+=), spread over
+// reduce_blocks(nblk) workgroups of kRedThreads threads.  Block k sums partials
+// [k kRedSlice, (k + 1) kRedSlice): thread t the four at t + j kRedThreads in
+// that order, then a fixed tree; it stores its four sums in slice[4k..4k+3] and
+// takes a ticket.  The block that takes the last ticket adds the slices in block
+// order into acc and re-arms the ticket (zero between launches; launches of one
+// accumulator are stream-ordered).  Which block finishes last decides only who
+// adds, not the order of the additions: the same sum for the same partials on
+// every run.
+__global__ __launch_bounds__(kRedThreads) void reduce_partials_kernel(const double *__restrict__ part, int nblk,
+                                                                      double *slice, unsigned *ticket,
+                                                                      double *acc) {
+  __shared__ double sh[4][kRedThreads];
   const int t = threadIdx.x;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  int i = t;
-  for (; i + 3 * 1024 < nblk; i += 4 * 1024) {
-    double2 v[4][2];
+  const int i0 = (int)blockIdx.x * kRedSlice + t;
+  double2 v[4][2];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      v[u][0] = reinterpret_cast<const double2 *>(part)[2 * (i + u * 1024)];
-      v[u][1] = reinterpret_cast<const double2 *>(part)[2 * (i + u * 1024) + 1];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      s[0] += v[u][0].x;
-      s[1] += v[u][0].y;
-      s[2] += v[u][1].x;
-      s[3] += v[u][1].y;
+  for (int u = 0; u < 4; u++) {
+    const int i = i0 + u * kRedThreads;
+    if (i < nblk) {
+      v[u][0] = reinterpret_cast<const double2 *>(part)[2 * i];
+      v[u][1] = reinterpret_cast<const double2 *>(part)[2 * i + 1];
+    } else {
+      v[u][0] = v[u][1] = make_double2(0.0, 0.0);
     }
   }
-  for (; i < nblk; i += 1024)
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < 4; c++) s[c] += part[4 * i + c];
+  for (int u = 0; u < 4; u++) {
+    s[0] += v[u][0].x;
+    s[1] += v[u][0].y;
+    s[2] += v[u][1].x;
+    s[3] += v[u][1].y;
+  }
 #pragma unroll
   for (int c = 0; c < 4; c++) sh[c][t] = s[c];
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if (t < w)
 #pragma unroll
       for (int c = 0; c < 4; c++) sh[c][t] += sh[c][t + w];
     __syncthreads();
   }
-  if (t == 0)
+  __shared__ int last;
+  if (t == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) acc[c] += sh[c][0];
+    for (int c = 0; c < 4; c++) slice[4 * blockIdx.x + c] = sh[c][0];
+    __threadfence(); // release: this block's slice before its ticket
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence(); // acquire: every block's slice
+  const int G = (int)gridDim.x;
+  if (G <= kRedThreads) {
+    // the slices loaded at once, then added in block order
+    if (t < G)
+#pragma unroll
+      for (int c = 0; c < 4; c++) sh[c][t] = slice[4 * t + c];
+    __syncthreads();
+    if (t < 4) {
+      double a = 0.0;
+      for (int k = 0; k < G; k++) a += sh[t][k];
+      acc[t] += a;
+    }
+  } else if (t < 4) {
+    double a = 0.0;
+    for (int k = 0; k < G; k++) a += slice[4 * k + t];
+    acc[t] += a;
+  }
+  if (t == 0) *ticket = 0u;
 }
 
 template <int N> static hipError_t launch_prep_n(const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st) {
@@ -687,8 +718,12 @@ hipError_t launch_lu(int order, int precision, int lanes, bool pair, const KdPar
 #undef KD_LU64_CALL
 }
 
-hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st) {
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, st, part, nblk, acc);
+hipError_t launch_reduce(double *part, int nblk, int cap, double *acc, hipStream_t st) {
+  if (nblk < 1 || nblk > cap) return hipErrorInvalidValue;
+  double *slice = part + 4 * (size_t)cap;
+  unsigned *ticket = reinterpret_cast<unsigned *>(slice + 4 * (size_t)reduce_blocks(cap));
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(reduce_blocks(nblk)), dim3(kRedThreads), 0, st, part, nblk, slice,
+                     ticket, acc);
   return hipGetLastError();
 }
 

@@ -50,8 +50,14 @@ hipError_t launch_ctree(int order, int precision, const KdParams &p, const PrepB
 // cofactor_kernel<order> (kernel method, orders 1..12, FP64): grid blocks of
 // 64 threads, same work items as the Keldysh-sum kernel.
 hipError_t launch_cofactor(int order, const KdParams &p, const PrepBuf &buf, int grid, hipStream_t st);
-// acc[0..3] += sum over blocks of part[4*blk + 0..3] (fixed order).
-hipError_t launch_reduce(const double *part, int nblk, double *acc, hipStream_t st);
+// acc[0..3] += sum over blocks of part[4*blk + 0..3] (fixed order).  `part`
+// holds partials_doubles(cap) doubles: 4 cap partials, then the reduction's
+// block sums and its ticket, zero before the first launch (the reduction leaves
+// it zero); nblk <= cap.
+constexpr int kRedThreads = 256, kRedSlice = 4 * kRedThreads;
+constexpr int reduce_blocks(int nblk) { return (nblk + kRedSlice - 1) / kRedSlice; }
+constexpr size_t partials_doubles(int cap) { return 4 * (size_t)cap + 4 * (size_t)reduce_blocks(cap) + 1; }
+hipError_t launch_reduce(double *part, int nblk, int cap, double *acc, hipStream_t st);
 // per-(axis, bin) sums of |vals| and counts over n samples l [n][order] (pilot projection)
 hipError_t launch_pilot_hist(const double *l, const double *vals, int64_t n, int order, int nbins, double *sums,
                              unsigned long long *counts, hipStream_t st);

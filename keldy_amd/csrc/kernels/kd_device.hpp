@@ -496,10 +496,15 @@ __device__ inline double map1d_reverse(const MapDev &md, const double *__restric
 }
 
 // N axes of a product of nearest-interpolation maps (map_reverse_k<KD_MAP_NEAREST>
-// on every axis, the same operations): each step runs over all axes before the
-// next starts, so the axes' dependent loads (guide, knot pair, guide, f pair)
-// are in flight together -- four memory round trips per point instead of four
-// per axis.  inv_f[a] = 1/f of axis a.
+// on every axis, the same results): each step runs over all axes before the
+// next starts, so the axes' dependent loads are in flight together.  The f pair
+// of interval i of the inverse-CDF search is loaded with that interval's record:
+// when the mapped time x lies in [t[i], t[i+1]), the search of x over the time
+// knots ends in i (the one interval of a sorted array that holds x), and for a
+// point outside [F[0], F[n_inv-1]] (x NaN) it ends in n_pts - 2.  The guided
+// time search runs only on the lanes where neither holds (x rounded onto
+// t[i+1], a NaN l), so a point costs two memory round trips (guide, record and
+// f pair) instead of four.  inv_f[a] = 1/f of axis a.
 template <int N>
 __device__ __forceinline__ void nearest_axes(const MapDev *__restrict__ ms, bool one, const double *__restrict__ tab,
                                              const int2 *__restrict__ gd, const double *__restrict__ kn,
@@ -515,18 +520,25 @@ __device__ __forceinline__ void nearest_axes(const MapDev *__restrict__ ms, bool
   }
 #pragma unroll
   for (int a = 0; a < N; a++) k[a] = bisect_knots(tab + ms[one ? 0 : a].m.F_off, r[a], y[a]);
-  // the axes' interval records, in groups of four axes whose loads are all
-  // issued before the first use (registers for the records of one group)
+  // the axes' interval records and f pairs, in groups of four axes whose loads
+  // are all issued before the first use (registers for the records of one group)
   constexpr int GA = 4;
+  unsigned slow = 0; // bit a: axis a needs the time search
 #pragma unroll
   for (int a0 = 0; a0 < N; a0 += GA) {
     double4 rec[GA];
-#pragma unroll
-    for (int a = a0; a < a0 + GA && a < N; a++)
-      rec[a - a0] = *reinterpret_cast<const double4 *>(kn + ms[one ? 0 : a].rec + 4 * (int64_t)k[a]); // F[i], F[i+1], t[i], t[i+1]
+    double2 fp[GA];
+    bool out[GA];
 #pragma unroll
     for (int a = a0; a < a0 + GA && a < N; a++) {
       const MapDev &md = ms[one ? 0 : a];
+      out[a - a0] = (y[a] < md.F0) | (y[a] > md.Fl);
+      const int kf = out[a - a0] ? md.m.n_pts - 2 : k[a];
+      rec[a - a0] = *reinterpret_cast<const double4 *>(kn + md.rec + 4 * (int64_t)k[a]); // F[i], F[i+1], t[i], t[i+1]
+      fp[a - a0] = *reinterpret_cast<const double2 *>(kn + md.fpair + 2 * (int64_t)kf);  // f[i], f[i+1]
+    }
+#pragma unroll
+    for (int a = a0; a < a0 + GA && a < N; a++) {
       const double4 r = rec[a - a0];
       const double dx = r.y - r.x;
       // on every lane (inf or NaN, and unused, unless dx > 0): kept out of a
@@ -534,21 +546,22 @@ __device__ __forceinline__ void nearest_axes(const MapDev *__restrict__ ms, bool
       double xa = r.z + (y[a] - r.x) / dx * (r.w - r.z);
       __asm__ volatile("" : "+v"(xa));
       xa = dx > 0.0 ? xa : 0.0;
-      const bool out = (y[a] < md.F0) | (y[a] > md.Fl);
-      x[a] = out ? __builtin_nan("") : xa;
+      x[a] = out[a - a0] ? __builtin_nan("") : xa;
+      slow |= (unsigned)!(out[a - a0] || (r.z <= xa && xa < r.w)) << a;
+      inv_f[a] = 1.0 / (0.5 * (fp[a - a0].x + fp[a - a0].y));
     }
   }
+  if (slow) {
+    // the guided search of x over the time knots, one axis after the other
 #pragma unroll
-  for (int a = 0; a < N; a++) {
-    const MapDev &md = ms[one ? 0 : a];
-    r[a] = guide_bounds(md.m.n_pts, gd, md.gt, x[a]);
-  }
-#pragma unroll
-  for (int a = 0; a < N; a++) k[a] = bisect_knots(tab + ms[one ? 0 : a].m.t_off, r[a], x[a]);
-#pragma unroll
-  for (int a = 0; a < N; a++) {
-    const double2 f = *reinterpret_cast<const double2 *>(kn + ms[one ? 0 : a].fpair + 2 * (int64_t)k[a]); // f[j], f[j+1]
-    inv_f[a] = 1.0 / (0.5 * (f.x + f.y));
+    for (int a = 0; a < N; a++) {
+      if (slow & (1u << a)) {
+        const MapDev &md = ms[one ? 0 : a];
+        const int j = bisect_knots(tab + md.m.t_off, guide_bounds(md.m.n_pts, gd, md.gt, x[a]), x[a]);
+        const double2 f = *reinterpret_cast<const double2 *>(kn + md.fpair + 2 * (int64_t)j);
+        inv_f[a] = 1.0 / (0.5 * (f.x + f.y));
+      }
+    }
   }
 }
 

@@ -12,10 +12,15 @@ the integrand value) both depend on it.
 The knots are adversarial: exact duplicates in time (zero-width intervals),
 a flat stretch of f (duplicated CDF knots), trailing zeros (n_inv < n_pts), a
 cluster of 96 time knots inside one bucket (bisection steps inside the guide's
-bounds).  The points put the inverse-CDF value exactly on every knot, halfway
-between knots, exactly on every bucket edge of the guide and one ulp either
-side, and at l = 0 and 1.  A two-knot map (no guide: plain bisection) runs
-beside them.
+bounds).  The points put the inverse-CDF value exactly on every knot, one ulp
+either side of every knot, halfway between knots, exactly on every bucket edge
+of the guide and one ulp either side, and at l = 0 and 1.  A two-knot map (no
+guide: plain bisection) runs beside them.
+
+The device skips the time search where the interval record of the inverse-CDF
+search already holds the mapped time (t[i] <= x < t[i+1]: the search would end
+in i); a value one ulp below a CDF knot F[i+1] maps onto t[i+1] itself, so
+those points take the time search.
 """
 import numpy as np
 import pytest
@@ -58,16 +63,33 @@ def _nearest(times, f1, tabs):
 
 
 def _edge_points(F):
-    """l such that y = -l lies on the knots, between them, and on the guide's
-    bucket edges (2 n_inv buckets over [F[0], F[-1]]) and one ulp either side."""
+    """l such that y = -l lies on the knots and one ulp either side, between
+    them, and on the guide's bucket edges (2 n_inv buckets over [F[0], F[-1]])
+    and one ulp either side."""
     n = len(F)
     G = 2 * n
     a, width = F[0], (F[-1] - F[0]) / G
-    y = [F, 0.5 * (F[:-1] + F[1:])]
+    # one ulp below a knot F[i+1]: the mapped time rounds onto t[i+1], outside
+    # the interval record [t[i], t[i+1]), so the device's time search runs
+    y = [F, 0.5 * (F[:-1] + F[1:]), np.nextafter(F, -np.inf), np.nextafter(F, np.inf)]
     e = a + np.arange(G + 1) * width
     y += [e, np.nextafter(e, np.inf), np.nextafter(e, -np.inf)]
     l = np.clip(-np.concatenate(y), 0.0, 1.0)
     return np.concatenate([l, [0.0, 1.0, np.nextafter(0.0, 1.0), np.nextafter(1.0, 0.0)]])
+
+
+def _time_search_points(om, l):
+    """Points whose mapped time (FP64, as the device forms it) is not inside the
+    interval record [t[i], t[i+1]) of their inverse-CDF interval i."""
+    t, _, F = om.keep
+    n_inv = om.c.n_inv
+    y = -np.asarray(l)
+    i = np.clip(np.searchsorted(F[:n_inv], y, side="right") - 1, 0, n_inv - 2)
+    dx = F[i + 1] - F[i]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        x = np.where(dx > 0, t[i] + (y - F[i]) / dx * (t[i + 1] - t[i]), 0.0)
+    inside = (y >= F[0]) & (y <= F[n_inv - 1])
+    return int(np.count_nonzero(inside & ~((t[i] <= x) & (x < t[i + 1]))))
 
 
 @pytest.mark.parametrize("case", ["guided", "guided_and_two_knots"])
@@ -82,6 +104,8 @@ def test_nearest_knot_guides_vs_oracle(flat_g0, case):
     om1, m1, F1 = _nearest(t1, f1, tabs)
     assert om0.c.n_inv < om0.c.n_pts
     l0, l1 = _edge_points(F0), _edge_points(F1)
+    n_search = _time_search_points(om0, l0)
+    assert n_search > 0
     rng = np.random.default_rng(3)
     n = max(len(l0), len(l1))
     l = np.stack([rng.permutation(np.resize(l0, n)), rng.permutation(np.resize(l1, n))], axis=1)
@@ -99,7 +123,8 @@ def test_nearest_knot_guides_vs_oracle(flat_g0, case):
     # y = 0 on the last inverse knot maps to the first zero interval of f: 1/f = inf there
     fin = np.isfinite(rjac)
     assert np.array_equal(np.isfinite(jac), fin) and np.count_nonzero(fin) > 0.99 * len(l)
-    print(f"{case}: {len(l)} points ({len(l) - np.count_nonzero(fin)} with 1/f = inf), "
+    print(f"{case}: {len(l)} points ({len(l) - np.count_nonzero(fin)} with 1/f = inf, {n_search} of axis 0's "
+          f"{len(l0)} with the time search), "
           f"max |jac - oracle| / |oracle| {np.max(np.abs(jac[fin] - rjac[fin]) / np.abs(rjac[fin])):.2e}")
     assert np.array_equal(dom, rdom)
     np.testing.assert_allclose(jac[fin], rjac[fin], rtol=1e-14, atol=0)
